@@ -1,0 +1,124 @@
+/*
+ * trench_hip.h -- C-ABI of libtrench_hip.so, the MI355X (gfx950) backend of
+ * trench01's per-batch charging hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes
+ * (TH_OK == 0, negative on error, message in th_last_error()).  No exceptions
+ * cross the boundary.  Every entry point selects the handle's device itself,
+ * so calls may come from any host thread (one call at a time per handle).
+ *
+ * Grid arrays are (nx, ny) C-order float64 / uint8 (cell index ix*ny + iy),
+ * exactly as the reference passes them (trench_charging_2d.py:185-226,
+ * trench_cuda.py:61).  Particle arrays are length-n float64 (x, y, vx, vy, qm)
+ * plus int8 species (0 ion, 1 electron) (trench_charging_2d.py:458-489).
+ */
+#ifndef TRENCH_HIP_H
+#define TRENCH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TH_OK 0
+#define TH_EINVAL (-1)   /* bad argument (shape, range, species, null pointer) */
+#define TH_EHIP (-2)     /* HIP runtime failure */
+#define TH_ENOMEM (-3)   /* device allocation failed */
+#define TH_ESTATE (-4)   /* handle used in the wrong order (e.g. trace before fields) */
+
+typedef struct th_tracer th_tracer;
+
+/* Push options; mirror of the Params fields the reference kernel receives
+ * (trench_cuda.py:107-116). */
+typedef struct th_trace_params {
+    double dx;            /* cell size [m] (Params.dx) */
+    double cfl;           /* 0 < cfl <= 1 (Params.cfl) */
+    int32_t max_steps;    /* >= 1 (Params.max_steps) */
+    int32_t tapered;      /* bool(trench_taper_deg or (mask_t > 0 and mask_taper_deg)) */
+    int32_t barrier;      /* 1: wall_model "barrier" (phi and wface must be set) */
+    int32_t precision;    /* 0: FP64, bit-identical with the reference; 1: FP32 push */
+    int32_t refill;       /* wave refill threshold in lanes, 0 = library default */
+    int32_t chunk;        /* particles per launch when a stop flag is given, 0 = default */
+} th_trace_params;
+
+/* Scalar tallies of one trace call (trench_cuda.py:119-122). */
+typedef struct th_tally {
+    int64_t escaped[2];   /* escaped through the top, per species */
+    int64_t lost;         /* still live after max_steps, or not traced after a stop */
+    int64_t steps;        /* sum over particles of push steps taken */
+    int64_t off_surface;  /* absorbed in a solid cell that is not a surface cell */
+    int32_t cancelled;    /* 1 if the stop flag ended the call early */
+    int32_t reserved;
+} th_tally;
+
+/* Thread-local message of the last failing call on this thread. */
+const char *th_last_error(void);
+
+/* Number of visible HIP devices. */
+int th_device_count(int32_t *count);
+
+/* Replaces CudaTracer.__init__ (trench_cuda.py:24-39): validates the device,
+ * uploads the vacuum map and builds the surface-cell index used by the
+ * LDS-privatised deposit tallies (surface cells as in
+ * trench_charging_2d.py:734-744). */
+int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device, th_tracer **out);
+void th_tracer_destroy(th_tracer *t);
+
+/* Device name (trench_cuda.py:37-39). */
+int th_tracer_device_name(const th_tracer *t, char *buf, int32_t len);
+
+/* Number of surface cells m and their flat indices (ascending), for callers
+ * that keep charge as an m-vector. */
+int th_tracer_surface(const th_tracer *t, int32_t *m, int32_t *flat /* nullable, length m */);
+
+/* Uploads the frozen field of one batch: the field uploads of
+ * trench_cuda.py:61 and :80.  phi and wface may both be NULL (absorb wall
+ * model); otherwise both are required. */
+int th_set_fields(th_tracer *t, const double *Ex, const double *Ey, const double *phi,
+                  const double *wface);
+
+/* One tracer call on host arrays: replaces CudaTracer._trace
+ * (trench_cuda.py:50-138) for the non-record, non-RF case.  Outputs the
+ * per-cell counts cnt_i / cnt_e (int64, nx*ny, overwritten), the scalar tally,
+ * and optionally the per-particle status (1 absorbed, 2 escaped, 3 live) and
+ * absorbed cell (-1 if none).  stop, if non-NULL, is polled between launches;
+ * once it reads non-zero the untraced particles are reported lost. */
+int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+             const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+             const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
+             int8_t *status /* nullable */, int32_t *hit /* nullable */);
+
+/* Device-resident particle pool, so a batch can be traced with its states
+ * already in HBM.  th_particles_reserve sizes the pool (contents are lost);
+ * th_particles_upload copies n host particles to pool[offset, offset+n).
+ * No reference counterpart: the reference uploads states on every call
+ * (trench_cuda.py:60). */
+int th_particles_reserve(th_tracer *t, int64_t capacity);
+int th_particles_upload(th_tracer *t, int64_t offset, const double *x, const double *y, const double *vx,
+                        const double *vy, const double *qm, const int8_t *spc, int64_t n);
+
+/* Traces pool[offset, offset+n) with the fields last set.  Same outputs as
+ * th_trace; cnt_i / cnt_e may be NULL to leave the tallies on the device
+ * only (read them with th_read_counts). */
+int th_trace_resident(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p,
+                      int64_t *cnt_i /* nullable */, int64_t *cnt_e /* nullable */, th_tally *tally,
+                      int8_t *status /* nullable */, int32_t *hit /* nullable */);
+
+/* Per-surface-cell counts of the last trace: counts[0..m) ions, [m..2m) electrons. */
+int th_read_counts(th_tracer *t, int32_t *counts);
+
+/* GPU time of the push launches of the last trace call, from HIP events
+ * recorded on the library's stream around them [ms]. */
+int th_last_push_ms(const th_tracer *t, double *ms);
+
+/* Device self-check of the FP64 arithmetic the push relies on: for each k,
+ * out[4k..4k+3] = {hypot(a,b), sqrt(a), a/b, fmod(a,b)} computed on the GPU
+ * with the kernels' own routines.  No reference counterpart; used by tests to
+ * pin the device arithmetic against the host C library. */
+int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

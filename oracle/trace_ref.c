@@ -1,0 +1,182 @@
+/*
+ * ORACLE -- test infrastructure only.  CPU restatement of trench01's FP64
+ * particle push, used by tests, smoke() and the bench's cpu_baseline leg as
+ * the checker.  The product (trench01_amd) never links or calls this file.
+ *
+ * Restates, in one scalar loop per particle, the algorithm of
+ *   trench_trace.cu:17-39     interpolate  (vacuum-weighted bilinear field)
+ *   trench_trace.cu:41-59     wall_entry   (first solid cell crossed in a step)
+ *   trench_trace.cu:92-185    push / classify / barrier / absorb loop
+ *   trench_charging_2d.py:492-509, :512-539, :542-702  (trace_cpu, same algorithm
+ *                              vectorised; its NumPy operation order is what
+ *                              the FP64 arithmetic below reproduces)
+ *   trench_cuda.py:118-122    tally conventions (bincount per species)
+ *
+ * Arithmetic contract: every expression is evaluated left to right exactly as
+ * the reference writes it, with contraction off (-ffp-contract=off), and hypot
+ * is glibc's hypot -- the function NumPy's np.hypot calls -- so results are
+ * bit-identical with trace_cpu on identical inputs.
+ *
+ * RF clocks (trench_trace.cu:97-110) and path recording are not restated here.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline int wrap_i(int i, int n) { int r = i % n; return r < 0 ? r + n : r; }
+static inline int clip_i(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+static inline double wrap_x(double a, double len) {
+    double r = fmod(a, len);          /* NumPy float %: fmod, then shift negatives */
+    return r < 0.0 ? r + len : r;
+}
+
+typedef struct {
+    const double *ex, *ey, *phi, *wface;
+    const uint8_t *vac;
+    int nx, ny, tapered, barrier;
+    double dx, cfl, lx, ly;
+} grid_t;
+
+/* trench_charging_2d.py:492-509 / trench_trace.cu:17-39 */
+static void field_at(const grid_t *g, double x, double y, double *ox, double *oy) {
+    double fx = x / g->dx - 0.5, fy = y / g->dx - 0.5;
+    int i = (int)floor(fx), j = (int)floor(fy);
+    double tx = fx - i, ty = fy - j;
+    int i0 = wrap_i(i, g->nx), i1 = wrap_i(i + 1, g->nx);
+    int j0 = clip_i(j, g->ny), j1 = clip_i(j + 1, g->ny);
+    int c[4] = {i0 * g->ny + j0, i1 * g->ny + j0, i0 * g->ny + j1, i1 * g->ny + j1};
+    double w[4] = {(1.0 - tx) * (1.0 - ty), tx * (1.0 - ty), (1.0 - tx) * ty, tx * ty};
+    double ax = 0.0, ay = 0.0, aw = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        int v = g->vac[c[k]] != 0;
+        ax += w[k] * (v ? g->ex[c[k]] : 0.0);
+        ay += w[k] * (v ? g->ey[c[k]] : 0.0);
+        aw += w[k] * (double)v;
+    }
+    if (aw < 1e-12) aw = 1e-12;
+    *ox = ax / aw;
+    *oy = ay / aw;
+}
+
+/* trench_charging_2d.py:512-539 / trench_trace.cu:41-59 */
+static void first_solid(const grid_t *g, double x0, double y0, double sx, double sy,
+                        int *cx, int *cy, int *vx_, int *vy_, int *face_x) {
+    int nx = g->nx, ny = g->ny;
+    int ix0 = (int)(x0 / g->dx); if (ix0 > nx - 1) ix0 = nx - 1;
+    int iy0 = clip_i((int)(y0 / g->dx), ny);
+    int dix = (int)floor((x0 + sx) / g->dx) - ix0;
+    int diy = clip_i((int)floor((y0 + sy) / g->dx), ny) - iy0;
+    double tfx = dix != 0 ? ((ix0 + (dix > 0)) * g->dx - x0) / sx : INFINITY;
+    double tfy = diy != 0 ? ((iy0 + (diy > 0)) * g->dx - y0) / sy : INFINITY;
+    int x_first = tfx < tfy;
+    int ax = x_first ? wrap_i(ix0 + dix, nx) : ix0;
+    int ay = x_first ? iy0 : iy0 + diy;
+    int last = dix != 0 && diy != 0 && g->vac[ax * ny + ay];
+    *cx = last ? wrap_i(ix0 + dix, nx) : ax;
+    *cy = last ? iy0 + diy : ay;
+    *vx_ = last ? ax : ix0;
+    *vy_ = last ? ay : iy0;
+    *face_x = last ? !x_first : x_first;
+}
+
+/* One particle to termination.  Returns the status (1 absorbed, 2 escaped,
+ * 3 still live after max_steps) and writes the hit cell and step count. */
+static int push_one(const grid_t *g, int max_steps, double x, double y, double vx, double vy,
+                    double qm, int32_t *hit, int64_t *nsteps) {
+    const double dx = g->dx;
+    const int nx = g->nx, ny = g->ny;
+    for (int k = 0; k < max_steps; ++k) {
+        double ex, ey;
+        field_at(g, x, y, &ex, &ey);
+        double ax = qm * ex, ay = qm * ey;
+        double dt = g->cfl * dx / (hypot(vx, vy) + sqrt(2.0 * dx * hypot(ax, ay)) + 1.0);
+        double vxn = vx + ax * dt, vyn = vy + ay * dt;
+        double x0 = x, y0 = y;
+        double sx = 0.5 * (vx + vxn) * dt, sy = 0.5 * (vy + vyn) * dt;
+        x = wrap_x(x + sx, g->lx);
+        y = y + sy;
+        vx = vxn; vy = vyn;
+        int escaped = y >= g->ly;
+        int ix = (int)(x / dx); if (ix > nx - 1) ix = nx - 1;
+        int iy = clip_i((int)(y / dx), ny);
+        int hitting = !escaped && !g->vac[ix * ny + iy];
+        int corner = 0;
+        if (g->tapered && !hitting && !escaped) {
+            int ix0 = (int)(x0 / dx); if (ix0 > nx - 1) ix0 = nx - 1;
+            corner = ix != ix0 && iy != clip_i((int)(y0 / dx), ny);
+        }
+        if ((hitting && (g->barrier || g->tapered)) || corner) {
+            int cx, cy, vcx, vcy, face_x;
+            first_solid(g, x0, y0, sx, sy, &cx, &cy, &vcx, &vcy, &face_x);
+            hitting = !g->vac[cx * ny + cy];
+            if (hitting && g->barrier) {
+                double vn = face_x ? vx : vy;
+                int c = cx * ny + cy, v = vcx * ny + vcy;
+                double dphi = g->wface[c] * (g->phi[c] - g->phi[v]);
+                if (0.5 * vn * vn < qm * dphi) {          /* cannot climb the wall barrier */
+                    if (face_x) vx = -vx; else vy = -vy;
+                    x = x0; y = y0;
+                    hitting = 0;
+                }
+            }
+            ix = cx; iy = cy;
+        }
+        if (hitting) { *hit = ix * ny + iy; *nsteps = k + 1; return 1; }
+        if (escaped) { *nsteps = k + 1; return 2; }
+    }
+    *nsteps = max_steps;
+    return 3;
+}
+
+/*
+ * Trace n particles.  Outputs: status[n] (int8), hit[n] (int32, -1 unless
+ * absorbed), optional steps[n]; tallies cnt_i/cnt_e[nx*ny] (int64, zeroed
+ * here), escaped[2], *lost and *total_steps.  nthreads <= 0 uses OpenMP's
+ * default.  Returns 0, or -1 on a species value other than 0/1.
+ */
+int oracle_trace(const double *x, const double *y, const double *vx, const double *vy,
+                 const double *qm, const int8_t *spc, int64_t n,
+                 const double *ex, const double *ey, const uint8_t *vac, int nx, int ny,
+                 const double *phi, const double *wface,
+                 double dx, double cfl, int max_steps, int tapered, int nthreads,
+                 int8_t *status, int32_t *hit, int64_t *steps,
+                 int64_t *cnt_i, int64_t *cnt_e, int64_t *escaped, int64_t *lost,
+                 int64_t *total_steps) {
+    for (int64_t k = 0; k < n; ++k)
+        if (spc[k] != 0 && spc[k] != 1) return -1;
+    grid_t g = {ex, ey, phi, wface, vac, nx, ny, tapered, phi != NULL && wface != NULL,
+                dx, cfl, nx * dx, ny * dx};
+    int64_t tot = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tot)
+#endif
+    for (int64_t k = 0; k < n; ++k) {
+        int32_t h = -1;
+        int64_t s = 0;
+        status[k] = (int8_t)push_one(&g, max_steps, x[k], y[k], vx[k], vy[k], qm[k], &h, &s);
+        hit[k] = h;
+        if (steps) steps[k] = s;
+        tot += s;
+    }
+    memset(cnt_i, 0, sizeof(int64_t) * (size_t)nx * ny);
+    memset(cnt_e, 0, sizeof(int64_t) * (size_t)nx * ny);
+    escaped[0] = escaped[1] = 0;
+    *lost = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        if (status[k] == 1) (spc[k] ? cnt_e : cnt_i)[hit[k]] += 1;
+        else if (status[k] == 2) escaped[spc[k]] += 1;
+        else *lost += 1;
+    }
+    *total_steps = tot;
+    return 0;
+}
+
+/* glibc hypot on arrays, for tests that pin the device hypot against it. */
+void oracle_hypot(const double *a, const double *b, double *out, int64_t n) {
+    for (int64_t k = 0; k < n; ++k) out[k] = hypot(a[k], b[k]);
+}

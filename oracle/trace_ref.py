@@ -1,0 +1,110 @@
+"""ORACLE -- test infrastructure only (tests, smoke() and the bench's cpu_baseline
+leg).  ctypes binding of ``trace_ref.c``, the C restatement of trench01's FP64
+particle push (trench_trace.cu:17-193, trench_charging_2d.py:542-702).
+
+``oracle_tracer`` has the reference tracer signature
+``tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None,
+stop=None, rf=None) -> (cnt_i, cnt_e, escaped, lost)`` (trench_charging_2d.py:542,
+:702).  Record mode and RF clocks are not restated (``NotImplementedError``).
+"""
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle_trace.so"
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build(force=False):
+    """Compile trace_ref.c with contraction off (the reference's --fmad=false)."""
+    if LIB.exists() and not force:
+        return LIB
+    tmp = LIB.with_suffix(".so.tmp%d" % os.getpid())
+    subprocess.run(["gcc", "-O2", "-fno-fast-math", "-ffp-contract=off", "-fopenmp", "-shared",
+                    "-fPIC", str(HERE / "trace_ref.c"), "-o", str(tmp), "-lm"], check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        L.oracle_trace.restype = ctypes.c_int
+        L.oracle_trace.argtypes = [_dp, _dp, _dp, _dp, _dp, _i8p, ctypes.c_int64,
+                                   _dp, _dp, _u8p, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, _i8p, _i32p, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p]
+        L.oracle_hypot.restype = None
+        L.oracle_hypot.argtypes = [_dp, _dp, _dp, ctypes.c_int64]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def is_tapered(p):
+    """trench_charging_2d.py:554 / trench_cuda.py:113."""
+    return bool(p.trench_taper_deg or (p.mask_t > 0 and p.mask_taper_deg))
+
+
+def trace_arrays(x, y, vx, vy, qm, spc, Ex, Ey, vac, dx, cfl, max_steps, tapered,
+                 barrier=None, threads=0):
+    """Run the C push.  Returns dict(status, hit, steps, cnt_i, cnt_e, escaped, lost, total_steps)."""
+    vac = np.ascontiguousarray(vac, dtype=np.uint8)
+    nx, ny = vac.shape
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    x, y, vx, vy, qm = (f64(a) for a in (x, y, vx, vy, qm))
+    spc = np.ascontiguousarray(spc, dtype=np.int8)
+    n = x.size
+    Ex, Ey = f64(Ex), f64(Ey)
+    phi = wface = None
+    if barrier is not None:
+        phi, wface = (f64(a) for a in barrier)
+    out = dict(status=np.empty(n, np.int8), hit=np.empty(n, np.int32), steps=np.empty(n, np.int64),
+               cnt_i=np.empty((nx, ny), np.int64), cnt_e=np.empty((nx, ny), np.int64),
+               escaped=np.zeros(2, np.int64))
+    lost = ctypes.c_int64(0)
+    total = ctypes.c_int64(0)
+    rc = lib().oracle_trace(_ptr(x, _dp), _ptr(y, _dp), _ptr(vx, _dp), _ptr(vy, _dp), _ptr(qm, _dp),
+                            _ptr(spc, _i8p), n, _ptr(Ex, _dp), _ptr(Ey, _dp), _ptr(vac, _u8p),
+                            nx, ny, _ptr(phi, _dp), _ptr(wface, _dp), float(dx), float(cfl),
+                            int(max_steps), int(bool(tapered)), int(threads),
+                            _ptr(out["status"], _i8p), _ptr(out["hit"], _i32p), _ptr(out["steps"], _i64p),
+                            _ptr(out["cnt_i"], _i64p), _ptr(out["cnt_e"], _i64p),
+                            _ptr(out["escaped"], _i64p), ctypes.byref(lost), ctypes.byref(total))
+    if rc != 0:
+        raise ValueError("species must be 0 (ion) or 1 (electron)")
+    out["lost"] = int(lost.value)
+    out["total_steps"] = int(total.value)
+    return out
+
+
+def oracle_tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None, stop=None, rf=None):
+    """Reference tracer API (trench_charging_2d.py:542) over the C push."""
+    if record is not None or rf is not None:
+        raise NotImplementedError("oracle push restates the non-record, non-RF path only")
+    r = trace_arrays(x, y, vx, vy, qm, spc, Ex, Ey, vac, p.dx, p.cfl, p.max_steps, is_tapered(p), barrier)
+    return r["cnt_i"], r["cnt_e"], r["escaped"], r["lost"]
+
+
+def glibc_hypot(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty_like(a)
+    lib().oracle_hypot(_ptr(a, _dp), _ptr(b, _dp), _ptr(out, _dp), a.size)
+    return out

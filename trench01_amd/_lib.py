@@ -1,0 +1,93 @@
+"""ctypes binding of libtrench_hip.so (C-ABI declared in include/trench_hip.h).
+
+Loading fails loudly: there is no CPU fallback in this package.
+"""
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtrench_hip.so"
+
+dp = ctypes.POINTER(ctypes.c_double)
+i8p = ctypes.POINTER(ctypes.c_int8)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+i32p = ctypes.POINTER(ctypes.c_int32)
+i64p = ctypes.POINTER(ctypes.c_int64)
+vp = ctypes.c_void_p
+
+
+class TraceParams(ctypes.Structure):
+    """th_trace_params."""
+    _fields_ = [("dx", ctypes.c_double), ("cfl", ctypes.c_double), ("max_steps", ctypes.c_int32),
+                ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32)]
+
+
+class Tally(ctypes.Structure):
+    """th_tally."""
+    _fields_ = [("escaped", ctypes.c_int64 * 2), ("lost", ctypes.c_int64), ("steps", ctypes.c_int64),
+                ("off_surface", ctypes.c_int64), ("cancelled", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# name -> (restype, argtypes); every symbol include/trench_hip.h declares.
+SIGNATURES = {
+    "th_last_error": (ctypes.c_char_p, []),
+    "th_device_count": (ctypes.c_int, [i32p]),
+    "th_tracer_create": (ctypes.c_int, [u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(vp)]),
+    "th_tracer_destroy": (None, [vp]),
+    "th_tracer_device_name": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int32]),
+    "th_tracer_surface": (ctypes.c_int, [vp, i32p, i32p]),
+    "th_set_fields": (ctypes.c_int, [vp, dp, dp, dp, dp]),
+    "th_trace": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                i32p, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_particles_reserve": (ctypes.c_int, [vp, ctypes.c_int64]),
+    "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
+    "th_trace_resident": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), i64p,
+                                         i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_read_counts": (ctypes.c_int, [vp, i32p]),
+    "th_last_push_ms": (ctypes.c_int, [vp, dp]),
+    "th_selftest_math": (ctypes.c_int, [vp, dp, dp, ctypes.c_int64, dp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (once).  Raises RuntimeError if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH.name} is not built; run __graft_entry__.build() "
+                                   "or python -m trench01_amd.build")
+            L = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().th_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise HipError(f"libtrench_hip error {rc}: {msg}")
+
+
+def ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def device_count():
+    c = ctypes.c_int32(0)
+    check(load().th_device_count(ctypes.byref(c)))
+    return int(c.value)

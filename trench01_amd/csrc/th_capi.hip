@@ -1,0 +1,470 @@
+// C-ABI of libtrench_hip.so (declarations and contracts in include/trench_hip.h).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/trench_hip.h"
+#include "th_push.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define TH_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? TH_ENOMEM : TH_EHIP, "%s: %s (%s:%d)",    \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+template <class T>
+int dev_alloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    TH_HIP(hipMalloc((void **)p, count * sizeof(T)));
+    return TH_OK;
+}
+
+}  // namespace
+
+struct th_tracer {
+    int device = 0, nx = 0, ny = 0, N = 0, m = 0, cus = 0;
+    hipStream_t stream = nullptr;
+    std::vector<uint8_t> vac;
+    std::vector<int32_t> surf;   // flat indices of surface cells, ascending
+    uint8_t *d_vac = nullptr;
+    int32_t *d_smap = nullptr;
+    double4 *d_cells = nullptr;
+    float4 *d_cells32 = nullptr;
+    double *d_wface = nullptr, *d_tmp = nullptr;   // d_tmp: 3*N upload staging
+    bool fields = false, barrier_fields = false;
+    // particle buffers
+    int64_t cap = 0;
+    double *d_x = nullptr, *d_y = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_qm = nullptr;
+    int8_t *d_spc = nullptr, *d_status = nullptr;
+    int32_t *d_hit = nullptr;
+    // tallies
+    uint32_t *d_counts = nullptr, *d_overflow = nullptr;
+    unsigned long long *d_scal = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the push launches
+    double push_ms = 0.0;
+    char name[256] = "";
+};
+
+namespace {
+
+// Surface cells: solid cells with a vacuum cell among their 8 neighbours,
+// x periodic, outside the y range counts as solid (trench_charging_2d.py:734-744).
+void surface_cells(const std::vector<uint8_t> &vac, int nx, int ny, std::vector<int32_t> &out) {
+    out.clear();
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy) {
+            if (vac[ix * ny + iy]) continue;
+            bool near = false;
+            for (int sx = -1; sx <= 1 && !near; ++sx)
+                for (int sy = -1; sy <= 1 && !near; ++sy) {
+                    if (!sx && !sy) continue;
+                    int jy = iy + sy;
+                    if (jy < 0 || jy >= ny) continue;
+                    int jx = ((ix + sx) % nx + nx) % nx;
+                    near = vac[jx * ny + jy] != 0;
+                }
+            if (near) out.push_back(ix * ny + iy);
+        }
+}
+
+void free_particles(th_tracer *t) {
+    for (void *p : {(void *)t->d_x, (void *)t->d_y, (void *)t->d_vx, (void *)t->d_vy, (void *)t->d_qm,
+                    (void *)t->d_spc, (void *)t->d_status, (void *)t->d_hit})
+        if (p) (void)hipFree(p);
+    t->d_x = t->d_y = t->d_vx = t->d_vy = t->d_qm = nullptr;
+    t->d_spc = t->d_status = nullptr;
+    t->d_hit = nullptr;
+    t->cap = 0;
+}
+
+int reserve_particles(th_tracer *t, int64_t n) {
+    if (n <= t->cap && n > 0) return TH_OK;
+    free_particles(t);
+    int64_t cap = std::max<int64_t>(n, 1024);
+    int rc;
+    if ((rc = dev_alloc(&t->d_x, cap)) || (rc = dev_alloc(&t->d_y, cap)) || (rc = dev_alloc(&t->d_vx, cap)) ||
+        (rc = dev_alloc(&t->d_vy, cap)) || (rc = dev_alloc(&t->d_qm, cap)) || (rc = dev_alloc(&t->d_spc, cap)) ||
+        (rc = dev_alloc(&t->d_status, cap)) || (rc = dev_alloc(&t->d_hit, cap))) {
+        free_particles(t);
+        return rc;
+    }
+    t->cap = cap;
+    return TH_OK;
+}
+
+th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
+    th::Grid g;
+    g.cells = t->d_cells;
+    g.cells32 = t->d_cells32;
+    g.wface = t->d_wface;
+    g.vac = t->d_vac;
+    g.smap = t->d_smap;
+    g.nx = t->nx;
+    g.ny = t->ny;
+    g.m = t->m;
+    g.dx = p->dx;
+    g.cfl_dx = p->cfl * p->dx;
+    g.two_dx = 2.0 * p->dx;
+    g.lx = t->nx * p->dx;
+    g.ly = t->ny * p->dx;
+    return g;
+}
+
+// Persistent launch: as many workgroups as can be resident at once (the
+// refill loop keeps them busy), never more than the particles need.
+template <bool B, bool T>
+int launch64(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds) {
+    int per_cu = 0;
+    TH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, th::push64_kernel<B, T>, th::kBlock, lds));
+    if (per_cu < 1) per_cu = 1;
+    int blocks = (int)std::min<int64_t>(want, (int64_t)t->cus * per_cu);
+    hipLaunchKernelGGL((th::push64_kernel<B, T>), dim3(blocks), dim3(th::kBlock), lds, t->stream, g, w);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+int launch_push(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered) {
+    int64_t n = w.end - w.begin;
+    if (n <= 0) return TH_OK;
+    size_t lds = sizeof(uint32_t) * 2 * (size_t)t->m;
+    int64_t want = (n + th::kBlock - 1) / th::kBlock;
+    if (barrier)
+        return tapered ? launch64<true, true>(t, g, w, want, lds) : launch64<true, false>(t, g, w, want, lds);
+    return tapered ? launch64<false, true>(t, g, w, want, lds) : launch64<false, false>(t, g, w, want, lds);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *th_last_error(void) { return g_err; }
+
+int th_device_count(int32_t *count) {
+    if (!count) return fail(TH_EINVAL, "count is NULL");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return TH_OK;
+}
+
+int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device, th_tracer **out) {
+    if (!out) return fail(TH_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!vac || nx < 1 || ny < 1) return fail(TH_EINVAL, "invalid vacuum map %dx%d", nx, ny);
+    if ((int64_t)nx * ny > (1 << 30)) return fail(TH_EINVAL, "grid too large");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+    if (device < 0 || device >= count)
+        return fail(TH_EINVAL, "HIP device %d is not available (%d visible)", device, count);
+    TH_HIP(hipSetDevice(device));
+    th_tracer *t = new th_tracer();
+    t->device = device;
+    t->nx = nx;
+    t->ny = ny;
+    t->N = nx * ny;
+    t->vac.assign(vac, vac + (size_t)t->N);
+    for (auto &v : t->vac) v = v ? 1 : 0;
+    surface_cells(t->vac, nx, ny, t->surf);
+    t->m = (int)t->surf.size();
+    std::vector<int32_t> smap((size_t)t->N, -1);
+    for (int k = 0; k < t->m; ++k) smap[t->surf[k]] = k;
+    hipDeviceProp_t prop;
+    int rc = TH_OK;
+    auto bad = [&](int code) { th_tracer_destroy(t); return code; };
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bad(fail(TH_EHIP, "hipGetDeviceProperties failed"));
+    snprintf(t->name, sizeof t->name, "%s", prop.name);
+    t->cus = prop.multiProcessorCount;
+    if (2 * (size_t)t->m * sizeof(uint32_t) > 96 * 1024)
+        return bad(fail(TH_EINVAL, "%d surface cells exceed the LDS tally capacity", t->m));
+    if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(TH_EHIP, "hipStreamCreate failed"));
+    if (hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess)
+        return bad(fail(TH_EHIP, "hipEventCreate failed"));
+    if ((rc = dev_alloc(&t->d_vac, t->N)) || (rc = dev_alloc(&t->d_smap, t->N)) ||
+        (rc = dev_alloc(&t->d_cells, t->N)) || (rc = dev_alloc(&t->d_cells32, t->N)) ||
+        (rc = dev_alloc(&t->d_wface, t->N)) || (rc = dev_alloc(&t->d_tmp, 3 * (size_t)t->N)) ||
+        (rc = dev_alloc(&t->d_counts, 2 * (size_t)std::max(t->m, 1))) ||
+        (rc = dev_alloc(&t->d_overflow, 2 * (size_t)t->N)) || (rc = dev_alloc(&t->d_scal, th::kScalCount)))
+        return bad(rc);
+    if (hipMemcpy(t->d_vac, t->vac.data(), t->N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t->d_smap, smap.data(), sizeof(int32_t) * t->N, hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(TH_EHIP, "upload of the vacuum map failed"));
+    *out = t;
+    return TH_OK;
+}
+
+void th_tracer_destroy(th_tracer *t) {
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    free_particles(t);
+    for (void *p : {(void *)t->d_vac, (void *)t->d_smap, (void *)t->d_cells, (void *)t->d_cells32,
+                    (void *)t->d_wface, (void *)t->d_tmp, (void *)t->d_counts, (void *)t->d_overflow,
+                    (void *)t->d_scal})
+        if (p) (void)hipFree(p);
+    if (t->ev0) (void)hipEventDestroy(t->ev0);
+    if (t->ev1) (void)hipEventDestroy(t->ev1);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+int th_tracer_device_name(const th_tracer *t, char *buf, int32_t len) {
+    if (!t || !buf || len < 1) return fail(TH_EINVAL, "invalid argument");
+    snprintf(buf, (size_t)len, "%s", t->name);
+    return TH_OK;
+}
+
+int th_tracer_surface(const th_tracer *t, int32_t *m, int32_t *flat) {
+    if (!t || !m) return fail(TH_EINVAL, "invalid argument");
+    *m = t->m;
+    if (flat) memcpy(flat, t->surf.data(), sizeof(int32_t) * (size_t)t->m);
+    return TH_OK;
+}
+
+int th_set_fields(th_tracer *t, const double *Ex, const double *Ey, const double *phi, const double *wface) {
+    if (!t || !Ex || !Ey) return fail(TH_EINVAL, "Ex and Ey are required");
+    if ((phi == nullptr) != (wface == nullptr)) return fail(TH_EINVAL, "phi and wface go together");
+    TH_HIP(hipSetDevice(t->device));
+    size_t bytes = sizeof(double) * (size_t)t->N;
+    TH_HIP(hipMemcpyAsync(t->d_tmp, Ex, bytes, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_tmp + t->N, Ey, bytes, hipMemcpyHostToDevice, t->stream));
+    if (phi) {
+        TH_HIP(hipMemcpyAsync(t->d_tmp + 2 * (size_t)t->N, phi, bytes, hipMemcpyHostToDevice, t->stream));
+        TH_HIP(hipMemcpyAsync(t->d_wface, wface, bytes, hipMemcpyHostToDevice, t->stream));
+    }
+    int threads = 256, blocks = (t->N + threads - 1) / threads;
+    // FP32 records hold E/dx so the FP32 push works in cell units; dx is not
+    // known here, so they are scaled when the push is launched (1.0 for now).
+    hipLaunchKernelGGL(th::pack_cells_kernel, dim3(blocks), dim3(threads), 0, t->stream, t->d_tmp,
+                       t->d_tmp + t->N, phi ? t->d_tmp + 2 * (size_t)t->N : nullptr, t->d_vac, t->N, 1.0,
+                       t->d_cells, t->d_cells32);
+    TH_HIP(hipGetLastError());
+    TH_HIP(hipStreamSynchronize(t->stream));
+    t->fields = true;
+    t->barrier_fields = phi != nullptr;
+    return TH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_params(const th_tracer *t, const th_trace_params *p) {
+    if (!(p->cfl > 0.0 && p->cfl <= 1.0) || p->max_steps < 1)
+        return fail(TH_EINVAL, "tracing needs 0 < cfl <= 1 and max_steps >= 1");
+    if (!(p->dx > 0.0)) return fail(TH_EINVAL, "dx must be positive");
+    if (p->precision != 0) return fail(TH_EINVAL, "precision %d is not available", p->precision);
+    if (!t->fields) return fail(TH_ESTATE, "th_set_fields must precede tracing");
+    if (p->barrier && !t->barrier_fields) return fail(TH_ESTATE, "barrier wall model needs phi and wface");
+    return TH_OK;
+}
+
+int upload(th_tracer *t, int64_t off, const double *x, const double *y, const double *vx, const double *vy,
+           const double *qm, const int8_t *spc, int64_t n) {
+    for (int64_t k = 0; k < n; ++k)
+        if (spc[k] != 0 && spc[k] != 1) return fail(TH_EINVAL, "species must be 0 (ion) or 1 (electron)");
+    size_t b8 = sizeof(double) * (size_t)n;
+    TH_HIP(hipMemcpyAsync(t->d_x + off, x, b8, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_y + off, y, b8, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_vx + off, vx, b8, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_vy + off, vy, b8, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_qm + off, qm, b8, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipMemcpyAsync(t->d_spc + off, spc, (size_t)n, hipMemcpyHostToDevice, t->stream));
+    return TH_OK;
+}
+
+// Traces pool[off, off+n) and collects the tallies (see th_trace).
+int trace_range(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
+                int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status, int32_t *hit) {
+    memset(tally, 0, sizeof *tally);
+    if (cnt_i) memset(cnt_i, 0, sizeof(int64_t) * (size_t)t->N);
+    if (cnt_e) memset(cnt_e, 0, sizeof(int64_t) * (size_t)t->N);
+    t->push_ms = 0.0;
+    if (n == 0) {
+        TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
+        TH_HIP(hipStreamSynchronize(t->stream));
+        return TH_OK;
+    }
+    TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
+    TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
+    TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
+    if (status) TH_HIP(hipMemsetAsync(t->d_status + off, 3, (size_t)n, t->stream));
+    if (hit) TH_HIP(hipMemsetAsync(t->d_hit + off, 0xff, sizeof(int32_t) * (size_t)n, t->stream));
+
+    th::Grid g = make_grid(t, p);
+    th::Work w;
+    w.x = t->d_x; w.y = t->d_y; w.vx = t->d_vx; w.vy = t->d_vy; w.qm = t->d_qm; w.spc = t->d_spc;
+    w.status = status ? t->d_status : nullptr;
+    w.hit = hit ? t->d_hit : nullptr;
+    w.counts = t->d_counts;
+    w.overflow = t->d_overflow;
+    w.scal = t->d_scal;
+    w.max_steps = p->max_steps;
+    w.refill = p->refill > 0 ? std::min(p->refill, 64) : 16;
+    int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
+    int64_t traced = off;
+    int rc;
+    TH_HIP(hipEventRecord(t->ev0, t->stream));
+    for (int64_t b = off; b < off + n; b += chunk) {
+        if (stop && *stop) {
+            tally->cancelled = 1;
+            break;
+        }
+        w.begin = b;
+        w.end = std::min(off + n, b + chunk);
+        TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
+        if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0))) return rc;
+        if (stop) TH_HIP(hipStreamSynchronize(t->stream));
+        traced = w.end;
+    }
+    TH_HIP(hipEventRecord(t->ev1, t->stream));
+    std::vector<uint32_t> counts(2 * (size_t)std::max(t->m, 1));
+    unsigned long long scal[th::kScalCount];
+    TH_HIP(hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream));
+    if (cnt_i || cnt_e)
+        TH_HIP(hipMemcpyAsync(counts.data(), t->d_counts, sizeof(uint32_t) * counts.size(),
+                              hipMemcpyDeviceToHost, t->stream));
+    if (status) TH_HIP(hipMemcpyAsync(status, t->d_status + off, (size_t)n, hipMemcpyDeviceToHost, t->stream));
+    if (hit)
+        TH_HIP(hipMemcpyAsync(hit, t->d_hit + off, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    float ms = 0.0f;
+    TH_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    t->push_ms = ms;
+    if (cnt_i && cnt_e) {
+        for (int k = 0; k < t->m; ++k) {
+            cnt_i[t->surf[k]] += counts[k];
+            cnt_e[t->surf[k]] += counts[t->m + k];
+        }
+        if (scal[th::kScalOff]) {
+            std::vector<uint32_t> over(2 * (size_t)t->N);
+            TH_HIP(hipMemcpy(over.data(), t->d_overflow, sizeof(uint32_t) * over.size(), hipMemcpyDeviceToHost));
+            for (int c = 0; c < t->N; ++c) {
+                cnt_i[c] += over[c];
+                cnt_e[c] += over[t->N + c];
+            }
+        }
+    }
+    tally->escaped[0] = (int64_t)scal[th::kScalEsc0];
+    tally->escaped[1] = (int64_t)scal[th::kScalEsc1];
+    tally->lost = (int64_t)scal[th::kScalLost] + (off + n - traced);
+    tally->steps = (int64_t)scal[th::kScalSteps];
+    tally->off_surface = (int64_t)scal[th::kScalOff];
+    return TH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+             const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+             const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
+             int32_t *hit) {
+    if (!t || !p || !cnt_i || !cnt_e || !tally) return fail(TH_EINVAL, "invalid argument");
+    if (n < 0) return fail(TH_EINVAL, "negative particle count");
+    if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; ++k)
+        if (spc[k] != 0 && spc[k] != 1) return fail(TH_EINVAL, "species must be 0 (ion) or 1 (electron)");
+    if (status)
+        for (int64_t k = 0; k < n; ++k) status[k] = 3;
+    if (hit)
+        for (int64_t k = 0; k < n; ++k) hit[k] = -1;
+    TH_HIP(hipSetDevice(t->device));
+    if (n > 0) {
+        if ((rc = reserve_particles(t, n))) return rc;
+        if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
+    }
+    return trace_range(t, 0, n, p, stop, cnt_i, cnt_e, tally, status, hit);
+}
+
+int th_particles_reserve(th_tracer *t, int64_t capacity) {
+    if (!t || capacity < 0) return fail(TH_EINVAL, "invalid argument");
+    TH_HIP(hipSetDevice(t->device));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    if (capacity <= t->cap) return TH_OK;
+    return reserve_particles(t, capacity);
+}
+
+int th_particles_upload(th_tracer *t, int64_t offset, const double *x, const double *y, const double *vx,
+                        const double *vy, const double *qm, const int8_t *spc, int64_t n) {
+    if (!t || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
+    if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    TH_HIP(hipSetDevice(t->device));
+    int rc = upload(t, offset, x, y, vx, vy, qm, spc, n);
+    if (rc) return rc;
+    TH_HIP(hipStreamSynchronize(t->stream));
+    return TH_OK;
+}
+
+int th_trace_resident(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p, int64_t *cnt_i,
+                      int64_t *cnt_e, th_tally *tally, int8_t *status, int32_t *hit) {
+    if (!t || !p || !tally || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
+    if ((cnt_i == nullptr) != (cnt_e == nullptr)) return fail(TH_EINVAL, "cnt_i and cnt_e go together");
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    TH_HIP(hipSetDevice(t->device));
+    return trace_range(t, offset, n, p, nullptr, cnt_i, cnt_e, tally, status, hit);
+}
+
+int th_read_counts(th_tracer *t, int32_t *counts) {
+    if (!t || !counts) return fail(TH_EINVAL, "invalid argument");
+    TH_HIP(hipSetDevice(t->device));
+    TH_HIP(hipMemcpy(counts, t->d_counts, sizeof(uint32_t) * 2 * (size_t)t->m, hipMemcpyDeviceToHost));
+    return TH_OK;
+}
+
+int th_last_push_ms(const th_tracer *t, double *ms) {
+    if (!t || !ms) return fail(TH_EINVAL, "invalid argument");
+    *ms = t->push_ms;
+    return TH_OK;
+}
+
+int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, double *out) {
+    if (!t || n < 0 || (n > 0 && (!a || !b || !out))) return fail(TH_EINVAL, "invalid argument");
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    double *d = nullptr;
+    TH_HIP(hipMalloc((void **)&d, sizeof(double) * 6 * (size_t)n));
+    int rc = TH_OK;
+    hipError_t e = hipMemcpyAsync(d, a, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + n, b, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) {
+        int threads = 256;
+        int64_t blocks = (n + threads - 1) / threads;
+        hipLaunchKernelGGL(th::selftest_math_kernel, dim3((unsigned)blocks), dim3(threads), 0, t->stream, d,
+                           d + n, n, d + 2 * n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) rc = fail(TH_EHIP, "selftest: %s", hipGetErrorString(e));
+    (void)hipFree(d);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+// Particle push kernels for gfx950 (CDNA4, wave64).
+//
+// One lane owns one particle at a time.  Lanes whose particle terminated
+// (absorbed, escaped, or out of steps) are refilled from a global work
+// counter with one atomic per wave (ballot + prefix popcount), so a wave stays
+// populated although path lengths run from 1 to max_steps.  Deposit tallies
+// are integer atomics into a per-workgroup LDS histogram over the surface
+// cells (exact and independent of order), flushed once per workgroup.
+//
+// The FP64 step reproduces the reference push bit for bit
+// (trench_trace.cu:17-59, :92-185; trench_charging_2d.py:492-702): same
+// operation order, contraction off, glibc-equivalent hypot (th_math.h).
+// Bit-preserving restructurings used here:
+//   * x/dx and y/dx of the new position are computed once per step and reused
+//     for the next step's interpolation (x/dx - 0.5) and for the previous-cell
+//     indices of wall entry; a reflected particle restores them with x0, y0;
+//   * the periodic wrap takes a - L or a + L where that equals the reference's
+//     fmod form exactly (Sterbenz), else falls back to fmod;
+//   * cfl*dx, 2*dx, nx*dx and ny*dx are formed once on the host (same products).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "th_math.h"
+
+namespace th {
+
+constexpr int kBlock = 256;          // 4 waves per workgroup
+constexpr int kStepsPerCheck = 8;    // steps between refill checks
+enum : int { kScalQueue = 0, kScalEsc0, kScalEsc1, kScalLost, kScalSteps, kScalOff, kScalCount };
+
+struct Grid {
+    const double4 *__restrict__ cells;   // {Ex*vac, Ey*vac, vac, phi} per cell
+    const float4 *__restrict__ cells32;  // FP32 push: {Ex*vac/dx, Ey*vac/dx, vac, 0}
+    const double *__restrict__ wface;    // barrier face weight per cell
+    const uint8_t *__restrict__ vac;     // 1 vacuum, 0 solid
+    const int32_t *__restrict__ smap;    // cell -> surface index, -1 elsewhere
+    int nx, ny, m;
+    double dx, cfl_dx, two_dx, lx, ly;   // cfl*dx, 2*dx, nx*dx, ny*dx formed on the host
+};
+
+struct Work {
+    const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
+    const int8_t *__restrict__ spc;
+    int64_t begin, end;                  // particles [begin, end)
+    int8_t *status;                      // nullable
+    int32_t *hit;                        // nullable
+    uint32_t *counts;                    // [2][m] per-surface-cell tallies
+    uint32_t *overflow;                  // [2][nx*ny] tallies for non-surface cells
+    unsigned long long *scal;            // kScalCount counters
+    int max_steps, refill;
+};
+
+__device__ __forceinline__ int wrap_cell(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i < 2 * n) return i - n;
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ int clip_cell(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+
+// NumPy's float %, i.e. r = fmod(a, L); r < 0 ? r + L : r, for L > 0.
+__device__ __forceinline__ double wrap_pos(double a, double L) {
+    if (a >= 0.0 && a < L) return a;
+    if (a >= L && a < 2.0 * L) return a - L;   // exact: L <= a < 2L
+    if (a < 0.0 && a > -L) return a + L;       // fmod(a, L) == a here
+    double r = fmod(a, L);
+    return r < 0.0 ? r + L : r;
+}
+
+struct State64 {
+    double x, y, vx, vy, qm;
+    double q, r;   // x/dx, y/dx of the current position
+};
+
+// One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2 (escaped).
+template <bool BARRIER, bool TAPERED>
+__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
+    const int nx = g.nx, ny = g.ny;
+    // Vacuum-weighted bilinear field at the cell-centred lattice.
+    double fx = s.q - 0.5, fy = s.r - 0.5;
+    double fi = floor(fx), fj = floor(fy);
+    int i = (int)fi, j = (int)fj;
+    double tx = fx - (double)i, ty = fy - (double)j;
+    int i0 = wrap_cell(i, nx), i1 = wrap_cell(i + 1, nx);
+    int j0 = clip_cell(j, ny), j1 = clip_cell(j + 1, ny);
+    double4 c00 = g.cells[i0 * ny + j0], c10 = g.cells[i1 * ny + j0];
+    double4 c01 = g.cells[i0 * ny + j1], c11 = g.cells[i1 * ny + j1];
+    double w00 = (1.0 - tx) * (1.0 - ty), w10 = tx * (1.0 - ty), w01 = (1.0 - tx) * ty, w11 = tx * ty;
+    double ex = 0.0, ey = 0.0, sw = 0.0;
+    ex += w00 * c00.x; ey += w00 * c00.y; sw += w00 * c00.z;
+    ex += w10 * c10.x; ey += w10 * c10.y; sw += w10 * c10.z;
+    ex += w01 * c01.x; ey += w01 * c01.y; sw += w01 * c01.z;
+    ex += w11 * c11.x; ey += w11 * c11.y; sw += w11 * c11.z;
+    sw = sw < 1e-12 ? 1e-12 : sw;
+    ex = ex / sw;
+    ey = ey / sw;
+    // CFL-limited trapezoidal push.
+    double ax = s.qm * ex, ay = s.qm * ey;
+    double dt = g.cfl_dx / (th_hypot(s.vx, s.vy) + sqrt(g.two_dx * th_hypot(ax, ay)) + 1.0);
+    double vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
+    const double x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
+    double sx = 0.5 * (s.vx + vxn) * dt, sy = 0.5 * (s.vy + vyn) * dt;
+    double xu = x0 + sx;
+    s.x = wrap_pos(xu, g.lx);
+    s.y = y0 + sy;
+    s.vx = vxn;
+    s.vy = vyn;
+    s.q = s.x / g.dx;
+    s.r = s.y / g.dx;
+    // Classification of the end point.
+    bool escaped = s.y >= g.ly;
+    int ix = min((int)s.q, nx - 1), iy = clip_cell((int)s.r, ny);
+    bool hit = !escaped && !g.vac[ix * ny + iy];
+    bool corner = false;
+    if (TAPERED && !hit && !escaped)
+        corner = ix != min((int)q0, nx - 1) && iy != clip_cell((int)r0, ny);
+    if ((hit && (BARRIER || TAPERED)) || corner) {
+        // First solid cell crossed during the step, and the vacuum cell before it.
+        int ix0 = min((int)q0, nx - 1), iy0 = clip_cell((int)r0, ny);
+        int dix = (int)floor(xu / g.dx) - ix0;
+        int diy = clip_cell((int)floor(s.r), ny) - iy0;   // (y0 + sy)/dx is s.r
+        double tfx = dix != 0 ? ((double)(ix0 + (dix > 0)) * g.dx - x0) / sx : INFINITY;
+        double tfy = diy != 0 ? ((double)(iy0 + (diy > 0)) * g.dx - y0) / sy : INFINITY;
+        bool x_first = tfx < tfy;
+        int ax_ = x_first ? wrap_cell(ix0 + dix, nx) : ix0;
+        int ay_ = x_first ? iy0 : iy0 + diy;
+        bool last = dix != 0 && diy != 0 && g.vac[ax_ * ny + ay_];
+        int cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
+        int cy = last ? iy0 + diy : ay_;
+        int vcx = last ? ax_ : ix0, vcy = last ? ay_ : iy0;
+        bool face_x = last ? !x_first : x_first;
+        hit = !g.vac[cx * ny + cy];
+        if (BARRIER && hit) {
+            double vn = face_x ? s.vx : s.vy;
+            int c = cx * ny + cy, v = vcx * ny + vcy;
+            double dphi = g.wface[c] * (g.cells[c].w - g.cells[v].w);
+            if (0.5 * vn * vn < s.qm * dphi) {   // cannot climb the wall barrier: reflect
+                if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
+                s.x = x0; s.y = y0; s.q = q0; s.r = r0;
+                hit = false;
+            }
+        }
+        ix = cx;
+        iy = cy;
+    }
+    if (hit) { cell = ix * ny + iy; return 1; }
+    return escaped ? 2 : 0;
+}
+
+// Records the end of one particle: per-particle outputs and LDS tallies.
+__device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *lds_counts,
+                                       unsigned long long *lds_scal, int64_t pid, int sp, int st,
+                                       int cell) {
+    if (w.status) w.status[pid] = (int8_t)st;
+    if (w.hit) w.hit[pid] = st == 1 ? cell : -1;
+    if (st == 1) {
+        int k = g.smap[cell];
+        if (k >= 0) {
+            atomicAdd(&lds_counts[sp * g.m + k], 1u);
+        } else {
+            atomicAdd(&w.overflow[(int64_t)sp * g.nx * g.ny + cell], 1u);
+            atomicAdd(&lds_scal[kScalOff], 1ull);
+        }
+    } else if (st == 2) {
+        atomicAdd(&lds_scal[kScalEsc0 + sp], 1ull);
+    } else {
+        atomicAdd(&lds_scal[kScalLost], 1ull);
+    }
+}
+
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
+    extern __shared__ uint32_t lds_counts[];   // [2][m]
+    __shared__ unsigned long long lds_scal[kScalCount];
+    for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
+    if (threadIdx.x < kScalCount) lds_scal[threadIdx.x] = 0ull;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
+    State64 s;
+    int sp = 0, k = 0;
+    unsigned long long my_steps = 0;
+    for (;;) {
+        unsigned long long idle = __ballot(pid == -1);
+        unsigned long long busy = __ballot(pid >= 0);
+        if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&w.scal[kScalQueue], (unsigned long long)__popcll(idle));
+            base = __shfl(base, leader);
+            if (pid == -1) {
+                int64_t q = w.begin + (int64_t)(base + (unsigned long long)__popcll(idle & below));
+                if (q < w.end) {
+                    pid = q;
+                    s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q]; s.qm = w.qm[q];
+                    s.q = s.x / g.dx; s.r = s.y / g.dx;
+                    sp = w.spc[q];
+                    k = 0;
+                } else {
+                    pid = -2;
+                }
+            }
+            busy = __ballot(pid >= 0);
+        }
+        if (busy == 0ull) break;
+        if (pid >= 0) {
+            for (int t = 0; t < kStepsPerCheck; ++t) {
+                int cell = -1;
+                int st = step64<BARRIER, TAPERED>(g, s, cell);
+                ++k;
+                if (st != 0 || k >= w.max_steps) {
+                    finish(g, w, lds_counts, lds_scal, pid, sp, st != 0 ? st : 3, cell);
+                    my_steps += (unsigned long long)k;
+                    pid = -1;
+                    break;
+                }
+            }
+        }
+    }
+    atomicAdd(&lds_scal[kScalSteps], my_steps);
+    __syncthreads();
+    for (int c = threadIdx.x; c < 2 * g.m; c += kBlock)
+        if (lds_counts[c]) atomicAdd(&w.counts[c], lds_counts[c]);
+    if (threadIdx.x >= kScalEsc0 && threadIdx.x < kScalCount && lds_scal[threadIdx.x])
+        atomicAdd(&w.scal[threadIdx.x], lds_scal[threadIdx.x]);
+}
+
+// Packs the frozen field of one batch into per-cell records.
+__global__ void pack_cells_kernel(const double *__restrict__ ex, const double *__restrict__ ey,
+                                  const double *__restrict__ phi, const uint8_t *__restrict__ vac,
+                                  int n, double inv_dx, double4 *__restrict__ cells,
+                                  float4 *__restrict__ cells32) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    bool v = vac[c] != 0;
+    double e_x = v ? ex[c] : 0.0, e_y = v ? ey[c] : 0.0;
+    cells[c] = make_double4(e_x, e_y, v ? 1.0 : 0.0, phi ? phi[c] : 0.0);
+    cells32[c] = make_float4((float)(e_x * inv_dx), (float)(e_y * inv_dx), v ? 1.0f : 0.0f, 0.0f);
+}
+
+__global__ void selftest_math_kernel(const double *a, const double *b, int64_t n, double *out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double x = a[k], y = b[k];
+    out[4 * k + 0] = th_hypot(x, y);
+    out[4 * k + 1] = sqrt(x);
+    out[4 * k + 2] = x / y;
+    out[4 * k + 3] = fmod(x, y);
+}
+
+}  // namespace th

@@ -1,0 +1,221 @@
+"""HIP particle tracer with the reference tracer API.
+
+``HipTracer`` is a drop-in for ``CudaTracer`` (trench_cuda.py:21-138):
+constructed once per geometry with the vacuum map, then called once per batch
+as ``tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None,
+stop=None, rf=None) -> (cnt_i, cnt_e, escaped, lost)`` (trench_cuda.py:41,
+trench_charging_2d.py:542, :702).  Argument checks and their exception types
+follow trench_cuda.py:42-59.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib
+
+BACKENDS = ("hip", "auto")
+
+
+def tapered_of(p):
+    """Staircase corner crossings are tested when either wall is tapered
+    (trench_charging_2d.py:554, trench_cuda.py:113)."""
+    return bool(p.trench_taper_deg or (p.mask_t > 0 and p.mask_taper_deg))
+
+
+class HipTracer:
+    """Reuses the device geometry; uploads the frozen field and particles per call
+    and traces them on the GPU (trench_cuda.py:21-39)."""
+
+    #: 0 = FP64 push, bit-identical with the reference; 1 = FP32 push.
+    precision = 0
+    #: wave refill threshold (lanes); 0 = library default
+    refill = 0
+
+    def __init__(self, vac, device=0):
+        L = _lib.load()
+        self.device = int(device)
+        n_dev = _lib.device_count()
+        if self.device < 0 or self.device >= n_dev:
+            raise RuntimeError(f"HIP device {self.device} is not available ({n_dev} visible)")
+        self.vac = np.asarray(vac, dtype=bool).copy()
+        if self.vac.ndim != 2:
+            raise ValueError("vac must be a 2-D (nx, ny) array")
+        nx, ny = self.vac.shape
+        handle = ctypes.c_void_p()
+        v8 = np.ascontiguousarray(self.vac, dtype=np.uint8)
+        _lib.check(L.th_tracer_create(_lib.ptr(v8, _lib.u8p), nx, ny, self.device, ctypes.byref(handle)))
+        self._h = handle
+        buf = ctypes.create_string_buffer(256)
+        _lib.check(L.th_tracer_device_name(self._h, buf, 256))
+        self.name = buf.value.decode(errors="replace")
+        self.last_tally = None
+        self.last_status = None
+        self.last_hit = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().th_tracer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def surface_cells(self):
+        """Flat indices (ix*ny+iy) of the surface cells the tallies are kept on."""
+        L = _lib.load()
+        m = ctypes.c_int32(0)
+        _lib.check(L.th_tracer_surface(self._h, ctypes.byref(m), None))
+        flat = np.empty(m.value, dtype=np.int32)
+        _lib.check(L.th_tracer_surface(self._h, ctypes.byref(m), _lib.ptr(flat, _lib.i32p)))
+        return flat
+
+    def __call__(self, x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None, stop=None, rf=None,
+                 keep_status=False):
+        if np.shape(vac) != self.vac.shape or not np.array_equal(vac, self.vac):
+            raise ValueError("the HIP tracer was built for another geometry; create a new tracer")
+        if not (0 < p.cfl <= 1) or p.max_steps < 1:
+            raise ValueError("HIP tracing needs 0 < cfl <= 1 and max_steps >= 1")
+        if record is not None:
+            raise NotImplementedError("record mode is not available in the HIP tracer yet")
+        if rf is not None:
+            raise NotImplementedError("RF particle clocks are not available in the HIP tracer yet")
+        return self._trace(x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, stop, keep_status)
+
+    def _trace(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, stop, keep_status):
+        L = _lib.load()
+        n = len(x)
+        nx, ny = self.vac.shape
+        if any(np.shape(v) != (n,) for v in (x, y, vx, vy, qm, spc)):
+            raise ValueError("particle arrays differ in length")
+        if any(np.shape(v) != (nx, ny) for v in (Ex, Ey)):
+            raise ValueError("field shape does not match the geometry")
+        spc = np.ascontiguousarray(spc, dtype=np.int8)
+        if np.any((spc != 0) & (spc != 1)):
+            raise ValueError("species must be 0 (ion) or 1 (electron)")
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        ex, ey = f64(Ex), f64(Ey)
+        if barrier is None:
+            phi = wface = None
+        else:
+            if any(np.shape(v) != (nx, ny) for v in barrier):
+                raise ValueError("wall potential shape does not match the geometry")
+            phi, wface = (f64(v) for v in barrier)
+        _lib.check(L.th_set_fields(self._h, _lib.ptr(ex, _lib.dp), _lib.ptr(ey, _lib.dp),
+                                   _lib.ptr(phi, _lib.dp), _lib.ptr(wface, _lib.dp)))
+        xs = [f64(v) for v in (x, y, vx, vy, qm)]
+        prm = _lib.TraceParams(dx=float(p.dx), cfl=float(p.cfl), max_steps=int(p.max_steps),
+                               tapered=int(tapered_of(p)), barrier=int(barrier is not None),
+                               precision=int(self.precision), refill=int(self.refill), chunk=0)
+        cnt_i = np.empty((nx, ny), dtype=np.int64)
+        cnt_e = np.empty((nx, ny), dtype=np.int64)
+        tally = _lib.Tally()
+        status = np.empty(n, dtype=np.int8) if keep_status else None
+        hit = np.empty(n, dtype=np.int32) if keep_status else None
+        flag = None
+        watcher = None
+        done = threading.Event()
+        if stop is not None:
+            # The C side polls this flag between launches; a watcher mirrors stop into it.
+            flag = ctypes.c_int32(1 if stop.is_set() else 0)
+
+            def watch():
+                while not done.is_set():
+                    if stop.is_set():
+                        flag.value = 1
+                        return
+                    done.wait(0.002)
+
+            watcher = threading.Thread(target=watch, daemon=True)
+            watcher.start()
+        try:
+            _lib.check(L.th_trace(self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n,
+                                  ctypes.byref(prm), ctypes.byref(flag) if flag is not None else None,
+                                  _lib.ptr(cnt_i, _lib.i64p), _lib.ptr(cnt_e, _lib.i64p), ctypes.byref(tally),
+                                  _lib.ptr(status, _lib.i8p), _lib.ptr(hit, _lib.i32p)))
+        finally:
+            done.set()
+            if watcher is not None:
+                watcher.join()
+        escaped = np.array([tally.escaped[0], tally.escaped[1]], dtype=np.int64)
+        self.last_tally = dict(steps=int(tally.steps), off_surface=int(tally.off_surface),
+                               cancelled=bool(tally.cancelled))
+        self.last_status, self.last_hit = status, hit
+        return cnt_i, cnt_e, escaped, int(tally.lost)
+
+    # ---- device-resident batches (no reference counterpart; for batch loops and benchmarks)
+    def params_for(self, p, barrier):
+        return _lib.TraceParams(dx=float(p.dx), cfl=float(p.cfl), max_steps=int(p.max_steps),
+                                tapered=int(tapered_of(p)), barrier=int(bool(barrier)),
+                                precision=int(self.precision), refill=int(self.refill), chunk=0)
+
+    def set_fields(self, Ex, Ey, barrier=None):
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        phi, wface = (None, None) if barrier is None else (f64(v) for v in barrier)
+        _lib.check(_lib.load().th_set_fields(self._h, _lib.ptr(f64(Ex), _lib.dp), _lib.ptr(f64(Ey), _lib.dp),
+                                             _lib.ptr(phi, _lib.dp), _lib.ptr(wface, _lib.dp)))
+
+    def reserve(self, capacity):
+        _lib.check(_lib.load().th_particles_reserve(self._h, int(capacity)))
+
+    def upload(self, offset, x, y, vx, vy, qm, spc):
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        xs = [f64(v) for v in (x, y, vx, vy, qm)]
+        spc = np.ascontiguousarray(spc, dtype=np.int8)
+        _lib.check(_lib.load().th_particles_upload(self._h, int(offset), *(_lib.ptr(v, _lib.dp) for v in xs),
+                                                   _lib.ptr(spc, _lib.i8p), len(xs[0])))
+
+    def trace_resident(self, offset, n, prm, with_counts=False):
+        """Trace pool[offset, offset+n); returns (tally dict, counts or None, push_ms)."""
+        L = _lib.load()
+        tally = _lib.Tally()
+        nx, ny = self.vac.shape
+        ci = np.empty((nx, ny), np.int64) if with_counts else None
+        ce = np.empty((nx, ny), np.int64) if with_counts else None
+        _lib.check(L.th_trace_resident(self._h, int(offset), int(n), ctypes.byref(prm), _lib.ptr(ci, _lib.i64p),
+                                       _lib.ptr(ce, _lib.i64p), ctypes.byref(tally), None, None))
+        ms = ctypes.c_double(0.0)
+        _lib.check(L.th_last_push_ms(self._h, ctypes.byref(ms)))
+        t = dict(escaped=(int(tally.escaped[0]), int(tally.escaped[1])), lost=int(tally.lost),
+                 steps=int(tally.steps), off_surface=int(tally.off_surface))
+        return t, ((ci, ce) if with_counts else None), float(ms.value)
+
+    def selftest_math(self, a, b):
+        """Device hypot / sqrt / divide / fmod on arrays (for arithmetic pinning)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty((a.size, 4), dtype=np.float64)
+        _lib.check(_lib.load().th_selftest_math(self._h, _lib.ptr(a, _lib.dp), _lib.ptr(b, _lib.dp), a.size,
+                                                _lib.ptr(out, _lib.dp)))
+        return out
+
+
+def make_particle_tracer(p, vac, log=None):
+    """Backend factory with the reference's contract (trench_charging_2d.py:705-724):
+    returns ``(tracer, backend_used)``.  ``"hip"`` raises RuntimeError with the
+    reason when the GPU path cannot start.  ``"auto"`` also raises: this package
+    has no CPU path to fall back to (the reference's own factory keeps its
+    ``trace_cpu`` for that)."""
+    backend = getattr(p, "backend", "hip")
+    if backend not in BACKENDS:
+        raise ValueError(f"backend must be one of {' / '.join(BACKENDS)}")
+    device = int(getattr(p, "hip_device", getattr(p, "cuda_device", 0)))
+    try:
+        tracer = HipTracer(vac, device)
+    except Exception as exc:
+        raise RuntimeError(f"HIP tracer could not start: {exc}") from exc
+    if log is not None:
+        log(f"particle tracing: HIP ({tracer.name}, device {device}, float64)")
+    return tracer, "hip"
+
+
+def trace(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None, stop=None, rf=None):
+    """One-shot tracer call (trench_charging_2d.py:727-730)."""
+    tracer, _ = make_particle_tracer(p, vac)
+    try:
+        return tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier, record, stop, rf=rf)
+    finally:
+        tracer.close()
