@@ -10,6 +10,9 @@
 // The FP64 step reproduces the reference push bit for bit
 // (trench_trace.cu:17-59, :92-185; trench_charging_2d.py:492-702): same
 // operation order, contraction off, glibc-equivalent hypot (th_math.h).
+// Cell indices are clamped to the grid on both sides (the reference clamps x
+// only from above; positions are never negative there, so results agree, and
+// a caller's out-of-range initial x cannot index outside the grid here).
 // Bit-preserving restructurings used here:
 //   * x/dx and y/dx of the new position are computed once per step and reused
 //     for the next step's interpolation (x/dx - 0.5) and for the previous-cell
@@ -110,14 +113,14 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
     s.r = s.y / g.dx;
     // Classification of the end point.
     bool escaped = s.y >= g.ly;
-    int ix = min((int)s.q, nx - 1), iy = clip_cell((int)s.r, ny);
+    int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
     bool hit = !escaped && !g.vac[ix * ny + iy];
     bool corner = false;
     if (TAPERED && !hit && !escaped)
-        corner = ix != min((int)q0, nx - 1) && iy != clip_cell((int)r0, ny);
+        corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
     if ((hit && (BARRIER || TAPERED)) || corner) {
         // First solid cell crossed during the step, and the vacuum cell before it.
-        int ix0 = min((int)q0, nx - 1), iy0 = clip_cell((int)r0, ny);
+        int ix0 = clip_cell((int)q0, nx), iy0 = clip_cell((int)r0, ny);
         int dix = (int)floor(xu / g.dx) - ix0;
         int diy = clip_cell((int)floor(s.r), ny) - iy0;   // (y0 + sy)/dx is s.r
         double tfx = dix != 0 ? ((double)(ix0 + (dix > 0)) * g.dx - x0) / sx : INFINITY;
