@@ -17,7 +17,9 @@
  * is glibc's hypot -- the function NumPy's np.hypot calls -- so results are
  * bit-identical with trace_cpu on identical inputs.
  *
- * RF clocks (trench_trace.cu:97-110) and path recording are not restated here.
+ * Record mode (trench_trace.cu:149-183, :189-192; trench_charging_2d.py:565-577,
+ * :652-686) is restated: end points clipped to the crossed face, final
+ * velocities, and probe paths.  RF clocks (trench_trace.cu:97-110) are not.
  */
 #include <math.h>
 #include <stdint.h>
@@ -37,9 +39,17 @@ static inline double wrap_x(double a, double len) {
 typedef struct {
     const double *ex, *ey, *phi, *wface;
     const uint8_t *vac;
-    int nx, ny, tapered, barrier;
+    int nx, ny, tapered, barrier, record;
     double dx, cfl, lx, ly;
 } grid_t;
+
+/* Record-mode outputs of one particle (all nullable except when record is on). */
+typedef struct {
+    double *end;      /* [2] end point */
+    double *vel;      /* [2] final velocity */
+    double *path;     /* [2 * (max_steps + 1)] probe path, or NULL */
+    int32_t *plen;    /* path length */
+} rec_t;
 
 /* trench_charging_2d.py:492-509 / trench_trace.cu:17-39 */
 static void field_at(const grid_t *g, double x, double y, double *ox, double *oy) {
@@ -86,9 +96,14 @@ static void first_solid(const grid_t *g, double x0, double y0, double sx, double
 /* One particle to termination.  Returns the status (1 absorbed, 2 escaped,
  * 3 still live after max_steps) and writes the hit cell and step count. */
 static int push_one(const grid_t *g, int max_steps, double x, double y, double vx, double vy,
-                    double qm, int32_t *hit, int64_t *nsteps) {
+                    double qm, int32_t *hit, int64_t *nsteps, const rec_t *rec) {
     const double dx = g->dx;
     const int nx = g->nx, ny = g->ny;
+    int status = 3;
+    int plen = 0;
+    double xp = x, yp = y;
+    if (rec && rec->path) { rec->path[0] = x; rec->path[1] = y; plen = 1; }
+    *nsteps = max_steps;
     for (int k = 0; k < max_steps; ++k) {
         double ex, ey;
         field_at(g, x, y, &ex, &ey);
@@ -109,8 +124,9 @@ static int push_one(const grid_t *g, int max_steps, double x, double y, double v
             int ix0 = (int)(x0 / dx); if (ix0 > nx - 1) ix0 = nx - 1;
             corner = ix != ix0 && iy != clip_i((int)(y0 / dx), ny);
         }
-        if ((hitting && (g->barrier || g->tapered)) || corner) {
-            int cx, cy, vcx, vcy, face_x;
+        int cx = ix, cy = iy, face_x = 0;
+        if ((hitting && (g->barrier || g->record || g->tapered)) || corner) {
+            int vcx, vcy;
             first_solid(g, x0, y0, sx, sy, &cx, &cy, &vcx, &vcy, &face_x);
             hitting = !g->vac[cx * ny + cy];
             if (hitting && g->barrier) {
@@ -125,18 +141,45 @@ static int push_one(const grid_t *g, int max_steps, double x, double y, double v
             }
             ix = cx; iy = cy;
         }
-        if (hitting) { *hit = ix * ny + iy; *nsteps = k + 1; return 1; }
-        if (escaped) { *nsteps = k + 1; return 2; }
+        xp = x; yp = y;
+        if (rec && hitting) {           /* end point on the crossed face */
+            double frac;
+            if (face_x) {
+                double xb = (cx + (sx < 0.0)) * dx;
+                xb += nearbyint((x0 - xb) / g->lx) * g->lx;
+                frac = (xb - x0) / sx;
+            } else {
+                frac = ((cy + (sy < 0.0)) * dx - y0) / sy;
+            }
+            frac = fmin(fmax(frac, 0.0), 1.0);
+            xp = wrap_x(x0 + frac * sx, g->lx);
+            yp = y0 + frac * sy;
+        }
+        if (rec && escaped) {
+            double frac = fmin(fmax((g->ly - y0) / sy, 0.0), 1.0);
+            xp = wrap_x(x0 + frac * sx, g->lx);
+            yp = g->ly;
+        }
+        if (rec && rec->path) { rec->path[2 * plen] = xp; rec->path[2 * plen + 1] = yp; ++plen; }
+        if (hitting) { *hit = ix * ny + iy; *nsteps = k + 1; status = 1; break; }
+        if (escaped) { *nsteps = k + 1; status = 2; break; }
     }
-    *nsteps = max_steps;
-    return 3;
+    if (rec) {
+        rec->end[0] = xp; rec->end[1] = yp;
+        rec->vel[0] = vx; rec->vel[1] = vy;
+        if (rec->path) *rec->plen = plen;
+    }
+    return status;
 }
 
 /*
  * Trace n particles.  Outputs: status[n] (int8), hit[n] (int32, -1 unless
  * absorbed), optional steps[n]; tallies cnt_i/cnt_e[nx*ny] (int64, zeroed
- * here), escaped[2], *lost and *total_steps.  nthreads <= 0 uses OpenMP's
- * default.  Returns 0, or -1 on a species value other than 0/1.
+ * here), escaped[2], *lost and *total_steps.  Record mode when endpoints and
+ * final_velocity are given ([n][2] each); slots[n] (-1 or a probe index),
+ * paths[n_probe][max_steps+1][2] and path_lengths[n_probe] then hold the
+ * probe paths.  nthreads <= 0 uses OpenMP's default.  Returns 0, or -1 on a
+ * species value other than 0/1.
  */
 int oracle_trace(const double *x, const double *y, const double *vx, const double *vy,
                  const double *qm, const int8_t *spc, int64_t n,
@@ -145,10 +188,13 @@ int oracle_trace(const double *x, const double *y, const double *vx, const doubl
                  double dx, double cfl, int max_steps, int tapered, int nthreads,
                  int8_t *status, int32_t *hit, int64_t *steps,
                  int64_t *cnt_i, int64_t *cnt_e, int64_t *escaped, int64_t *lost,
-                 int64_t *total_steps) {
+                 int64_t *total_steps,
+                 double *endpoints, double *final_velocity, const int32_t *slots,
+                 double *paths, int32_t *path_lengths) {
     for (int64_t k = 0; k < n; ++k)
         if (spc[k] != 0 && spc[k] != 1) return -1;
-    grid_t g = {ex, ey, phi, wface, vac, nx, ny, tapered, phi != NULL && wface != NULL,
+    int record = endpoints != NULL && final_velocity != NULL;
+    grid_t g = {ex, ey, phi, wface, vac, nx, ny, tapered, phi != NULL && wface != NULL, record,
                 dx, cfl, nx * dx, ny * dx};
     int64_t tot = 0;
 #ifdef _OPENMP
@@ -158,7 +204,16 @@ int oracle_trace(const double *x, const double *y, const double *vx, const doubl
     for (int64_t k = 0; k < n; ++k) {
         int32_t h = -1;
         int64_t s = 0;
-        status[k] = (int8_t)push_one(&g, max_steps, x[k], y[k], vx[k], vy[k], qm[k], &h, &s);
+        rec_t rec, *rp = NULL;
+        if (record) {
+            int slot = slots ? slots[k] : -1;
+            rec.end = endpoints + 2 * k;
+            rec.vel = final_velocity + 2 * k;
+            rec.path = slot >= 0 ? paths + (int64_t)slot * 2 * (max_steps + 1) : NULL;
+            rec.plen = slot >= 0 ? path_lengths + slot : NULL;
+            rp = &rec;
+        }
+        status[k] = (int8_t)push_one(&g, max_steps, x[k], y[k], vx[k], vy[k], qm[k], &h, &s, rp);
         hit[k] = h;
         if (steps) steps[k] = s;
         tot += s;

@@ -46,7 +46,8 @@ def lib():
         L.oracle_trace.argtypes = [_dp, _dp, _dp, _dp, _dp, _i8p, ctypes.c_int64,
                                    _dp, _dp, _u8p, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, _i8p, _i32p, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p]
+                                   ctypes.c_int, _i8p, _i32p, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p,
+                                   _dp, _dp, _i32p, _dp, _i32p]
         L.oracle_hypot.restype = None
         L.oracle_hypot.argtypes = [_dp, _dp, _dp, ctypes.c_int64]
         _lib = L
@@ -62,9 +63,15 @@ def is_tapered(p):
     return bool(p.trench_taper_deg or (p.mask_t > 0 and p.mask_taper_deg))
 
 
+def probe_selection(n, probe_trajectories):
+    """Probe particles of record mode (trench_cuda.py:85-87, trench_charging_2d.py:566-569)."""
+    return np.unique(np.linspace(0, n - 1, min(probe_trajectories, n), dtype=int))
+
+
 def trace_arrays(x, y, vx, vy, qm, spc, Ex, Ey, vac, dx, cfl, max_steps, tapered,
-                 barrier=None, threads=0):
-    """Run the C push.  Returns dict(status, hit, steps, cnt_i, cnt_e, escaped, lost, total_steps)."""
+                 barrier=None, threads=0, record=False, probe_trajectories=0):
+    """Run the C push.  Returns dict(status, hit, steps, cnt_i, cnt_e, escaped, lost, total_steps)
+    plus, in record mode, endpoints, final_velocity, selected and paths (list of (k, 2))."""
     vac = np.ascontiguousarray(vac, dtype=np.uint8)
     nx, ny = vac.shape
     f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
@@ -80,15 +87,29 @@ def trace_arrays(x, y, vx, vy, qm, spc, Ex, Ey, vac, dx, cfl, max_steps, tapered
                escaped=np.zeros(2, np.int64))
     lost = ctypes.c_int64(0)
     total = ctypes.c_int64(0)
+    ends = vel = slots = paths = plen = None
+    if record:
+        ends = np.empty((n, 2))
+        vel = np.empty((n, 2))
+        selected = probe_selection(n, probe_trajectories) if n else np.zeros(0, int)
+        slots = np.full(n, -1, np.int32)
+        slots[selected] = np.arange(selected.size)
+        paths = np.zeros((max(selected.size, 1), max_steps + 1, 2))
+        plen = np.zeros(max(selected.size, 1), np.int32)
     rc = lib().oracle_trace(_ptr(x, _dp), _ptr(y, _dp), _ptr(vx, _dp), _ptr(vy, _dp), _ptr(qm, _dp),
                             _ptr(spc, _i8p), n, _ptr(Ex, _dp), _ptr(Ey, _dp), _ptr(vac, _u8p),
                             nx, ny, _ptr(phi, _dp), _ptr(wface, _dp), float(dx), float(cfl),
                             int(max_steps), int(bool(tapered)), int(threads),
                             _ptr(out["status"], _i8p), _ptr(out["hit"], _i32p), _ptr(out["steps"], _i64p),
                             _ptr(out["cnt_i"], _i64p), _ptr(out["cnt_e"], _i64p),
-                            _ptr(out["escaped"], _i64p), ctypes.byref(lost), ctypes.byref(total))
+                            _ptr(out["escaped"], _i64p), ctypes.byref(lost), ctypes.byref(total),
+                            _ptr(ends, _dp), _ptr(vel, _dp), _ptr(slots, _i32p), _ptr(paths, _dp),
+                            _ptr(plen, _i32p))
     if rc != 0:
         raise ValueError("species must be 0 (ion) or 1 (electron)")
+    if record:
+        out.update(endpoints=ends, final_velocity=vel, selected=selected,
+                   paths=[paths[k, :plen[k]].copy() for k in range(selected.size)])
     out["lost"] = int(lost.value)
     out["total_steps"] = int(total.value)
     return out

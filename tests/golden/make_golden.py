@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors under tests/golden/ by importing the reference
+(Reasonia-TK/trench01, read-only checkout) in the build container.
+
+Only run here, never on the GPU box; the reference does not travel.  Each
+vector pins a reference function that the reference's own fixtures
+(tests/data/base_*.npz: absorb wall, no mask, no taper) leave open:
+
+* ``trace_*.npz``   trace_cpu (trench_charging_2d.py:542-702) in record mode on
+  barrier / absorb walls, conductive masks and tapered walls, with charged
+  fields from the reference's own Poisson set-up (setup_fields :340-413,
+  compute_field :439-454): inputs, fields, per-particle status / hit cell /
+  end point / final velocity, per-cell counts.
+* ``setup_*.npz``   the conductor-collapsed LU (make_solver :297-309),
+  conductor_response / C_cond (:807-812, :1089), leak operator
+  (build_leakage :768-803) and conductor_step (:815-830) on small geometries.
+* ``run_ar5.npz``   a whole run() (:1045-1369) of the plain AR 5 workload
+  (80x200, conductive mask, sigma_s 0, barrier wall, 100 eV ions with 3 degree
+  polar spread, Te 4 eV), 50 batches of 1e4 particles per species, seed 0;
+  ``trace_ar5_marched.npz``: a fresh record-mode batch through its final field.
+
+Usage:  python tests/golden/make_golden.py [--reference /root/reference] [--only NAME]
+"""
+import argparse
+import json
+import sys
+import time
+from dataclasses import asdict, replace
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+
+
+def load_reference(path):
+    sys.path.insert(0, str(path))
+    import matplotlib
+    matplotlib.use("Agg")
+    import trench_charging_2d as T
+    return T
+
+
+def ion_sampler(T, kind, sigma_deg, energy_eV=None, e_lo=None, e_hi=None, mass_amu=40.0):
+    """Polar-angle Gaussian ion velocities (monoenergetic or arcsine IED),
+    the ion_velocity(rng, n) hook of the reference's inject (:458-474)."""
+    m_i = mass_amu * T.AMU
+    sig = np.radians(sigma_deg)
+
+    def sample(rng, n):
+        if kind == "mono":
+            E = np.full(n, float(energy_eV))
+        else:
+            E = e_lo + (e_hi - e_lo) * np.sin(0.5 * np.pi * rng.uniform(size=n)) ** 2
+        theta = rng.normal(0.0, sig, n)
+        v = np.sqrt(2.0 * E * T.E_CHARGE / m_i)
+        return v * np.sin(theta), -v * np.cos(theta)
+
+    return sample
+
+
+def case_params(T, name):
+    base = T.Params()
+    if name.startswith("small"):
+        return replace(base, nx=30, trench_w=10, trench_d=20, floor_t=5, mask_t=6, n_vac=12, bias="dc",
+                       sigma_s=0, n_per_batch=400, max_steps=1000, probe_trajectories=12,
+                       wall_model="absorb" if name.endswith("absorb") else "barrier")
+    if name.startswith("taper"):
+        return replace(base, nx=40, trench_w=16, trench_d=20, floor_t=4, mask_t=12, n_vac=10,
+                       trench_taper_deg=float(np.degrees(np.arctan(.25))),
+                       mask_taper_deg=float(np.degrees(np.arctan(.5))), bias="dc", sigma_s=0,
+                       n_per_batch=1000, max_steps=800, probe_trajectories=12,
+                       wall_model="absorb" if name.endswith("absorb") else "barrier")
+    if name.startswith("ar5"):
+        return replace(base, nx=80, trench_w=20, trench_d=100, floor_t=10, mask_t=20, n_vac=70,
+                       mask_type="conductor", field_mode="floating", sigma_s=0.0, wall_model="barrier",
+                       bias="dc", electron_temp_eV=4.0, n_per_batch=2000, seed=0, probe_trajectories=12)
+    if name.startswith("ar8"):
+        return replace(base, nx=120, trench_w=20, trench_d=160, floor_t=10, mask_t=40, n_vac=110,
+                       trench_taper_deg=2.0, mask_taper_deg=5.0, mask_type="conductor",
+                       field_mode="floating", sigma_s=1e-15, wall_model="barrier", bias="dc",
+                       electron_temp_eV=4.0, n_per_batch=2000, seed=0, probe_trajectories=12)
+    raise KeyError(name)
+
+
+TRACE_CASES = {
+    # name: (sidewall charge left/right [C/m^3], ion sampler)
+    "small_barrier": ((2e7, -1e7), None),
+    "small_absorb": ((2e7, -1e7), None),
+    "taper_barrier": ((2e7, -1e7), None),
+    "taper_absorb": ((2e7, -1e7), None),
+    "ar5_barrier": ((3e7, -1.5e7), dict(kind="mono", energy_eV=100.0, sigma_deg=3.0)),
+    "ar8_barrier": ((4e7, -2e7), dict(kind="arcsine", e_lo=40.0, e_hi=260.0, sigma_deg=4.0)),
+}
+
+
+def params_json(p):
+    """The run's Params as JSON (stored with every vector)."""
+    return np.array(json.dumps(asdict(p)))
+
+
+def write_trace_case(T, name):
+    (ql, qr), ion = TRACE_CASES[name]
+    p = case_params(T, name)
+    solid, mask, i0, i1 = T.build_geometry(p)
+    fields = T.setup_fields(p, solid, mask)
+    rho = np.zeros(solid.shape)
+    rho[i0 - 1, p.floor_t:p.floor_t + p.trench_d] = ql
+    rho[i1, p.floor_t:p.floor_t + p.trench_d] = qr
+    phi = fields["solve"]((-rho / T.EPS0).ravel()).reshape(solid.shape) + fields["external"]
+    ex, ey = T.compute_field(phi, ~solid, p)
+    wface = np.where(fields["cond"], 1.0, fields["eps"] / (1 + fields["eps"]))
+    barrier = (phi, wface) if p.wall_model == "barrier" else None
+    sampler = ion_sampler(T, **ion) if ion else None
+    x, y, vx, vy, qm, spc = T.inject(p, p.n_per_batch, solid.shape[1], np.random.default_rng(20),
+                                     ion_velocity=sampler)
+    record = {}
+    ci, ce, esc, lost = T.trace_cpu(x.copy(), y.copy(), vx.copy(), vy.copy(), qm.copy(), spc.copy(), ex, ey,
+                                    ~solid, p, barrier, record)
+    np.savez_compressed(HERE / f"trace_{name}.npz", params=params_json(p), solid=solid, mask=mask, rho=rho,
+                        phi=phi, Ex=ex, Ey=ey,
+                        wface=wface, x=x, y=y, vx=vx, vy=vy, qm=qm, spc=spc, cnt_i=ci, cnt_e=ce,
+                        escaped=esc, lost=np.int64(lost), status=record["status"], hit_cells=record["hit_cells"],
+                        endpoints=record["endpoints"], final_velocity=record["final_velocity"],
+                        selected=record["selected"],
+                        path_lengths=np.array([len(v) for v in record["paths"]]),
+                        paths=np.concatenate(record["paths"]) if record["paths"] else np.zeros((0, 2)))
+    print(f"trace_{name}: n={x.size} absorbed={int((record['status'] == 1).sum())} escaped={esc} lost={lost}")
+
+
+def write_setup_case(T, name):
+    """Leak / conductor operators on the small geometries (conductor mask,
+    sigma_s 1e-14), floating field mode, plus a fixed-voltage external field."""
+    p = replace(case_params(T, name + "_barrier"), sigma_s=1e-14)
+    solid, mask, i0, i1 = T.build_geometry(p)
+    fields = T.setup_fields(p, solid, mask)
+    leak = T.build_leakage(solid, fields["solve"], i0, i1, p)
+    ix, iy = np.nonzero(T.surface_cells(solid))
+    flat = ix * solid.shape[1] + iy
+    B = np.zeros((solid.size, flat.size))
+    B[flat, np.arange(flat.size)] = -1.0 / (p.dx ** 2 * T.EPS0)
+    green = fields["solve"](B)
+    cond = fields["cond"]
+    g_cond = T.conductor_response(fields["solve"], cond, p.dx)
+    C_cond = 1.0 / g_cond[cond][0]
+    pf = replace(p, field_mode="fixed", substrate_voltage=-20.0, mask_voltage=-10.0, top_voltage=5.0)
+    ext_fixed = T.setup_fields(pf, solid, mask)["external"]
+    steps = []
+    for n_i, n_e, drift in ((10, 0, 0.0), (10, 12, 0.0), (0, 7, 0.3), (25, 25, -0.2), (3, 40, 1.5)):
+        steps.append((n_i, n_e, drift, T.conductor_step(n_i, n_e, 1e-15, C_cond, p.electron_temp_eV, drift)))
+    np.savez_compressed(HERE / f"setup_{name}.npz", params=params_json(p), solid=solid, mask=mask, flat=flat,
+                        M=leak["M"], G=leak["G"],
+                        region=leak["region"], green=green, g_cond=g_cond, C_cond=np.float64(C_cond),
+                        external=fields["external"], external_fixed=ext_fixed, conductor_steps=np.array(steps))
+    print(f"setup_{name}: m={flat.size} C_cond={C_cond:.4g}")
+
+
+def write_run_ar5(T):
+    p = replace(case_params(T, "ar5"), n_per_batch=10_000, n_batches=50)
+    sampler = ion_sampler(T, kind="mono", energy_eV=100.0, sigma_deg=3.0)
+    original = T.inject
+    T.inject = lambda p_, n, ny, rng, ion_energy=None, ion_velocity=None: original(p_, n, ny, rng, ion_energy,
+                                                                                    sampler)
+    try:
+        t0 = time.time()
+        res = T.run(p, log=lambda s: None)
+    finally:
+        T.inject = original
+    arrays = dict(params=params_json(p), rho_raw=res["rho_raw"], phi=res["phi"], Ex=res["Ex"], Ey=res["Ey"],
+                  solid=res["solid"])
+    arrays.update({f"hist_{k}": v for k, v in res["hist"].items()})
+    arrays.update({f"charge_{k}": v for k, v in res["charge_monitor_hist"].items()})
+    np.savez_compressed(HERE / "run_ar5.npz", **arrays)
+    print(f"run_ar5: {time.time() - t0:.0f}s, bottom centre {res['hist']['bottom center'][-1]:.3f} V")
+    # A fresh batch traced in record mode through the charged field the run ends with.
+    solid = res["solid"]
+    fields = T.setup_fields(p, solid, res["mask"])
+    wface = np.where(fields["cond"], 1.0, fields["eps"] / (1 + fields["eps"]))
+    x, y, vx, vy, qm, spc = T.inject(replace(p, n_per_batch=2000), 2000, solid.shape[1],
+                                     np.random.default_rng(21), ion_velocity=sampler)
+    record = {}
+    ci, ce, esc, lost = T.trace_cpu(x.copy(), y.copy(), vx.copy(), vy.copy(), qm.copy(), spc.copy(),
+                                    res["Ex"], res["Ey"], ~solid, p, (res["phi"], wface), record)
+    np.savez_compressed(HERE / "trace_ar5_marched.npz", params=params_json(p), solid=solid, mask=res["mask"],
+                        rho=res["rho_raw"],
+                        phi=res["phi"], Ex=res["Ex"], Ey=res["Ey"], wface=wface, x=x, y=y, vx=vx, vy=vy, qm=qm,
+                        spc=spc, cnt_i=ci, cnt_e=ce, escaped=esc, lost=np.int64(lost), status=record["status"],
+                        hit_cells=record["hit_cells"], endpoints=record["endpoints"],
+                        final_velocity=record["final_velocity"], selected=record["selected"],
+                        path_lengths=np.array([len(v) for v in record["paths"]]),
+                        paths=np.concatenate(record["paths"]))
+    print(f"trace_ar5_marched: absorbed={int((record['status'] == 1).sum())} escaped={esc} lost={lost}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    T = load_reference(a.reference)
+    jobs = [*(f"trace_{k}" for k in TRACE_CASES), "setup_small", "setup_taper", "run_ar5"]
+    for job in jobs:
+        if a.only and a.only != job:
+            continue
+        if job.startswith("trace_"):
+            write_trace_case(T, job[6:])
+        elif job.startswith("setup_"):
+            write_setup_case(T, job[6:])
+        else:
+            write_run_ar5(T)
+
+
+if __name__ == "__main__":
+    main()

@@ -49,6 +49,9 @@ class Params:
     side_bc: str = "periodic"
     left_voltage: float = 0.0
     right_voltage: float = 0.0
+    probe_ions: int = 1000
+    probe_energy_eV: float = 100.0
+    probe_trajectories: int = 40
     flux: float = 1e20
     ion_mass_amu: float = 40.0
     ion_energy_eV: float = 100.0
