@@ -105,12 +105,90 @@ int th_trace_resident(th_tracer *t, int64_t offset, int64_t n, const th_trace_pa
                       int64_t *cnt_i /* nullable */, int64_t *cnt_e /* nullable */, th_tally *tally,
                       int8_t *status /* nullable */, int32_t *hit /* nullable */);
 
+/* Copies pool[offset, offset+n) back to the host (any pointer may be NULL). */
+int th_particles_read(th_tracer *t, int64_t offset, int64_t n, double *x, double *y, double *vx, double *vy,
+                      double *qm, int8_t *spc);
+
+/* The field the push currently uses, as Ex, Ey on the grid (nx*ny each; zero
+ * on solid cells, where the push never reads it). */
+int th_read_field(th_tracer *t, double *Ex, double *Ey);
+
 /* Per-surface-cell counts of the last trace: counts[0..m) ions, [m..2m) electrons. */
 int th_read_counts(th_tracer *t, int32_t *counts);
 
 /* GPU time of the push launches of the last trace call, from HIP events
  * recorded on the library's stream around them [ms]. */
 int th_last_push_ms(const th_tracer *t, double *ms);
+
+/* ---------------------------------------------------------------------------
+ * Device-side injection.  Writes n ions then n electrons into
+ * pool[offset, offset + 2n) with the distributions of the reference's host
+ * injection (trench_charging_2d.py:458-489), drawn from a Philox4x32-10
+ * stream keyed by (seed, batch, particle).  No reference counterpart (the
+ * reference draws on the host with PCG64): statistically equivalent, not the
+ * same draws.
+ */
+typedef struct th_inject_params {
+    int64_t n;             /* particles per species */
+    double lx, y0;         /* nx*dx, ny*dx - 1e-3*dx */
+    double qm_i, qm_e;     /* e/m_i, -e/m_e */
+    int32_t ion_kind;      /* 0: E = e_ion, lateral normal vth_i; 1: mono e_ion + polar angle;
+                              2: arcsine IED on [e_lo, e_hi] + polar angle */
+    int32_t reserved;
+    double e_ion, e_lo, e_hi, sigma_theta, vth_i, v_per_sqrt_ev, s_e;
+} th_inject_params;
+
+int th_inject(th_tracer *t, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t batch);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident charging state: the per-batch path after the push
+ * (trench_charging_2d.py:1239-1269) on the GPU.  Charge lives on the m surface
+ * cells of the tracer; the potential is phi = G Q + phi_ext on the kept rows,
+ * with G the Green's matrix (columns of the reference's LU solve, B at
+ * :795-797, merged where surface cells share one response).
+ */
+typedef struct th_charging th_charging;
+
+typedef struct th_charging_setup {
+    int32_t n_cols;              /* distinct Green's columns */
+    int32_t n_rows;              /* kept rows (cells whose potential the push reads) */
+    const int32_t *col_of;       /* [m] surface index -> column, -1 for none */
+    const int32_t *rows;         /* [n_rows] flat cell per row */
+    const double *green;         /* [n_rows][n_cols], V per C/m */
+    const double *external;      /* [nx*ny] potential of the applied voltages */
+    const double *wface;         /* [nx*ny] barrier face weight (:1085) */
+    const double *leak_M;        /* [m][m] backward-Euler leak operator (:768-803), NULL = no leak */
+    const double *leak_c;        /* [m] dt G phi_ext on surface cells (:1250-1251), NULL = zero */
+    const uint8_t *region;       /* [m] leak monitor region (:800-802), NULL = none */
+    const uint8_t *cond_s;       /* [m] 1 on conductor surface cells */
+    const double *g_cond_s;      /* [m] conductor response at surface cells (:807-812), floating only */
+    double q_macro, dx, C_cond, Te, dt_batch, top_voltage, substrate_voltage;
+    int32_t c_idx;               /* surface index of the conductor's charge cell (:1090) */
+    int32_t floating, driven, top_dirichlet;
+} th_charging_setup;
+
+typedef struct th_batch_diag {
+    th_tally tally;
+    double leak_dq;              /* charge leaving the leak monitor region [C/m] */
+    double drift, dv;            /* conductor drift and potential change [V] */
+    double n_i_cond, n_e_cond;   /* macro-particles absorbed on the conductor */
+    double push_ms, update_ms;   /* GPU time of the push and of the charge/field update */
+} th_batch_diag;
+
+int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **out);
+void th_charging_destroy(th_charging *c);
+
+/* Sets the surface charge density rho (m values, C/m^3) and recomputes the
+ * potential and the push's field records from it. */
+int th_charging_set_rho(th_charging *c, const double *rho_s);
+
+/* One batch: traces pool[offset, offset+n) through the current field, then
+ * deposit, leak, conductor, potential and field update, on the device. */
+int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, th_batch_diag *out);
+
+/* Reads the state: rho (m), potential on the grid (nx*ny; cells outside the
+ * kept rows read 0), and the last batch's per-surface-cell counts (2m). */
+int th_charging_read(th_charging *c, double *rho_s, double *phi /* nullable */, int32_t *counts /* nullable */);
 
 /* Device self-check of the FP64 arithmetic the push relies on: for each k,
  * out[4k..4k+3] = {hypot(a,b), sqrt(a), a/b, fmod(a,b)} computed on the GPU

@@ -455,6 +455,58 @@ def probes_for(solid, p, cond):
 
 
 # ------------------------------------------------------------------ batch loop
+def batch_context(p):
+    """Set-up of the per-batch update for a geometry (trench_charging_2d.py:1063-1094)."""
+    solid, mask, i0, i1 = build_geometry(p)
+    fields = setup_fields(p, solid, mask)
+    cond, solve = fields["cond"], fields["solve"]
+    ctx = dict(p=p, vac=~solid, fields=fields, cond=cond, conductor=fields["floating"],
+               driven=p.field_mode != "floating", solve=solve,
+               leak=build_leakage(solid, solve, p) if p.sigma_s > 0 else None,
+               q_macro=E_CHARGE * (p.flux * (solid.shape[0] * p.dx) * p.dt_batch / p.n_per_batch))
+    ctx["e0"], ctx["e1"] = opening_span(solid, p.floor_t + p.trench_d + p.mask_t - 1)
+    if ctx["conductor"]:
+        g = conductor_response(solve, cond, p.dx)
+        ctx.update(g_cond=g, C_cond=1.0 / g[cond][0],
+                   c_cell=np.flatnonzero(surface_cells(solid) & cond)[0])
+    return ctx
+
+
+def batch_update(ctx, rho, cnt_i, cnt_e):
+    """One batch after the push (trench_charging_2d.py:1239-1269): deposit,
+    surface leak, driven / floating conductor, Poisson solve and field.
+    Updates ``rho`` in place; returns (rho, phi, Ex, Ey, leak_ratio)."""
+    p, fields, cond = ctx["p"], ctx["fields"], ctx["cond"]
+    dx = p.dx
+    nx, ny = rho.shape
+    leak = ctx["leak"]
+    rho_prev = rho.copy() if ctx["conductor"] else None
+    drho = (cnt_i - cnt_e) * ctx["q_macro"] / dx ** 2
+    if cond.any():
+        drho[cond] = 0.0
+    rho += drho
+    leak_ratio = 0.0
+    if leak is not None:
+        Qs = rho.flat[leak["flat"]] * dx ** 2
+        Qs_old = Qs
+        if np.any(fields["external"]):
+            Qs = Qs + p.dt_batch * (leak["G"] @ fields["external"].ravel()[leak["flat"]])
+        Qs_new = leak["M"] @ Qs
+        rho.flat[leak["flat"]] = Qs_new / dx ** 2
+        dQ = Qs_old[leak["region"]].sum() - Qs_new[leak["region"]].sum()
+        leak_ratio = dQ / p.dt_batch / (E_CHARGE * p.flux * (ctx["e1"] - ctx["e0"]) * dx)
+    if ctx["driven"]:
+        rho[cond] = 0.0
+    if ctx["conductor"]:
+        drift = ((rho - rho_prev) * ctx["g_cond"]).sum() * dx ** 2
+        dV = conductor_step(cnt_i[cond].sum(), cnt_e[cond].sum(), ctx["q_macro"], ctx["C_cond"],
+                            p.electron_temp_eV, drift)
+        rho.flat[ctx["c_cell"]] += ctx["C_cond"] * (dV - drift) / dx ** 2
+    phi = ctx["solve"]((-rho / EPS0).ravel()).reshape(nx, ny) + fields["external"]
+    Ex, Ey = compute_field(phi, ctx["vac"], p, fields["volts"]["top"])
+    return rho, phi, Ex, Ey, leak_ratio
+
+
 def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_batch=None):
     """trench_charging_2d.py:1045-1369 (setup :1063-1223, loop :1225-1335,
     result :1337-1369), restated for dc bias, floating / fixed fields, no resume.
@@ -480,6 +532,10 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
     q_macro = E_CHARGE * (p.flux * (nx * dx) * p.dt_batch / p.n_per_batch)
     b0, b1 = opening_span(solid, p.floor_t)
     e0, e1 = opening_span(solid, p.floor_t + p.trench_d + p.mask_t - 1)
+    ctx = dict(p=p, vac=vac, fields=fields, cond=cond, conductor=conductor, driven=driven, leak=leak,
+               q_macro=q_macro, e0=e0, e1=e1, solve=solve)
+    if conductor:
+        ctx.update(g_cond=g_cond, C_cond=C_cond, c_cell=c_cell)
     probes = probes_for(solid, p, cond)
     hist = {k: [] for k in ("t", "ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "leak", *probes)}
     charge_hist = {"left": [], "right": []}
@@ -495,30 +551,7 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
         x, y, vx, vy, qm, spc = inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity)
         cnt_i, cnt_e, esc, n_lost = tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p,
                                            (phi, wface) if p.wall_model == "barrier" else None)
-        rho_prev = rho.copy() if conductor else None
-        drho = (cnt_i - cnt_e) * q_macro / dx ** 2
-        if cond.any():
-            drho[cond] = 0.0
-        rho += drho
-        leak_ratio = 0.0
-        if leak is not None:
-            Qs = rho.flat[leak["flat"]] * dx ** 2
-            Qs_old = Qs
-            if np.any(fields["external"]):
-                Qs = Qs + p.dt_batch * (leak["G"] @ fields["external"].ravel()[leak["flat"]])
-            Qs_new = leak["M"] @ Qs
-            rho.flat[leak["flat"]] = Qs_new / dx ** 2
-            dQ = Qs_old[leak["region"]].sum() - Qs_new[leak["region"]].sum()
-            leak_ratio = dQ / p.dt_batch / (E_CHARGE * p.flux * (e1 - e0) * dx)
-        if driven:
-            rho[cond] = 0.0
-        if conductor:
-            drift = ((rho - rho_prev) * g_cond).sum() * dx ** 2
-            dV = conductor_step(cnt_i[cond].sum(), cnt_e[cond].sum(), q_macro, C_cond,
-                                p.electron_temp_eV, drift)
-            rho.flat[c_cell] += C_cond * (dV - drift) / dx ** 2
-        phi = solve((-rho / EPS0).ravel()).reshape(nx, ny) + fields["external"]
-        Ex, Ey = compute_field(phi, vac, p, fields["volts"]["top"])
+        rho, phi, Ex, Ey, leak_ratio = batch_update(ctx, rho, cnt_i, cnt_e)
         hist["t"].append((b + 1) * p.dt_batch)
         for k, (ix, iy) in probes.items():
             hist[k].append(phi[ix, iy])

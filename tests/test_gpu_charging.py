@@ -1,0 +1,179 @@
+"""Device-resident charging path against the reference's golden vectors and
+the oracle's per-batch update."""
+from dataclasses import fields as dc_fields, replace
+
+import numpy as np
+import pytest
+
+from oracle import charging_ref as R
+from oracle import trace_ref
+from tests import cases
+from tests.test_golden import load, params_of, TRACE_CASES
+from trench01_amd import HipTracer, charging, configs, model
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def product_params_from(rp, **extra):
+    names = {f.name for f in dc_fields(model.Params)}
+    return replace(model.Params(), **{k: getattr(rp, k) for k in names if hasattr(rp, k)}, **extra)
+
+
+@pytest.mark.parametrize("name", [c for c in TRACE_CASES if c != "small_absorb"])
+def test_reference_golden_trace_on_gpu(name):
+    """The reference's own trace_cpu outputs (record mode; in barrier or tapered
+    runs record mode does not change which cell absorbs) against the HIP push."""
+    _case(f"golden trace {name}")
+    g = load(f"trace_{name}")
+    p = params_of(g)
+    vac = ~g["solid"]
+    barrier = (g["phi"], g["wface"]) if p.wall_model == "barrier" else None
+    t = HipTracer(vac)
+    ci, ce, esc, lost = t(g["x"], g["y"], g["vx"], g["vy"], g["qm"], g["spc"], g["Ex"], g["Ey"], vac, p, barrier,
+                          keep_status=True)
+    np.testing.assert_array_equal(ci, g["cnt_i"])
+    np.testing.assert_array_equal(ce, g["cnt_e"])
+    np.testing.assert_array_equal(esc, g["escaped"])
+    assert lost == int(g["lost"])
+    np.testing.assert_array_equal(t.last_status, g["status"])
+    ny = vac.shape[1]
+    cells = np.where(t.last_status[:, None] == 1, np.column_stack((t.last_hit // ny, t.last_hit % ny)), -1)
+    np.testing.assert_array_equal(cells, g["hit_cells"])
+
+
+STEP_CASES = {
+    "small-leak-floating": lambda: replace(cases.small_params(), sigma_s=1e-14, n_per_batch=2000),
+    "small-fixed-driven": lambda: replace(cases.small_params(), sigma_s=1e-14, field_mode="fixed",
+                                          substrate_voltage=-20.0, mask_voltage=-10.0, n_per_batch=2000),
+    "taper-dielectric-mask": lambda: replace(cases.taper_params(), sigma_s=1e-14, mask_type="dielectric",
+                                             n_per_batch=2000),
+    "ar5-floating": lambda: replace(cases.ar5_params(), n_per_batch=5000),
+    "ar8-leak-floating": lambda: replace(cases.ar8_params(), n_per_batch=5000),
+}
+
+
+@pytest.mark.parametrize("name", list(STEP_CASES))
+def test_device_batch_matches_oracle_update(name):
+    """One device batch from a charged state: the field the push reads is the
+    stencil of the device potential bit for bit, the push's counts equal the
+    oracle push on that field, and deposit + leak + conductor + potential agree
+    with the oracle's update (trench_charging_2d.py:1239-1269) to 1e-12 (charge)
+    and 1e-10 (potential) of their largest values."""
+    _case(f"device batch {name}")
+    rp = STEP_CASES[name]()
+    p = product_params_from(rp)
+    setup = charging.ChargingSetup(p)
+    dev = charging.DeviceCharging(setup)
+    geo = setup.geo
+    surf = geo.surface
+    rng = np.random.default_rng(5)
+    rho0 = rng.normal(0, 2e6, surf.size) * (1 - setup.cond_s)
+    if setup.fields.floating:
+        rho0[setup.c_idx] = 3e6
+    dev.set_rho(rho0)
+    ctx = R.batch_context(rp)
+    rho_full = np.zeros(geo.solid.shape)
+    rho_full.flat[surf] = rho0
+    phi_ref = ctx["solve"]((-rho_full / R.EPS0).ravel()).reshape(rho_full.shape) + ctx["fields"]["external"]
+    _, phi_dev, _ = dev.read()
+    rows = setup.rows
+    scale = np.abs(phi_ref).max()
+    np.testing.assert_allclose(phi_dev[rows], phi_ref.ravel()[rows], rtol=0, atol=1e-10 * scale)
+    # the push's field is the vacuum-only stencil of the device potential, bit for bit
+    phi_grid = phi_dev.reshape(geo.solid.shape)
+    ex_ref, ey_ref = R.compute_field(phi_grid, geo.vac, rp, ctx["fields"]["volts"]["top"])
+    ex_dev, ey_dev = dev.tracer.read_field()
+    np.testing.assert_array_equal(ex_dev[geo.vac], ex_ref[geo.vac])
+    np.testing.assert_array_equal(ey_dev[geo.vac], ey_ref[geo.vac])
+    # one batch
+    parts = model.inject(p, p.n_per_batch, geo.ny, np.random.default_rng(9))
+    dev.reserve(2 * p.n_per_batch)
+    dev.tracer.upload(0, *parts)
+    barrier = (phi_grid, setup.fields.wface) if p.wall_model == "barrier" else None
+    ref = trace_ref.trace_arrays(*parts, ex_ref, ey_ref, geo.vac, p.dx, p.cfl, p.max_steps, model.is_tapered(p),
+                                 barrier)
+    d = dev.step(0, 2 * p.n_per_batch)
+    rho1, phi1, counts = dev.read()
+    m = surf.size
+    np.testing.assert_array_equal(counts[:m], ref["cnt_i"].ravel()[surf])
+    np.testing.assert_array_equal(counts[m:], ref["cnt_e"].ravel()[surf])
+    assert d.tally.escaped[0] == ref["escaped"][0] and d.tally.escaped[1] == ref["escaped"][1]
+    assert d.tally.lost == ref["lost"] and d.tally.steps == ref["total_steps"]
+    rho_ref, phi_ref1, _, _, leak_ratio = R.batch_update(ctx, rho_full, ref["cnt_i"], ref["cnt_e"])
+    np.testing.assert_allclose(rho1, rho_ref.ravel()[surf], rtol=0, atol=1e-12 * np.abs(rho_ref).max())
+    assert np.abs(rho_ref.ravel()[np.setdiff1d(np.arange(rho_ref.size), surf)]).max(initial=0) == 0
+    np.testing.assert_allclose(phi1[rows], phi_ref1.ravel()[rows], rtol=0, atol=1e-10 * np.abs(phi_ref1).max())
+    if setup.leak is not None:
+        dq_ref = leak_ratio * p.dt_batch * (R.E_CHARGE * p.flux * (ctx["e1"] - ctx["e0"]) * p.dx)
+        assert d.leak_dq == pytest.approx(dq_ref, rel=1e-9, abs=1e-12 * np.abs(rho_ref).max() * p.dx ** 2)
+
+
+@pytest.mark.parametrize("kind", ["AR5", "AR8"])
+def test_device_injection_distributions(kind):
+    """Device injection draws the host injection's distributions (moments and
+    quantiles within 6 standard errors) and is reproducible per (seed, batch)."""
+    _case(f"device injection {kind}")
+    p, _ = configs.product_params(kind, n_per_batch=200_000)
+    setup = charging.ChargingSetup(replace(p, sigma_s=0.0))
+    dev = charging.DeviceCharging(setup)
+    n = p.n_per_batch
+    dev.reserve(4 * n)
+    dev.inject_device(0, n, seed=123, batch=7)
+    dev.inject_device(2 * n, n, seed=123, batch=7)
+    a = dev.tracer.read_particles(0, 2 * n)
+    b = dev.tracer.read_particles(2 * n, 2 * n)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    dev.inject_device(2 * n, n, seed=123, batch=8)
+    c = dev.tracer.read_particles(2 * n, 2 * n)
+    assert not np.array_equal(a[0], c[0])
+    x, y, vx, vy, qm, spc = a
+    Lx, Ly = p.nx * p.dx, setup.geo.ny * p.dx
+    np.testing.assert_array_equal(spc, np.r_[np.zeros(n), np.ones(n)])
+    np.testing.assert_array_equal(y, np.full(2 * n, Ly - 1e-3 * p.dx))
+    m_i = p.ion_mass_amu * model.AMU
+    np.testing.assert_array_equal(qm[:n], model.E_CHARGE / m_i)
+    np.testing.assert_array_equal(qm[n:], -model.E_CHARGE / model.M_E)
+    assert 0 <= x.min() and x.max() < Lx
+    se = Lx / np.sqrt(12 * n)
+    assert abs(x.mean() - Lx / 2) < 6 * se
+    s = np.sqrt(p.electron_temp_eV * model.E_CHARGE / model.M_E)
+    assert abs(vx[n:].std() / s - 1) < 6 / np.sqrt(2 * n)
+    assert abs(vx[n:].mean()) < 6 * s / np.sqrt(n)
+    assert abs(-vy[n:].mean() / (s * np.sqrt(np.pi / 2)) - 1) < 6 * np.sqrt(4 / np.pi - 1) / np.sqrt(n)
+    E = 0.5 * m_i * (vx[:n] ** 2 + vy[:n] ** 2) / model.E_CHARGE
+    theta = np.arctan2(vx[:n], -vy[:n])
+    sig = np.radians(p.ion_angle_deg)
+    assert abs(theta.std() / sig - 1) < 6 / np.sqrt(2 * n)
+    if p.ion_distribution == "mono":
+        np.testing.assert_allclose(E, p.ion_energy_eV, rtol=1e-12)
+    else:
+        lo, hi = p.ion_energy_lo_eV, p.ion_energy_hi_eV
+        assert lo <= E.min() and E.max() <= hi * (1 + 1e-12)
+        for q in (0.1, 0.25, 0.5, 0.75, 0.9):   # arcsine CDF F(E) = (2/pi) asin(sqrt((E-lo)/(hi-lo)))
+            frac = np.mean(E <= lo + (hi - lo) * np.sin(np.pi * q / 2) ** 2)
+            assert abs(frac - q) < 6 * np.sqrt(q * (1 - q) / n)
+
+
+def test_regression_run_on_device_matches_oracle_run():
+    """The whole 40-batch regression run (reference tests/test_regression.py:28-29)
+    with the batch on the device and host injection: batch 1 is identical to
+    the oracle's; later batches follow the same charging history within its
+    Monte Carlo noise (the device potential differs from the LU's in the last
+    bits, so trajectories decorrelate slowly)."""
+    _case("regression run")
+    rp = cases.regression_params()
+    p = product_params_from(rp)
+    res = charging.run(p, log=None)
+    ref = R.run(rp)
+    for k in ("ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost"):
+        assert res["hist"][k][0] == ref["hist"][k][0], k
+    for k in ("bottom center", "sidewall middle", "sidewall lower"):
+        a, b = res["hist"][k], ref["hist"][k]
+        print(k, a[-5:], b[-5:])
+        assert abs(a[-10:].mean() - b[-10:].mean()) <= 0.05 * max(1.0, abs(b[-10:].mean())), k
+    assert res["backend_used"] == "hip" and res["converged"] is False

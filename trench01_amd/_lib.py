@@ -29,6 +29,32 @@ class Tally(ctypes.Structure):
                 ("off_surface", ctypes.c_int64), ("cancelled", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class ChargingSetupC(ctypes.Structure):
+    """th_charging_setup."""
+    _fields_ = [("n_cols", ctypes.c_int32), ("n_rows", ctypes.c_int32), ("col_of", i32p),
+                ("rows", i32p), ("green", dp), ("external", dp), ("wface", dp),
+                ("leak_M", dp), ("leak_c", dp), ("region", u8p), ("cond_s", u8p),
+                ("g_cond_s", dp), ("q_macro", ctypes.c_double), ("dx", ctypes.c_double),
+                ("C_cond", ctypes.c_double), ("Te", ctypes.c_double), ("dt_batch", ctypes.c_double),
+                ("top_voltage", ctypes.c_double), ("substrate_voltage", ctypes.c_double),
+                ("c_idx", ctypes.c_int32), ("floating", ctypes.c_int32), ("driven", ctypes.c_int32),
+                ("top_dirichlet", ctypes.c_int32)]
+
+
+class BatchDiag(ctypes.Structure):
+    _fields_ = [("tally", Tally), ("leak_dq", ctypes.c_double), ("drift", ctypes.c_double),
+                ("dv", ctypes.c_double), ("n_i_cond", ctypes.c_double), ("n_e_cond", ctypes.c_double),
+                ("push_ms", ctypes.c_double), ("update_ms", ctypes.c_double)]
+
+
+class InjectParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("lx", ctypes.c_double), ("y0", ctypes.c_double),
+                ("qm_i", ctypes.c_double), ("qm_e", ctypes.c_double), ("ion_kind", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("e_ion", ctypes.c_double), ("e_lo", ctypes.c_double),
+                ("e_hi", ctypes.c_double), ("sigma_theta", ctypes.c_double), ("vth_i", ctypes.c_double),
+                ("v_per_sqrt_ev", ctypes.c_double), ("s_e", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); every symbol include/trench_hip.h declares.
 SIGNATURES = {
     "th_last_error": (ctypes.c_char_p, []),
@@ -45,9 +71,19 @@ SIGNATURES = {
     "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
     "th_trace_resident": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), i64p,
                                          i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_particles_read": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, dp, dp, dp, dp, dp, i8p]),
+    "th_read_field": (ctypes.c_int, [vp, dp, dp]),
     "th_read_counts": (ctypes.c_int, [vp, i32p]),
     "th_last_push_ms": (ctypes.c_int, [vp, dp]),
     "th_selftest_math": (ctypes.c_int, [vp, dp, dp, ctypes.c_int64, dp]),
+    "th_inject": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(InjectParams), ctypes.c_uint64,
+                                 ctypes.c_uint64]),
+    "th_charging_create": (ctypes.c_int, [vp, ctypes.POINTER(ChargingSetupC), ctypes.POINTER(vp)]),
+    "th_charging_destroy": (None, [vp]),
+    "th_charging_set_rho": (ctypes.c_int, [vp, dp]),
+    "th_charging_step": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.POINTER(TraceParams), ctypes.POINTER(BatchDiag)]),
+    "th_charging_read": (ctypes.c_int, [vp, dp, dp, i32p]),
 }
 
 _lib = None

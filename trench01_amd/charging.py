@@ -1,0 +1,341 @@
+"""Charging batch loop with the per-batch hot path on the GPU.
+
+``run(p, log, progress, stop)`` keeps the contract of the reference's
+``run()`` (trench_charging_2d.py:1045-1369): same batch order (inject, trace,
+deposit, leak, floating conductor, Poisson, field, record, steady test), same
+histories, probes, charge monitors, steady criterion and result keys.  The
+difference is where the batch runs: after a one-off host set-up (LU of the
+Poisson operator, the Green's matrix columns, the leak operator), every batch
+stays on the device -- push, deposit tallies, leak, conductor step,
+potential (phi = G Q + phi_ext) and field -- and only a few kilobytes of
+diagnostics come back per batch.
+
+Injection: ``"host"`` draws every batch with the reference's PCG64 stream
+and draw order (trench_charging_2d.py:458-489) and uploads it; ``"device"``
+draws it on the GPU (Philox, same distributions, different stream).
+"""
+import ctypes
+import time
+from dataclasses import asdict
+
+import numpy as np
+
+from . import _lib, model, monitors
+from .model import E_CHARGE, EPS0, M_E, AMU
+from .tracer import HipTracer
+
+STEADY_CRITERION_VERSION = 2
+
+
+class ChargingSetup:
+    """Host-side, once per geometry: geometry, LU, Green's matrix on the rows
+    the push reads, leak operator, floating-conductor response, monitors."""
+
+    def __init__(self, p):
+        if p.wall_model not in model.WALL_MODELS:
+            raise ValueError(f"wall_model must be one of {model.WALL_MODELS}: {p.wall_model!r}")
+        self.p = p
+        self.geo = geo = model.Geometry(p)
+        self.fields = fields = model.Fields(p, geo)
+        N = geo.solid.size
+        surf = geo.surface
+        m = surf.size
+        cond_flat = fields.cond.ravel()
+        self.cond_s = cond_flat[surf].astype(np.uint8)
+        # One Green's column per distinct surface response: a floating
+        # conductor's cells share the conductor's column; a driven conductor's
+        # cells carry no charge (:1257-1258) and get none.
+        col_of = np.full(m, -1, dtype=np.int32)
+        reps = []
+        cond_col = -1
+        for k in range(m):
+            if self.cond_s[k]:
+                if fields.floating:
+                    if cond_col < 0:
+                        cond_col = len(reps)
+                        reps.append(k)
+                    col_of[k] = cond_col
+            else:
+                col_of[k] = len(reps)
+                reps.append(k)
+        self.col_of = col_of
+        self.cond_col = cond_col
+        B = np.zeros((N, len(reps)))
+        B[surf[reps], np.arange(len(reps))] = -1.0 / (p.dx ** 2 * EPS0)   # 1 C/m per column (:795-797)
+        G_full = fields.solve(B)
+        # rows the push reads: vacuum cells (field) and surface cells (barrier)
+        keep = geo.vac.ravel().copy()
+        keep[surf] = True
+        self.rows = np.flatnonzero(keep).astype(np.int32)
+        self.green = np.ascontiguousarray(G_full[self.rows])
+        # surface response P (m x m) for the leak operator (:795-798)
+        P = np.zeros((m, m))
+        has = col_of >= 0
+        P[:, has] = G_full[surf][:, col_of[has]]
+        self.leak = self._leakage(P) if p.sigma_s > 0 else None
+        self.g_cond = G_full[:, cond_col].reshape(geo.solid.shape) if fields.floating else None
+        if fields.floating:
+            self.C_cond = 1.0 / self.g_cond[fields.cond][0]                      # :1089
+            c_cell = np.flatnonzero(geo.surface_mask() & fields.cond)[0]        # :1090
+            self.c_idx = int(geo.smap[c_cell])
+        else:
+            self.C_cond, self.c_idx = 0.0, -1
+        self.q_macro = E_CHARGE * (p.flux * (geo.nx * p.dx) * p.dt_batch / p.n_per_batch)   # :1093-1094
+        self.probes = monitors.probes(geo, p, fields.cond)
+        self.regions = monitors.charge_regions(geo, p)
+        self.b0, self.b1 = geo.opening_span(p.floor_t)
+        self.e0, self.e1 = geo.opening_span(p.floor_t + p.trench_d + p.mask_t - 1)
+        bottom = np.arange(self.b0, self.b1) * geo.ny + (p.floor_t - 1)
+        self.bottom_s = geo.smap[bottom]
+        if (self.bottom_s < 0).any():
+            raise RuntimeError("trench floor cells are expected to be surface cells")
+
+    def _leakage(self, P):
+        """Surface leak (:768-803): conductance graph of 4-connected surface cells
+        (g = sigma_s/dx, x periodic), backward-Euler operator M = inv(I - dt G P)."""
+        p, geo = self.p, self.geo
+        m = geo.surface.size
+        smap = geo.smap.reshape(geo.solid.shape)
+        g = p.sigma_s / p.dx
+        G = np.zeros((m, m))
+        for a, b in ((smap, np.roll(smap, -1, axis=0)), (smap[:, :-1], smap[:, 1:])):
+            sel = (a >= 0) & (b >= 0)
+            a, b = a[sel], b[sel]
+            np.add.at(G, (a, b), g)
+            np.add.at(G, (b, a), g)
+            np.add.at(G, (a, a), -g)
+            np.add.at(G, (b, b), -g)
+        M = np.linalg.inv(np.eye(m) - p.dt_batch * (G @ P))
+        iy = geo.surface % geo.ny
+        cut = p.floor_t + p.trench_d // 2
+        region = geo.inner_surface_mask().ravel()[geo.surface] & (iy < cut)
+        ext = self.fields.external.ravel()
+        c = p.dt_batch * (G @ ext[geo.surface]) if np.any(ext) else None
+        return dict(G=G, M=M, region=region, c=c)
+
+
+def ion_spec(p):
+    """Ion energy / angle distribution of a parameter set for device injection:
+    the reference's dc injection (energy ion_energy_eV plus a lateral thermal
+    normal of ion_temp_eV), or a polar-angle Gaussian with a monoenergetic or
+    arcsine-shaped IED (``ion_distribution`` "mono" / "arcsine")."""
+    kind = getattr(p, "ion_distribution", "thermal")
+    m_i = p.ion_mass_amu * AMU
+    spec = _lib.InjectParams(qm_i=E_CHARGE / m_i, qm_e=-E_CHARGE / M_E, e_ion=float(p.ion_energy_eV),
+                        e_lo=float(getattr(p, "ion_energy_lo_eV", 0.0)),
+                        e_hi=float(getattr(p, "ion_energy_hi_eV", 0.0)),
+                        sigma_theta=float(np.radians(getattr(p, "ion_angle_deg", 0.0))),
+                        vth_i=float(np.sqrt(p.ion_temp_eV * E_CHARGE / m_i)),
+                        v_per_sqrt_ev=float(np.sqrt(2.0 * E_CHARGE / m_i)),
+                        s_e=float(np.sqrt(p.electron_temp_eV * E_CHARGE / M_E)))
+    spec.ion_kind = {"thermal": 0, "mono": 1, "arcsine": 2}[kind]
+    return spec
+
+
+class DeviceCharging:
+    """The device-resident charging state of one geometry (th_charging)."""
+
+    def __init__(self, setup, tracer=None, device=0):
+        self.setup = s = setup
+        p = s.p
+        self.tracer = tracer or HipTracer(s.geo.vac, device)
+        flat = self.tracer.surface_cells()
+        if not np.array_equal(flat, s.geo.surface):
+            raise RuntimeError("tracer and host disagree on the surface cells")
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        self._keep = dict(col_of=np.ascontiguousarray(s.col_of, np.int32), rows=s.rows, green=s.green,
+                          external=f64(s.fields.external.ravel()), wface=f64(s.fields.wface.ravel()),
+                          leak_M=f64(s.leak["M"]) if s.leak else None,
+                          leak_c=f64(s.leak["c"]) if s.leak and s.leak["c"] is not None else None,
+                          region=np.ascontiguousarray(s.leak["region"], np.uint8) if s.leak else None,
+                          cond_s=np.ascontiguousarray(s.cond_s, np.uint8),
+                          g_cond_s=f64(s.g_cond.ravel()[s.geo.surface]) if s.g_cond is not None else None)
+        k = self._keep
+        su = _lib.ChargingSetupC(n_cols=int(s.green.shape[1]), n_rows=int(s.rows.size),
+                    col_of=_lib.ptr(k["col_of"], _lib.i32p), rows=_lib.ptr(k["rows"], _lib.i32p),
+                    green=_lib.ptr(k["green"], _lib.dp), external=_lib.ptr(k["external"], _lib.dp),
+                    wface=_lib.ptr(k["wface"], _lib.dp), leak_M=_lib.ptr(k["leak_M"], _lib.dp),
+                    leak_c=_lib.ptr(k["leak_c"], _lib.dp), region=_lib.ptr(k["region"], _lib.u8p),
+                    cond_s=_lib.ptr(k["cond_s"], _lib.u8p), g_cond_s=_lib.ptr(k["g_cond_s"], _lib.dp),
+                    q_macro=s.q_macro, dx=p.dx, C_cond=s.C_cond, Te=p.electron_temp_eV, dt_batch=p.dt_batch,
+                    top_voltage=s.fields.volts["top"], substrate_voltage=p.substrate_voltage, c_idx=s.c_idx,
+                    floating=int(s.fields.floating), driven=int(s.fields.driven and s.fields.cond.any()),
+                    top_dirichlet=int(p.top_bc == "dirichlet"))
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.th_charging_create(self.tracer._h, ctypes.byref(su), ctypes.byref(h)))
+        self._h = h
+        self.m = s.geo.surface.size
+        self.N = s.geo.solid.size
+        self.prm = self.tracer.params_for(p, p.wall_model == "barrier")
+        self.capacity = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().th_charging_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rho(self, rho_s):
+        rho_s = np.ascontiguousarray(rho_s, dtype=np.float64)
+        _lib.check(_lib.load().th_charging_set_rho(self._h, _lib.ptr(rho_s, _lib.dp)))
+
+    def reserve(self, n):
+        if n > self.capacity:
+            self.tracer.reserve(n)
+            self.capacity = n
+
+    def inject_device(self, offset, n_per_species, seed, batch):
+        p = self.setup.p
+        spec = ion_spec(p)
+        spec.n = int(n_per_species)
+        spec.lx = p.nx * p.dx
+        spec.y0 = self.setup.geo.ny * p.dx - 1e-3 * p.dx
+        _lib.check(_lib.load().th_inject(self.tracer._h, int(offset), ctypes.byref(spec), int(seed) & (2**64 - 1),
+                                         int(batch)))
+
+    def step(self, offset, n):
+        d = _lib.BatchDiag()
+        _lib.check(_lib.load().th_charging_step(self._h, int(offset), int(n), ctypes.byref(self.prm),
+                                                ctypes.byref(d)))
+        return d
+
+    def read(self, phi=True, counts=True):
+        rho = np.empty(self.m)
+        ph = np.empty(self.N) if phi else None
+        cn = np.empty(2 * self.m, np.int32) if counts else None
+        _lib.check(_lib.load().th_charging_read(self._h, _lib.ptr(rho, _lib.dp), _lib.ptr(ph, _lib.dp),
+                                                _lib.ptr(cn, _lib.i32p)))
+        return rho, ph, cn
+
+
+def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=None, ion_velocity=None,
+        device=0):
+    """Charging run (trench_charging_2d.py:1045-1369) with the batch on the GPU.
+    Returns the reference's result dict (plus ``timing``), or None if stopped
+    before the first batch."""
+    log = log or (lambda s: None)
+    if injection not in ("host", "device"):
+        raise ValueError("injection must be 'host' or 'device'")
+    if p.until_steady and (p.steady_window < 5 or p.steady_hold < 1 or p.steady_rtol <= 0
+                           or not np.isfinite(p.steady_charge_rtol) or p.steady_charge_rtol <= 0
+                           or not np.isfinite(p.steady_charge_atol) or p.steady_charge_atol < 0):
+        raise ValueError("steady_window >= 5, steady_hold >= 1, positive tolerances are required")
+    t_setup = time.perf_counter()
+    s = ChargingSetup(p)
+    geo, fields = s.geo, s.fields
+    nx, ny = geo.solid.shape
+    dev = DeviceCharging(s, device=device)
+    log(f"particle tracing: HIP ({dev.tracer.name}, device {device}, float64); grid {nx} x {ny}, "
+        f"{geo.surface.size} surface cells, {s.green.shape[1]} Green's columns on {s.rows.size} rows")
+    m = geo.surface.size
+    dev.set_rho(np.zeros(m))
+    rng = np.random.default_rng(p.seed)
+    n = 2 * p.n_per_batch
+    dev.reserve(n)
+    t_setup = time.perf_counter() - t_setup
+
+    probes = s.probes
+    hist = {k: [] for k in ("t", "ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "leak", *probes)}
+    charge_hist = {"left": [], "right": []}
+    charge_monitor_hist = {k: [] for k in (*s.regions, "surface absolute")}
+    frac_open = (s.b1 - s.b0) / nx
+    n_max = p.max_batches if p.until_steady else p.n_batches
+    n_ok, converged_at, stopped = 0, None, False
+    push_ms = update_ms = 0.0
+    rho_full = np.zeros(geo.solid.size)
+    phi = fields.external.ravel().copy()
+    t0 = time.time()
+    for b in range(n_max):
+        if stop is not None and stop.is_set():
+            stopped = True
+            break
+        if injection == "host":
+            dev.tracer.upload(0, *model.inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity))
+        else:
+            dev.inject_device(0, p.n_per_batch, p.seed, b)
+        d = dev.step(0, n)
+        if d.tally.off_surface:
+            raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
+        push_ms += d.push_ms
+        update_ms += d.update_ms
+        rho_s, phi, counts = dev.read()
+        rho_full[:] = 0.0
+        rho_full[geo.surface] = rho_s
+        hist["t"].append((b + 1) * p.dt_batch)
+        for k, (ix, iy) in probes.items():
+            hist[k].append(phi[ix * ny + iy])
+        norm = p.n_per_batch * frac_open
+        hist["ratio_i_bottom"].append(counts[s.bottom_s].sum() / norm)
+        hist["ratio_e_bottom"].append(counts[m + s.bottom_s].sum() / norm)
+        hist["esc_e"].append(d.tally.escaped[1] / p.n_per_batch)
+        hist["lost"].append(d.tally.lost / (2 * p.n_per_batch))
+        hist["leak"].append(d.leak_dq / p.dt_batch / (E_CHARGE * p.flux * (s.e1 - s.e0) * p.dx)
+                            if s.leak is not None else 0.0)
+        charges = monitors.sample_charges(rho_full, s.regions, p.dx)
+        for side in ("left", "right"):
+            charge_hist[side].append(charges[side])
+        for key, q in charges.items():
+            charge_monitor_hist[key].append(q)
+        if (b + 1) % 10 == 0 or b == 0:
+            log(f"[{b + 1:4d}/{n_max}] t={hist['t'][-1] * 1e6:7.1f} us  "
+                f"phi(bottom)={hist['bottom center'][-1]:7.2f} V  "
+                f"phi(side mid)={hist['sidewall middle'][-1]:7.2f} V  lost={hist['lost'][-1] * 100:.2f}%  "
+                f"({time.time() - t0:.1f}s)")
+        W = p.steady_window
+        check_info = None
+        if p.until_steady and (b + 1) % W == 0 and b + 1 >= 2 * W:
+            v_ok, name, dvolt, tol = monitors.check_steady(hist, probes, p)
+            charge = monitors.check_charge_steady(charge_monitor_hist, p)
+            ok = bool(v_ok and charge["ok"])
+            n_ok = min(n_ok + 1, p.steady_hold) if ok else 0
+            voltage = dict(ok=bool(v_ok), name=name, d=float(dvolt), tol=float(tol), unit="V", kind="voltage")
+            check_info = {**(charge if not charge["ok"] else voltage), "ok": ok, "n_ok": n_ok,
+                          "voltage": voltage, "charge": charge}
+            log(f"  [steady] voltage {'OK' if v_ok else 'NG'} ({name} {dvolt:+.3g} V, tol {tol:.3g}); "
+                f"charge {'OK' if charge['ok'] else 'NG'} -> {n_ok}/{p.steady_hold}")
+            if n_ok >= p.steady_hold:
+                converged_at = b + 1
+        if progress is not None:
+            progress(dict(batch=b + 1, start_batch=0, n_max=n_max, elapsed=time.time() - t0,
+                          phi=phi.reshape(nx, ny).copy(), rho=rho_full.reshape(nx, ny).copy(),
+                          hist={k: np.array(v) for k, v in hist.items()},
+                          charge_hist={k: np.array(v) for k, v in charge_hist.items()},
+                          check=check_info, converged=converged_at is not None, probes=list(probes)))
+        if converged_at is not None:
+            break
+    if not hist["t"]:
+        return None
+    wall = time.time() - t0
+    hist = {k: np.array(v) for k, v in hist.items()}
+    Wl = min(p.steady_window, len(hist["t"]))
+    steady = {k: float(hist[k][-Wl:].mean()) for k in probes}
+    rho = rho_full.reshape(nx, ny).copy()
+    # Final potential and field on the whole grid with the LU (the device keeps
+    # only the rows the push reads).
+    phi_full = fields.solve((-rho / EPS0).ravel()).reshape(nx, ny) + fields.external
+    Ex, Ey = model.compute_field(phi_full, geo.vac, p, fields.volts["top"])
+    shown = rho.copy()
+    if fields.cond.any():
+        induced = (fields.A @ phi_full.ravel()).reshape(rho.shape) - fields.boundary_rhs
+        shown[fields.cond] = (-EPS0 * induced)[fields.cond]
+    t_conv = converged_at * p.dt_batch if converged_at else np.nan
+    nb = len(hist["t"])
+    return dict(phi=phi_full, mean_phi=phi_full, rho=shown, Ex=Ex, Ey=Ey, rho_raw=rho,
+                external_phi=fields.external, voltages=fields.volts,
+                charge_hist={k: np.array(v) for k, v in charge_hist.items()},
+                charge_monitor_hist={k: np.array(v) for k, v in charge_monitor_hist.items()},
+                steady_criterion_version=STEADY_CRITERION_VERSION,
+                checkpoint=dict(version=1, batches_completed=nb,
+                                rng_state=rng.bit_generator.state if injection == "host" else None,
+                                phase_rng_state=None, steady_n_ok=n_ok),
+                start_batch=0, params=asdict(p), converged=converged_at is not None, backend_used="hip",
+                solid=geo.solid, mask=geo.mask, i0=geo.i0, i1=geo.i1, hist=hist, probes=probes, steady=steady,
+                t_conv=t_conv, stopped=stopped, circuit=None,
+                timing=dict(setup_s=t_setup, loop_s=wall, batches=nb, push_ms=push_ms, update_ms=update_ms,
+                            device_phi=phi.reshape(nx, ny)))

@@ -44,3 +44,19 @@ def ion_velocity_sampler(ion_mass_amu, kind, sigma_deg, energy_eV=None, e_lo=Non
         return v * np.sin(theta), -v * np.cos(theta)
 
     return sample
+
+
+def product_params(name, **changes):
+    """Parameters of a named workload ("AR5" or "AR8") for this package's
+    ``model.Params``, with the ion distribution as Params fields (used by the
+    device-side injection) and the matching host sampler."""
+    from dataclasses import replace
+    from .model import Params
+    geo, ion = {"AR5": (AR5, AR5_ION), "AR8": (AR8, AR8_ION)}[name]
+    extra = dict(ion_distribution=ion["kind"], ion_angle_deg=ion["sigma_deg"])
+    if ion["kind"] == "mono":
+        extra["ion_energy_eV"] = ion["energy_eV"]
+    else:
+        extra.update(ion_energy_lo_eV=ion["e_lo"], ion_energy_hi_eV=ion["e_hi"])
+    p = replace(Params(), **{**geo, **extra, **changes})
+    return p, ion_velocity_sampler(p.ion_mass_amu, **ion)

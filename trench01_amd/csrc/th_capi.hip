@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "../../include/trench_hip.h"
+#include "th_charge.hpp"
+#include "th_inject.hpp"
 #include "th_push.hpp"
 
 namespace {
@@ -293,59 +295,73 @@ int upload(th_tracer *t, int64_t off, const double *x, const double *y, const do
     return TH_OK;
 }
 
+// Clears the tallies and launches the push over pool[off, off+n) on the
+// tracer's stream.  With a stop flag the range is cut into chunks and the flag
+// is read between them; *traced is the end of the traced prefix.
+int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
+                 bool want_status, bool want_hit, int64_t *traced, th_tally *tally) {
+    TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
+    TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
+    TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
+    *traced = off;
+    TH_HIP(hipEventRecord(t->ev0, t->stream));
+    if (n > 0) {
+        if (want_status) TH_HIP(hipMemsetAsync(t->d_status + off, 3, (size_t)n, t->stream));
+        if (want_hit) TH_HIP(hipMemsetAsync(t->d_hit + off, 0xff, sizeof(int32_t) * (size_t)n, t->stream));
+        th::Grid g = make_grid(t, p);
+        th::Work w;
+        w.x = t->d_x; w.y = t->d_y; w.vx = t->d_vx; w.vy = t->d_vy; w.qm = t->d_qm; w.spc = t->d_spc;
+        w.status = want_status ? t->d_status : nullptr;
+        w.hit = want_hit ? t->d_hit : nullptr;
+        w.counts = t->d_counts;
+        w.overflow = t->d_overflow;
+        w.scal = t->d_scal;
+        w.max_steps = p->max_steps;
+        w.refill = p->refill > 0 ? std::min(p->refill, 64) : 16;
+        int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
+        int rc;
+        for (int64_t b = off; b < off + n; b += chunk) {
+            if (stop && *stop) {
+                tally->cancelled = 1;
+                break;
+            }
+            w.begin = b;
+            w.end = std::min(off + n, b + chunk);
+            TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
+            if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0))) return rc;
+            if (stop) TH_HIP(hipStreamSynchronize(t->stream));
+            *traced = w.end;
+        }
+    }
+    TH_HIP(hipEventRecord(t->ev1, t->stream));
+    return TH_OK;
+}
+
+void fill_tally(const unsigned long long *scal, int64_t untraced, th_tally *tally) {
+    tally->escaped[0] = (int64_t)scal[th::kScalEsc0];
+    tally->escaped[1] = (int64_t)scal[th::kScalEsc1];
+    tally->lost = (int64_t)scal[th::kScalLost] + untraced;
+    tally->steps = (int64_t)scal[th::kScalSteps];
+    tally->off_surface = (int64_t)scal[th::kScalOff];
+}
+
 // Traces pool[off, off+n) and collects the tallies (see th_trace).
 int trace_range(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
                 int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status, int32_t *hit) {
     memset(tally, 0, sizeof *tally);
     if (cnt_i) memset(cnt_i, 0, sizeof(int64_t) * (size_t)t->N);
     if (cnt_e) memset(cnt_e, 0, sizeof(int64_t) * (size_t)t->N);
-    t->push_ms = 0.0;
-    if (n == 0) {
-        TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
-        TH_HIP(hipStreamSynchronize(t->stream));
-        return TH_OK;
-    }
-    TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
-    TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
-    TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
-    if (status) TH_HIP(hipMemsetAsync(t->d_status + off, 3, (size_t)n, t->stream));
-    if (hit) TH_HIP(hipMemsetAsync(t->d_hit + off, 0xff, sizeof(int32_t) * (size_t)n, t->stream));
-
-    th::Grid g = make_grid(t, p);
-    th::Work w;
-    w.x = t->d_x; w.y = t->d_y; w.vx = t->d_vx; w.vy = t->d_vy; w.qm = t->d_qm; w.spc = t->d_spc;
-    w.status = status ? t->d_status : nullptr;
-    w.hit = hit ? t->d_hit : nullptr;
-    w.counts = t->d_counts;
-    w.overflow = t->d_overflow;
-    w.scal = t->d_scal;
-    w.max_steps = p->max_steps;
-    w.refill = p->refill > 0 ? std::min(p->refill, 64) : 16;
-    int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
     int64_t traced = off;
-    int rc;
-    TH_HIP(hipEventRecord(t->ev0, t->stream));
-    for (int64_t b = off; b < off + n; b += chunk) {
-        if (stop && *stop) {
-            tally->cancelled = 1;
-            break;
-        }
-        w.begin = b;
-        w.end = std::min(off + n, b + chunk);
-        TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-        if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0))) return rc;
-        if (stop) TH_HIP(hipStreamSynchronize(t->stream));
-        traced = w.end;
-    }
-    TH_HIP(hipEventRecord(t->ev1, t->stream));
+    int rc = launch_trace(t, off, n, p, stop, status != nullptr, hit != nullptr, &traced, tally);
+    if (rc) return rc;
     std::vector<uint32_t> counts(2 * (size_t)std::max(t->m, 1));
     unsigned long long scal[th::kScalCount];
     TH_HIP(hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream));
     if (cnt_i || cnt_e)
         TH_HIP(hipMemcpyAsync(counts.data(), t->d_counts, sizeof(uint32_t) * counts.size(),
                               hipMemcpyDeviceToHost, t->stream));
-    if (status) TH_HIP(hipMemcpyAsync(status, t->d_status + off, (size_t)n, hipMemcpyDeviceToHost, t->stream));
-    if (hit)
+    if (status && n) TH_HIP(hipMemcpyAsync(status, t->d_status + off, (size_t)n, hipMemcpyDeviceToHost, t->stream));
+    if (hit && n)
         TH_HIP(hipMemcpyAsync(hit, t->d_hit + off, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, t->stream));
     TH_HIP(hipStreamSynchronize(t->stream));
     float ms = 0.0f;
@@ -365,11 +381,7 @@ int trace_range(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, 
             }
         }
     }
-    tally->escaped[0] = (int64_t)scal[th::kScalEsc0];
-    tally->escaped[1] = (int64_t)scal[th::kScalEsc1];
-    tally->lost = (int64_t)scal[th::kScalLost] + (off + n - traced);
-    tally->steps = (int64_t)scal[th::kScalSteps];
-    tally->off_surface = (int64_t)scal[th::kScalOff];
+    fill_tally(scal, off + n - traced, tally);
     return TH_OK;
 }
 
@@ -465,6 +477,239 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
     if (e != hipSuccess) rc = fail(TH_EHIP, "selftest: %s", hipGetErrorString(e));
     (void)hipFree(d);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-resident charging state.
+
+struct th_charging {
+    th_tracer *t = nullptr;
+    th::ChargeDev d{};
+    int32_t *col_of = nullptr, *rows = nullptr, *surf = nullptr;
+    double *green = nullptr, *external = nullptr, *leak_M = nullptr, *leak_c = nullptr, *g_cond_s = nullptr;
+    uint8_t *region = nullptr, *cond_s = nullptr;
+    double *rho = nullptr, *rho_prev = nullptr, *qcol = nullptr, *phi = nullptr, *diag = nullptr;
+    hipEvent_t ev2 = nullptr;
+    int update_lds = 0;
+};
+
+namespace {
+
+template <class T>
+int upload_new(T **dst, const T *src, size_t count, hipStream_t s) {
+    int rc = dev_alloc(dst, count);
+    if (rc) return rc;
+    if (src) TH_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    else TH_HIP(hipMemsetAsync(*dst, 0, sizeof(T) * std::max<size_t>(count, 1), s));
+    return TH_OK;
+}
+
+// Potential from the column charges, then the push's field records.
+int launch_field(th_charging *c) {
+    th_tracer *t = c->t;
+    int waves = 4, rows_per_block = waves;
+    int blocks = (int)std::min<int64_t>((c->d.n_rows + rows_per_block - 1) / rows_per_block, (int64_t)t->cus * 8);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(th::green_gemv_kernel, dim3(blocks), dim3(64 * waves), sizeof(double) * c->d.n_cols,
+                       t->stream, c->d);
+    TH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(th::field_pack_kernel, dim3((t->N + 255) / 256), dim3(256), 0, t->stream, c->d);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void th_charging_destroy(th_charging *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->t->device);
+    (void)hipStreamSynchronize(c->t->stream);
+    for (void *p : {(void *)c->col_of, (void *)c->rows, (void *)c->surf, (void *)c->green, (void *)c->external,
+                    (void *)c->leak_M, (void *)c->leak_c, (void *)c->g_cond_s, (void *)c->region,
+                    (void *)c->cond_s, (void *)c->rho, (void *)c->rho_prev, (void *)c->qcol, (void *)c->phi,
+                    (void *)c->diag})
+        if (p) (void)hipFree(p);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    delete c;
+}
+
+int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **out) {
+    if (!t || !s || !out) return fail(TH_EINVAL, "invalid argument");
+    *out = nullptr;
+    const int m = t->m;
+    if (s->n_cols < 1 || s->n_rows < 1 || !s->col_of || !s->rows || !s->green || !s->external || !s->wface ||
+        !s->cond_s)
+        return fail(TH_EINVAL, "incomplete charging set-up");
+    if (s->floating && (!s->g_cond_s || s->c_idx < 0 || s->c_idx >= m || !(s->C_cond > 0.0)))
+        return fail(TH_EINVAL, "floating conductor needs g_cond_s, C_cond > 0 and a charge cell");
+    if (!(s->dx > 0.0) || !(s->Te > 0.0)) return fail(TH_EINVAL, "dx and Te must be positive");
+    for (int k = 0; k < m; ++k)
+        if (s->col_of[k] < -1 || s->col_of[k] >= s->n_cols) return fail(TH_EINVAL, "col_of out of range");
+    for (int r = 0; r < s->n_rows; ++r)
+        if (s->rows[r] < 0 || s->rows[r] >= t->N) return fail(TH_EINVAL, "row cell out of range");
+    if ((size_t)2 * m * sizeof(double) > 64 * 1024) return fail(TH_EINVAL, "too many surface cells for the update");
+    TH_HIP(hipSetDevice(t->device));
+    th_charging *c = new th_charging();
+    c->t = t;
+    auto bad = [&](int code) { th_charging_destroy(c); return code; };
+    hipStream_t st = t->stream;
+    int rc;
+    if ((rc = upload_new(&c->col_of, s->col_of, m, st)) || (rc = upload_new(&c->rows, s->rows, s->n_rows, st)) ||
+        (rc = upload_new(&c->surf, t->surf.data(), m, st)) ||
+        (rc = upload_new(&c->green, s->green, (size_t)s->n_rows * s->n_cols, st)) ||
+        (rc = upload_new(&c->external, s->external, t->N, st)) ||
+        (rc = upload_new(&c->cond_s, s->cond_s, m, st)) ||
+        (rc = upload_new(&c->region, s->region, m, st)) ||
+        (rc = upload_new(&c->g_cond_s, s->g_cond_s, m, st)) ||
+        (rc = upload_new(&c->rho, (const double *)nullptr, m, st)) ||
+        (rc = upload_new(&c->rho_prev, (const double *)nullptr, m, st)) ||
+        (rc = upload_new(&c->qcol, (const double *)nullptr, s->n_cols, st)) ||
+        (rc = upload_new(&c->phi, (const double *)nullptr, t->N, st)) ||
+        (rc = upload_new(&c->diag, (const double *)nullptr, th::kDiagCount, st)))
+        return bad(rc);
+    if (s->leak_M && (rc = upload_new(&c->leak_M, s->leak_M, (size_t)m * m, st))) return bad(rc);
+    if (s->leak_c && (rc = upload_new(&c->leak_c, s->leak_c, m, st))) return bad(rc);
+    if (hipMemcpyAsync(t->d_wface, s->wface, sizeof(double) * t->N, hipMemcpyHostToDevice, st) != hipSuccess)
+        return bad(fail(TH_EHIP, "wface upload failed"));
+    if (hipEventCreate(&c->ev2) != hipSuccess) return bad(fail(TH_EHIP, "hipEventCreate failed"));
+    th::ChargeDev &d = c->d;
+    d.m = m; d.n_cols = s->n_cols; d.n_rows = s->n_rows; d.nx = t->nx; d.ny = t->ny;
+    d.col_of = c->col_of; d.rows = c->rows; d.surf = c->surf; d.green = c->green; d.external = c->external;
+    d.leak_M = c->leak_M; d.leak_c = c->leak_c; d.region = c->region; d.cond_s = c->cond_s;
+    d.g_cond_s = c->g_cond_s; d.vac = t->d_vac; d.wface = t->d_wface;
+    d.rho = c->rho; d.rho_prev = c->rho_prev; d.qcol = c->qcol; d.phi = c->phi; d.diag = c->diag;
+    d.cells = t->d_cells; d.counts = t->d_counts;
+    d.q_macro = s->q_macro; d.dx = s->dx; d.C_cond = s->C_cond; d.Te = s->Te; d.dt_batch = s->dt_batch;
+    d.top_voltage = s->top_voltage; d.substrate_voltage = s->substrate_voltage;
+    d.c_idx = s->c_idx; d.floating = s->floating; d.driven = s->driven; d.has_leak = s->leak_M != nullptr;
+    d.top_dirichlet = s->top_dirichlet; d.leak_has_c = s->leak_c != nullptr;
+    c->update_lds = (int)(2 * (size_t)std::max(m, 1) * sizeof(double));
+    if (hipStreamSynchronize(st) != hipSuccess) return bad(fail(TH_EHIP, "set-up upload failed"));
+    *out = c;
+    return TH_OK;
+}
+
+int th_charging_set_rho(th_charging *c, const double *rho_s) {
+    if (!c || !rho_s) return fail(TH_EINVAL, "invalid argument");
+    th_tracer *t = c->t;
+    TH_HIP(hipSetDevice(t->device));
+    TH_HIP(hipMemcpyAsync(c->rho, rho_s, sizeof(double) * c->d.m, hipMemcpyHostToDevice, t->stream));
+    // column charges on the host (one-off), then potential and records on the device
+    std::vector<int32_t> col(c->d.m);
+    TH_HIP(hipMemcpyAsync(col.data(), c->col_of, sizeof(int32_t) * c->d.m, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    std::vector<double> q(c->d.n_cols, 0.0);
+    const double dx2 = c->d.dx * c->d.dx;
+    for (int k = 0; k < c->d.m; ++k)
+        if (col[k] >= 0) q[col[k]] += rho_s[k] * dx2;
+    TH_HIP(hipMemcpyAsync(c->qcol, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice, t->stream));
+    int rc = launch_field(c);
+    if (rc) return rc;
+    TH_HIP(hipStreamSynchronize(t->stream));
+    t->fields = true;
+    t->barrier_fields = true;
+    return TH_OK;
+}
+
+int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, th_batch_diag *out) {
+    if (!c || !p || !out || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
+    th_tracer *t = c->t;
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    TH_HIP(hipSetDevice(t->device));
+    memset(out, 0, sizeof *out);
+    int64_t traced = offset;
+    if ((rc = launch_trace(t, offset, n, p, nullptr, false, false, &traced, &out->tally))) return rc;
+    hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
+    TH_HIP(hipGetLastError());
+    if ((rc = launch_field(c))) return rc;
+    TH_HIP(hipEventRecord(c->ev2, t->stream));
+    unsigned long long scal[th::kScalCount];
+    double diag[th::kDiagCount];
+    TH_HIP(hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipMemcpyAsync(diag, c->diag, sizeof diag, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    fill_tally(scal, offset + n - traced, &out->tally);
+    out->leak_dq = diag[th::kDiagLeakDQ];
+    out->drift = diag[th::kDiagDrift];
+    out->dv = diag[th::kDiagDV];
+    out->n_i_cond = diag[th::kDiagNi];
+    out->n_e_cond = diag[th::kDiagNe];
+    float a = 0.0f, b = 0.0f;
+    TH_HIP(hipEventElapsedTime(&a, t->ev0, t->ev1));
+    TH_HIP(hipEventElapsedTime(&b, t->ev1, c->ev2));
+    out->push_ms = a;
+    out->update_ms = b;
+    t->push_ms = a;
+    return TH_OK;
+}
+
+int th_charging_read(th_charging *c, double *rho_s, double *phi, int32_t *counts) {
+    if (!c) return fail(TH_EINVAL, "invalid argument");
+    th_tracer *t = c->t;
+    TH_HIP(hipSetDevice(t->device));
+    if (rho_s) TH_HIP(hipMemcpyAsync(rho_s, c->rho, sizeof(double) * c->d.m, hipMemcpyDeviceToHost, t->stream));
+    if (phi) TH_HIP(hipMemcpyAsync(phi, c->phi, sizeof(double) * t->N, hipMemcpyDeviceToHost, t->stream));
+    if (counts)
+        TH_HIP(hipMemcpyAsync(counts, t->d_counts, sizeof(uint32_t) * 2 * t->m, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    return TH_OK;
+}
+
+int th_inject(th_tracer *t, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t batch) {
+    if (!t || !ip || offset < 0 || ip->n < 0) return fail(TH_EINVAL, "invalid argument");
+    if (ip->ion_kind < 0 || ip->ion_kind > 2) return fail(TH_EINVAL, "unknown ion distribution %d", ip->ion_kind);
+    if (offset + 2 * ip->n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    if (ip->n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    th::InjectDev d;
+    d.n = ip->n; d.lx = ip->lx; d.y0 = ip->y0; d.qm_i = ip->qm_i; d.qm_e = ip->qm_e; d.ion_kind = ip->ion_kind;
+    d.e_ion = ip->e_ion; d.e_lo = ip->e_lo; d.e_hi = ip->e_hi; d.sigma_theta = ip->sigma_theta;
+    d.vth_i = ip->vth_i; d.v_per_sqrt_ev = ip->v_per_sqrt_ev; d.s_e = ip->s_e; d.seed = seed; d.batch = batch;
+    d.x = t->d_x + offset; d.y = t->d_y + offset; d.vx = t->d_vx + offset; d.vy = t->d_vy + offset;
+    d.qm = t->d_qm + offset; d.spc = t->d_spc + offset;
+    int64_t blocks = (2 * ip->n + 255) / 256;
+    hipLaunchKernelGGL(th::inject_kernel, dim3((unsigned)blocks), dim3(256), 0, t->stream, d);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int th_particles_read(th_tracer *t, int64_t offset, int64_t n, double *x, double *y, double *vx, double *vy,
+                      double *qm, int8_t *spc) {
+    if (!t || offset < 0 || n < 0 || offset + n > t->cap) return fail(TH_EINVAL, "invalid range");
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    size_t b8 = sizeof(double) * (size_t)n;
+    if (x) TH_HIP(hipMemcpyAsync(x, t->d_x + offset, b8, hipMemcpyDeviceToHost, t->stream));
+    if (y) TH_HIP(hipMemcpyAsync(y, t->d_y + offset, b8, hipMemcpyDeviceToHost, t->stream));
+    if (vx) TH_HIP(hipMemcpyAsync(vx, t->d_vx + offset, b8, hipMemcpyDeviceToHost, t->stream));
+    if (vy) TH_HIP(hipMemcpyAsync(vy, t->d_vy + offset, b8, hipMemcpyDeviceToHost, t->stream));
+    if (qm) TH_HIP(hipMemcpyAsync(qm, t->d_qm + offset, b8, hipMemcpyDeviceToHost, t->stream));
+    if (spc) TH_HIP(hipMemcpyAsync(spc, t->d_spc + offset, (size_t)n, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    return TH_OK;
+}
+
+int th_read_field(th_tracer *t, double *Ex, double *Ey) {
+    if (!t || !Ex || !Ey) return fail(TH_EINVAL, "invalid argument");
+    if (!t->fields) return fail(TH_ESTATE, "no field has been set");
+    TH_HIP(hipSetDevice(t->device));
+    std::vector<double> rec(4 * (size_t)t->N);
+    TH_HIP(hipMemcpy(rec.data(), t->d_cells, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < t->N; ++c) {
+        Ex[c] = rec[4 * (size_t)c];
+        Ey[c] = rec[4 * (size_t)c + 1];
+    }
+    return TH_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,210 @@
+// Per-batch charge and field update on the GPU (after the push):
+//
+//   deposit   rho_k += (n_i,k - n_e,k) q / dx^2 on surface cells, conductor
+//             cells excluded (trench_charging_2d.py:1239-1243)
+//   leak      Q = rho dx^2 (+ dt G phi_ext);  Q' = M Q;  rho = Q' / dx^2   (:1246-1256)
+//   driven    rho[conductor] = 0                                             (:1257-1258)
+//   floating  drift = sum (rho - rho_prev) g_cond dx^2; dV = conductor_step;
+//             rho[c_cell] += C (dV - drift) / dx^2                          (:1260-1266)
+//   potential phi = G Q + phi_ext on the kept rows, G the Green's matrix
+//             (one column per distinct surface response; the reference's
+//             solve(-rho/eps0) restricted to the rows the push reads)       (:1268, :795-797)
+//   field     E = -grad(phi) from vacuum cells only, packed with vac and phi
+//             into the push's cell records                                  (:429-454, :1269)
+//
+// Charge state lives on the m surface cells: deposits, leak and the
+// conductor's charge all land there, so rho is an m-vector.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace th {
+
+constexpr int kUpdBlock = 1024;   // the m-vector update runs in one workgroup
+
+struct ChargeDev {
+    int m, n_cols, n_rows, nx, ny;
+    const int32_t *col_of;      // [m] surface index -> Green's column, -1: none
+    const int32_t *rows;        // [n_rows] flat cell of each kept row
+    const int32_t *surf;        // [m] flat cell of each surface index
+    const double *green;        // [n_rows][n_cols]
+    const double *external;     // [N]
+    const double *leak_M;       // [m][m] or null
+    const double *leak_c;       // [m] or null
+    const uint8_t *region;      // [m] leak-monitor region (bottom half of the trench)
+    const uint8_t *cond_s;      // [m] conductor surface cells
+    const double *g_cond_s;     // [m] conductor response at surface cells (floating)
+    const uint8_t *vac;         // [N]
+    const double *wface;        // [N]
+    double *rho;                // [m] charge density [C/m^3]
+    double *rho_prev;           // [m]
+    double *qcol;               // [n_cols] charge per Green's column [C/m]
+    double *phi;                // [N] potential (kept rows; others stay 0)
+    double *diag;               // per-batch scalars, see kDiag*
+    double4 *cells;             // push records {Ex*vac, Ey*vac, vac, phi}
+    const uint32_t *counts;     // [2][m] deposit tallies of the batch
+    double q_macro, dx, C_cond, Te, dt_batch, top_voltage, substrate_voltage;
+    int c_idx, floating, driven, has_leak, top_dirichlet, leak_has_c;
+};
+
+enum : int { kDiagLeakDQ = 0, kDiagDrift, kDiagDV, kDiagNi, kDiagNe, kDiagCount };
+
+// numpy.logaddexp for finite arguments.
+__device__ __forceinline__ double logaddexp_d(double a, double b) {
+    if (a == b) return a + 0.693147180559945309417232121458176568;
+    double t = a - b;
+    return t > 0.0 ? a + log1p(exp(-t)) : b + log1p(exp(t));
+}
+
+// Exact one-batch solution of the floating-conductor ODE
+// dv/dt = (q n_i/C + drift - q n_e/C exp(v/Te)) / dt (trench_charging_2d.py:815-830).
+__device__ __forceinline__ double conductor_step_d(double n_i, double n_e, double q, double C, double Te,
+                                                   double drift) {
+    double x_i = q * n_i / C + drift, x_e = q * n_e / C;
+    if (n_e == 0.0) return x_i;
+    double k = x_i / Te;
+    double frac = k != 0.0 ? -expm1(-k) / k : 1.0;
+    return -Te * logaddexp_d(-k, log(x_e / Te * frac));
+}
+
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+    // wave reduction then across the 16 waves of the workgroup
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += scratch[w];
+        scratch[32] = s;
+    }
+    __syncthreads();
+    return scratch[32];
+}
+
+// Deposit, leak, conductor and column charges: one workgroup over the m-vector.
+__global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {
+    extern __shared__ double sh[];          // q_old[m], q_new[m]
+    __shared__ double scratch[40];
+    double *q_old = sh, *q_new = sh + d.m;
+    const double dx2 = d.dx * d.dx;
+    const int m = d.m;
+    // deposit
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        double r = d.rho[k];
+        d.rho_prev[k] = r;
+        double drho = (double)((int64_t)d.counts[k] - (int64_t)d.counts[m + k]) * d.q_macro / dx2;
+        if (d.cond_s[k]) drho = 0.0;
+        r += drho;
+        d.rho[k] = r;
+    }
+    __syncthreads();
+    // leak (backward Euler over the surface conductance network)
+    double leak_dq = 0.0;
+    if (d.has_leak) {
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            double q = d.rho[k] * dx2;
+            q_old[k] = q;
+            q_new[k] = d.leak_has_c ? q + d.leak_c[k] : q;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const double *row = d.leak_M + (int64_t)k * m;
+            double acc = 0.0;
+            for (int j = 0; j < m; ++j) acc += row[j] * q_new[j];
+            d.rho[k] = acc / dx2;
+            q_old[k] = d.region[k] ? q_old[k] - acc : 0.0;   // region outflow terms
+        }
+        __syncthreads();
+        double part = 0.0;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) part += q_old[k];
+        leak_dq = block_sum(part, scratch);
+    }
+    if (d.driven)
+        for (int k = threadIdx.x; k < m; k += blockDim.x)
+            if (d.cond_s[k]) d.rho[k] = 0.0;
+    __syncthreads();
+    if (d.floating) {
+        double part = 0.0, ni = 0.0, ne = 0.0;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            part += (d.rho[k] - d.rho_prev[k]) * d.g_cond_s[k];
+            if (d.cond_s[k]) {
+                ni += (double)d.counts[k];
+                ne += (double)d.counts[m + k];
+            }
+        }
+        double drift = block_sum(part, scratch) * dx2;
+        double n_i = block_sum(ni, scratch), n_e = block_sum(ne, scratch);
+        if (threadIdx.x == 0) {
+            double dv = conductor_step_d(n_i, n_e, d.q_macro, d.C_cond, d.Te, drift);
+            d.rho[d.c_idx] += d.C_cond * (dv - drift) / dx2;
+            d.diag[kDiagDrift] = drift;
+            d.diag[kDiagDV] = dv;
+            d.diag[kDiagNi] = n_i;
+            d.diag[kDiagNe] = n_e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.diag[kDiagLeakDQ] = leak_dq;
+    // charge per Green's column
+    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) d.qcol[c] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < m; ++k)
+            if (d.col_of[k] >= 0) d.qcol[d.col_of[k]] += d.rho[k] * dx2;
+}
+
+// phi[rows[r]] = sum_c G[r][c] qcol[c] + external[rows[r]]: one wave per row,
+// lanes stride the row (coalesced), column charges staged in LDS.
+__global__ void __launch_bounds__(256) green_gemv_kernel(ChargeDev d) {
+    extern __shared__ double q[];
+    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) q[c] = d.qcol[c];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int waves = blockDim.x >> 6;
+    for (int r = blockIdx.x * waves + (threadIdx.x >> 6); r < d.n_rows; r += gridDim.x * waves) {
+        const double *g = d.green + (int64_t)r * d.n_cols;
+        double acc = 0.0;
+        for (int c = lane; c < d.n_cols; c += 64) acc += g[c] * q[c];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+        if (lane == 0) {
+            int cell = d.rows[r];
+            d.phi[cell] = acc + d.external[cell];
+        }
+    }
+}
+
+// Vacuum-only gradient and packing of the push records (trench_charging_2d.py:429-454).
+__global__ void field_pack_kernel(ChargeDev d) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int N = d.nx * d.ny;
+    if (c >= N) return;
+    const int ny = d.ny, nx = d.nx;
+    const int ix = c / ny, iy = c - ix * ny;
+    const bool v = d.vac[c] != 0;
+    const double dx = d.dx;
+    const double pc = d.phi[c];
+    double ex = 0.0, ey = 0.0;
+    if (v) {
+        int cw = ((ix + nx - 1) % nx) * ny + iy, ce = ((ix + 1) % nx) * ny + iy;
+        bool vw = d.vac[cw] != 0, ve = d.vac[ce] != 0;
+        double g = 0.0;
+        if (vw && ve) g = (d.phi[ce] - d.phi[cw]) / (2 * dx);
+        else if (ve) g = (d.phi[ce] - pc) / dx;
+        else if (vw) g = (pc - d.phi[cw]) / dx;
+        ex = -g;
+        bool vs = iy > 0 && d.vac[c - 1] != 0;
+        bool vn = iy < ny - 1 ? d.vac[c + 1] != 0 : true;
+        double ps = iy > 0 ? d.phi[c - 1] : 2 * d.substrate_voltage - pc;
+        double pn = iy < ny - 1 ? d.phi[c + 1] : (d.top_dirichlet ? 2 * d.top_voltage - pc : pc);
+        g = 0.0;
+        if (vs && vn) g = (pn - ps) / (2 * dx);
+        else if (vn) g = (pn - pc) / dx;
+        else if (vs) g = (pc - ps) / dx;
+        ey = -g;
+    }
+    d.cells[c] = make_double4(v ? ex : 0.0, v ? ey : 0.0, v ? 1.0 : 0.0, pc);
+}
+
+}  // namespace th

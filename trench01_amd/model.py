@@ -57,6 +57,15 @@ class Params:
     ion_energy_eV: float = 100.0
     ion_temp_eV: float = 0.5
     electron_temp_eV: float = 3.0
+    # Ion distribution for device-side injection (no reference field: the
+    # reference takes such distributions through its ion_velocity hook):
+    # "thermal" (ion_energy_eV + lateral ion_temp_eV, the reference's dc
+    # injection), "mono" (ion_energy_eV) or "arcsine" (between the lo / hi
+    # energies), the last two with a Gaussian polar angle of ion_angle_deg.
+    ion_distribution: str = "thermal"
+    ion_angle_deg: float = 0.0
+    ion_energy_lo_eV: float = 0.0
+    ion_energy_hi_eV: float = 0.0
     bias: str = "dc"
     sigma_s: float = 1e-15
     until_steady: bool = False

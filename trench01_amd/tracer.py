@@ -183,6 +183,21 @@ class HipTracer:
                  steps=int(tally.steps), off_surface=int(tally.off_surface))
         return t, ((ci, ce) if with_counts else None), float(ms.value)
 
+    def read_particles(self, offset, n):
+        """Copy pool[offset, offset+n) back: (x, y, vx, vy, qm, spc)."""
+        out = [np.empty(n) for _ in range(5)] + [np.empty(n, np.int8)]
+        _lib.check(_lib.load().th_particles_read(self._h, int(offset), int(n),
+                                                 *(_lib.ptr(a, _lib.dp) for a in out[:5]),
+                                                 _lib.ptr(out[5], _lib.i8p)))
+        return tuple(out)
+
+    def read_field(self):
+        """The push's current field (Ex, Ey) on the grid; zero on solid cells."""
+        nx, ny = self.vac.shape
+        ex, ey = np.empty((nx, ny)), np.empty((nx, ny))
+        _lib.check(_lib.load().th_read_field(self._h, _lib.ptr(ex, _lib.dp), _lib.ptr(ey, _lib.dp)))
+        return ex, ey
+
     def selftest_math(self, a, b):
         """Device hypot / sqrt / divide / fmod on arrays (for arithmetic pinning)."""
         a = np.ascontiguousarray(a, dtype=np.float64)
@@ -219,3 +234,4 @@ def trace(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None, stop
         return tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier, record, stop, rf=rf)
     finally:
         tracer.close()
+
