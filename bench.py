@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
 """Throughput of the per-batch charging hot path on one MI355X.
 
-One step = one batch of the AR 8 production workload (120x320 cells, trench
-20x160 with 2 / 5 degree tapers, conductive mask 40 cells, barrier wall model):
-1e6 particles (5e5 ions from the arcsine 40-260 eV IED with 4 degree angular
-spread, 5e5 electrons at Te = 4 eV) traced through the frozen field with
-LDS-privatised deposit tallies.  Particle states are generated on the host
-(the reference's injection) and uploaded before the timed region, one
-distinct batch per step, so the timed work starts from HBM.
+Workload: the AR 8 production case -- 120x320 cells (dx 5 nm), trench 20x160
+with 2 / 5 degree tapers, conductive floating mask of 40 cells, barrier wall
+model, surface leak sigma_s 1e-15 S; 1e6 particles per batch (5e5 ions from
+the arcsine 40-260 eV IED with a 4 degree polar spread, 5e5 electrons at
+Te = 4 eV).
 
-Prints one JSON line (metric, value, roofline of the push kernel, and the CPU
-baseline: the oracle's C restatement of the push on this host's cores, on a
-bounded sample of the same workload).
+One step = one batch, entirely on the device: injection (Philox), FP64 push
+with LDS deposit tallies, deposit, leak, floating-conductor step, potential
+(Green's matrix GEMV) and field, plus the batch's diagnostics read back for
+the host loop.  Timed from a pre-charged state (the same loop run first with
+2e4 particles per species per batch, which deposits the same charge per
+batch), so path lengths are those of a charged trench.
+
+Prints one JSON line: the metric, the roofline of the push kernel (FP64
+FLOP at 69 per particle-step over its HIP-event time), and the CPU baseline
+(the oracle's C push on this host's cores over one whole batch, same states,
+same field).  ``--steady`` adds the wall time of plain time-marching to the
+steady-state criterion at this workload.
 """
 import argparse
 import json
@@ -27,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 FLOP_PER_STEP = 69          # algorithmic FP64 FLOP per particle-step
 BYTES_PER_PARTICLE = 78     # algorithmic HBM bytes per particle (state in/out, species, status, hit)
-FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector / matrix peak
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector (= FP64 matrix) peak
 HBM_PEAK_GBS = 8000.0
 
 
@@ -37,27 +44,43 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-per-batch", type=int, default=500_000, help="particles per species per batch")
-    ap.add_argument("--cpu-sample", type=int, default=20_000, help="particles per species in the CPU sample")
+    ap.add_argument("--precharge", type=int, default=600, help="pre-charge batches (2e4 per species)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores this process may use")
+    ap.add_argument("--steady", action="store_true", help="also time plain time-marching to steady state")
+    ap.add_argument("--max-batches", type=int, default=5000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
 
 
-def workload(n_per_batch):
-    from trench01_amd import configs, model
-    p = replace(model.Params(), **{**configs.AR8, "n_per_batch": n_per_batch})
-    geo = model.Geometry(p)
-    fields = model.Fields(p, geo)
-    # Frozen field of a charged trench: positive charge on the left dielectric
-    # wall, negative on the right (the reference's CUDA parity field,
-    # tests/test_cuda.py:52-57), on top of the applied boundary potentials.
-    rho = np.zeros(geo.solid.shape)
-    rho[geo.i0 - 1, p.floor_t:p.floor_t + p.trench_d] = 4e7
-    rho[geo.i1, p.floor_t:p.floor_t + p.trench_d] = -2e7
-    phi = fields.solve((-rho / model.EPS0).ravel()).reshape(rho.shape) + fields.external
-    ex, ey = model.compute_field(phi, geo.vac, p)
-    sampler = configs.ion_velocity_sampler(p.ion_mass_amu, **configs.AR8_ION)
-    return p, geo, ex, ey, (phi, fields.wface), sampler
+def copy_probe(mib):
+    import ctypes
+    from trench01_amd import HipTracer, _lib
+    t = HipTracer(np.ones((4, 4), bool))
+    ms = ctypes.c_double(0.0)
+    nbytes = mib << 20
+    _lib.check(_lib.load().th_probe_copy(t._h, nbytes, 5, ctypes.byref(ms)))
+    print(json.dumps({"copy_probe_bytes": nbytes, "avg_kernel_ms": ms.value,
+                      "gbs": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+
+
+def precharge(p, batches, log):
+    """Charge the trench with small batches (same charge per batch) and return rho on surface cells."""
+    from trench01_amd import charging
+    pc = replace(p, n_per_batch=20_000)
+    dev = charging.DeviceCharging(charging.ChargingSetup(pc))
+    dev.set_rho(np.zeros(dev.m))
+    dev.reserve(2 * pc.n_per_batch)
+    t0 = time.perf_counter()
+    for b in range(batches):
+        dev.inject_device(0, pc.n_per_batch, seed=1000 + p.seed, batch=b)
+        dev.step(0, 2 * pc.n_per_batch)
+    rho, phi, _ = dev.read(counts=False)
+    log(f"pre-charge: {batches} batches of {2 * pc.n_per_batch} in {time.perf_counter() - t0:.2f}s")
+    dev.close()
+    dev.tracer.close()
+    return rho
 
 
 def main():
@@ -66,71 +89,97 @@ def main():
         print(f"bench.py measures one GPU; this build has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         sys.exit(2)
-    from trench01_amd import HipTracer, model
-    p, geo, ex, ey, barrier, sampler = workload(a.n_per_batch)
-    tracer = HipTracer(geo.vac)
-    tracer.refill = a.refill
-    tracer.set_fields(ex, ey, barrier)
-    prm = tracer.params_for(p, barrier)
-    nb = a.warmup + a.steps
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
+    from trench01_amd import charging, configs
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    p, _ = configs.product_params("AR8", n_per_batch=a.n_per_batch)
+    rho0 = precharge(p, a.precharge, log) if a.precharge > 0 else None
+    setup = charging.ChargingSetup(p)
+    dev = charging.DeviceCharging(setup)
+    m = dev.m
+    dev.set_rho(rho0 if rho0 is not None else np.zeros(m))
     n = 2 * p.n_per_batch
-    rng = np.random.default_rng(p.seed)
-    tracer.reserve(nb * n)
-    first = None
-    for b in range(nb):
-        parts = model.inject(p, p.n_per_batch, geo.ny, rng, ion_velocity=sampler)
-        if first is None:
-            first = parts
-        tracer.upload(b * n, *parts)
+    dev.reserve(n)
     for b in range(a.warmup):
-        tracer.trace_resident(b * n, n, prm)
+        dev.inject_device(0, p.n_per_batch, p.seed, b)
+        dev.step(0, n)
+        dev.read()
+    # keep the state of the first timed batch for the CPU baseline
+    dev.inject_device(0, p.n_per_batch, p.seed, a.warmup)
+    cpu_inputs = (dev.tracer.read_particles(0, n), dev.tracer.read_field(), dev.read(counts=False)[1])
     steps_total = 0
-    kernel_ms = 0.0
+    push_ms = update_ms = 0.0
     t0 = time.perf_counter()
-    for b in range(a.warmup, nb):
-        tally, _, ms = tracer.trace_resident(b * n, n, prm)
-        steps_total += tally["steps"]
-        kernel_ms += ms
+    for b in range(a.warmup, a.warmup + a.steps):
+        dev.inject_device(0, p.n_per_batch, p.seed, b)
+        d = dev.step(0, n)
+        dev.read()
+        steps_total += d.tally.steps
+        push_ms += d.push_ms
+        update_ms += d.update_ms
     wall = time.perf_counter() - t0
     value = a.steps * n / wall
-    achieved = FLOP_PER_STEP * steps_total / (kernel_ms * 1e-3) / 1e12
+    achieved = FLOP_PER_STEP * steps_total / (push_ms * 1e-3) / 1e12
     out = {
         "metric": "traced particles/s/GPU (AR=8, 1e6 particles per batch)",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (host-injected particle states)",
-        "config": {"workload": "AR8 trench 120x320, tapers 2/5 deg, conductive mask 40, barrier wall, "
-                               "1e6 particles per batch, FP64 push + LDS deposit tallies",
-                   "particles_per_batch": n, "grid": [geo.nx, geo.ny], "surface_cells": int(geo.surface.size),
-                   "max_steps": p.max_steps, "mean_steps_per_particle": steps_total / (a.steps * n)},
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (device-injected particles, Philox)",
+        "config": {"workload": "AR8 trench 120x320, tapers 2/5 deg, conductive floating mask 40, barrier wall, "
+                               "sigma_s 1e-15; 1e6 particles per batch; full batch on device (inject, FP64 push, "
+                               "deposit, leak, conductor, Green's potential, field)",
+                   "particles_per_batch": n, "grid": [setup.geo.nx, setup.geo.ny], "surface_cells": m,
+                   "green_rows_cols": list(setup.green.shape), "precharge_batches": a.precharge,
+                   "max_steps": p.max_steps, "mean_steps_per_particle": steps_total / (a.steps * n),
+                   "push_ms_per_batch": push_ms / a.steps, "update_ms_per_batch": update_ms / a.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "push64_kernel", "kernel_ms_per_step": kernel_ms / a.steps,
+                     "kernel": "push64_kernel", "kernel_ms_per_launch": push_ms / a.steps,
                      "flop_per_particle_step": FLOP_PER_STEP,
-                     "hbm_gbs_algorithmic": BYTES_PER_PARTICLE * n * a.steps / (kernel_ms * 1e-3) / 1e9},
+                     "note": "FP64 vector-ALU roof (78.6 TFLOP/s, same as FP64 MFMA); the push is ALU/latency "
+                             "bound, not HBM: algorithmic HBM traffic is 78 B/particle",
+                     "hbm_gbs_algorithmic": BYTES_PER_PARTICLE * n / (push_ms / a.steps * 1e-3) / 1e9},
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, geo, ex, ey, barrier, first, a.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
+    if a.steady:
+        out["time_to_steady"] = time_to_steady(p, a.max_batches, log)
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(p, geo, ex, ey, barrier, parts, sample):
-    """The oracle's C push (OpenMP) on a bounded sample of batch 0: ``sample``
-    ions and ``sample`` electrons, on all host cores this process may use."""
+def cpu_baseline(p, setup, parts, field, phi, threads):
+    """The oracle's C push (OpenMP) over one whole batch of the timed workload:
+    same particle states and field as the GPU's first timed batch."""
     from oracle import trace_ref
-    from trench01_amd.model import is_tapered
-    n = p.n_per_batch
-    idx = np.concatenate([np.arange(sample), n + np.arange(sample)])
-    x, y, vx, vy, qm, spc = (v[idx] for v in parts)
-    cores = len(os.sched_getaffinity(0))
+    geo = setup.geo
+    cores = threads or len(os.sched_getaffinity(0))
     trace_ref.lib()
+    ex, ey = field
+    x, y, vx, vy, qm, spc = parts
     t0 = time.perf_counter()
     r = trace_ref.trace_arrays(x, y, vx, vy, qm, spc, ex, ey, geo.vac, p.dx, p.cfl, p.max_steps,
-                               is_tapered(p), barrier, threads=cores)
+                               True, (phi.reshape(geo.solid.shape), setup.fields.wface), threads=cores)
     wall = time.perf_counter() - t0
-    return {"value": idx.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": f"first {sample} ions + {sample} electrons of batch 0 (same field), "
-                      f"{r['total_steps']} particle-steps in {wall:.2f} s"}
+    return {"value": x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"one whole batch ({x.size} particles, {r['total_steps']} particle-steps) of the timed "
+                      f"workload, push only, in {wall:.2f} s"}
+
+
+def time_to_steady(p, max_batches, log):
+    """Plain time-marching from zero charge until the reference's steady
+    criterion passes (W = 30, hold = 2), device injection, 1e6 per batch."""
+    from trench01_amd import charging
+    ps = replace(p, until_steady=True, max_batches=max_batches)
+    t0 = time.perf_counter()
+    res = charging.run(ps, log=lambda s: log(s) if "steady" in s else None, injection="device")
+    wall = time.perf_counter() - t0
+    nb = res["timing"]["batches"]
+    log(f"time-to-steady: converged={res['converged']} after {nb} batches in {wall:.1f}s")
+    return {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb,
+            "wall_s": wall, "setup_s": res["timing"]["setup_s"],
+            "bottom_center_V": res["steady"]["bottom center"], "t_conv_s": res["t_conv"]}
 
 
 if __name__ == "__main__":

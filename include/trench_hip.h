@@ -196,6 +196,12 @@ int th_charging_read(th_charging *c, double *rho_s, double *phi /* nullable */, 
  * pin the device arithmetic against the host C library. */
 int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, double *out);
 
+/* Streams `bytes` (a multiple of 32) from one device buffer to another with a
+ * plain grid-stride copy kernel, `repeats` times, and reports the average
+ * kernel time [ms].  No reference counterpart: it calibrates the HBM traffic
+ * counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
+int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

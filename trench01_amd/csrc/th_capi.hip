@@ -124,6 +124,7 @@ th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
     g.ny = t->ny;
     g.m = t->m;
     g.dx = p->dx;
+    g.inv_dx = 1.0 / p->dx;
     g.cfl_dx = p->cfl * p->dx;
     g.two_dx = 2.0 * p->dx;
     g.lx = t->nx * p->dx;
@@ -713,3 +714,31 @@ int th_read_field(th_tracer *t, double *Ex, double *Ey) {
 }
 
 }  // extern "C"
+
+extern "C" int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, double *ms) {
+    if (!t || !ms || bytes <= 0 || bytes % 32 || repeats < 1) return fail(TH_EINVAL, "invalid argument");
+    TH_HIP(hipSetDevice(t->device));
+    double4 *a = nullptr, *b = nullptr;
+    TH_HIP(hipMalloc((void **)&a, (size_t)bytes));
+    if (hipMalloc((void **)&b, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(TH_ENOMEM, "copy probe allocation failed");
+    }
+    int64_t n = bytes / 32;
+    int blocks = t->cus * 8;
+    hipError_t e = hipMemsetAsync(a, 0, (size_t)bytes, t->stream);
+    if (e == hipSuccess) e = hipEventRecord(t->ev0, t->stream);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(th::copy_kernel, dim3(blocks), dim3(256), 0, t->stream, a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(t->ev1, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    float f = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, t->ev0, t->ev1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(TH_EHIP, "copy probe: %s", hipGetErrorString(e));
+    *ms = f / repeats;
+    return TH_OK;
+}

@@ -42,14 +42,19 @@ TH_HD double th_hypot(double x, double y) {
     x = fabs(x);
     y = fabs(y);
     double ax = x < y ? y : x, ay = x < y ? x : y;
+    // One core call for all three ranges: scaling by 2^-600 / 2^600 is exact,
+    // and multiplying the result by 2^600 equals dividing it by 2^-600.
+    double down = 1.0, up = 1.0;
     if (ax > 0x1p+511) {
         if (ay <= ax * 0x1p-54) return ax + ay;
-        return th_hypot_core(ax * 0x1p-600, ay * 0x1p-600) / 0x1p-600;
+        down = 0x1p-600;
+        up = 0x1p+600;
+    } else if (ay < 0x1p-459) {
+        if (ax >= ay * 0x1p+54) return ax + ay;
+        down = 0x1p+600;
+        up = 0x1p-600;
+    } else if (ay <= ax * 0x1p-54) {
+        return ax + ay;
     }
-    if (ay < 0x1p-459) {
-        if (ax >= ay / 0x1p-54) return ax + ay;
-        return th_hypot_core(ax / 0x1p-600, ay / 0x1p-600) * 0x1p-600;
-    }
-    if (ay <= ax * 0x1p-54) return ax + ay;
-    return th_hypot_core(ax, ay);
+    return th_hypot_core(ax * down, ay * down) * up;
 }

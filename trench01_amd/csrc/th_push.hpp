@@ -19,7 +19,12 @@
 //     indices of wall entry; a reflected particle restores them with x0, y0;
 //   * the periodic wrap takes a - L or a + L where that equals the reference's
 //     fmod form exactly (Sterbenz), else falls back to fmod;
-//   * cfl*dx, 2*dx, nx*dx and ny*dx are formed once on the host (same products).
+//   * cfl*dx, 2*dx, nx*dx and ny*dx are formed once on the host (same products);
+//   * x/dx and y/dx use Markstein's correction (P. Markstein, "Computation of
+//     elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
+//     Dev. 34(1), 1990): with y = RN(1/dx) and a faithful quotient q, one fused
+//     residual step RN(q + (x - q dx) y) is the correctly rounded x/dx, i.e. the
+//     same double as the IEEE division (positions never over- or underflow).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,7 +43,7 @@ struct Grid {
     const uint8_t *__restrict__ vac;     // 1 vacuum, 0 solid
     const int32_t *__restrict__ smap;    // cell -> surface index, -1 elsewhere
     int nx, ny, m;
-    double dx, cfl_dx, two_dx, lx, ly;   // cfl*dx, 2*dx, nx*dx, ny*dx formed on the host
+    double dx, inv_dx, cfl_dx, two_dx, lx, ly;   // 1/dx, cfl*dx, 2*dx, nx*dx, ny*dx formed on the host
 };
 
 struct Work {
@@ -69,6 +74,15 @@ __device__ __forceinline__ double wrap_pos(double a, double L) {
     if (a < 0.0 && a > -L) return a + L;       // fmod(a, L) == a here
     double r = fmod(a, L);
     return r < 0.0 ? r + L : r;
+}
+
+// x / dx, correctly rounded, from the host-rounded reciprocal (see above).
+__device__ __forceinline__ double div_dx(double a, double dx, double inv_dx) {
+    double q = a * inv_dx;
+    double r = fma(-q, dx, a);
+    q = fma(r, inv_dx, q);
+    r = fma(-q, dx, a);
+    return fma(r, inv_dx, q);
 }
 
 struct State64 {
@@ -109,8 +123,8 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
     s.y = y0 + sy;
     s.vx = vxn;
     s.vy = vyn;
-    s.q = s.x / g.dx;
-    s.r = s.y / g.dx;
+    s.q = div_dx(s.x, g.dx, g.inv_dx);
+    s.r = div_dx(s.y, g.dx, g.inv_dx);
     // Classification of the end point.
     bool escaped = s.y >= g.ly;
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
@@ -199,7 +213,7 @@ __global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
                 if (q < w.end) {
                     pid = q;
                     s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q]; s.qm = w.qm[q];
-                    s.q = s.x / g.dx; s.r = s.y / g.dx;
+                    s.q = div_dx(s.x, g.dx, g.inv_dx); s.r = div_dx(s.y, g.dx, g.inv_dx);
                     sp = w.spc[q];
                     k = 0;
                 } else {
@@ -252,6 +266,11 @@ __global__ void selftest_math_kernel(const double *a, const double *b, int64_t n
     out[4 * k + 1] = sqrt(x);
     out[4 * k + 2] = x / y;
     out[4 * k + 3] = fmod(x, y);
+}
+
+__global__ void copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
 }
 
 }  // namespace th
