@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores this process may use")
     ap.add_argument("--steady", action="store_true", help="also time plain time-marching to steady state")
     ap.add_argument("--max-batches", type=int, default=5000)
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -98,6 +99,7 @@ def main():
     rho0 = precharge(p, a.precharge, log) if a.precharge > 0 else None
     setup = charging.ChargingSetup(p)
     dev = charging.DeviceCharging(setup)
+    dev.set_schedule(precision=a.precision)
     m = dev.m
     dev.set_rho(rho0 if rho0 is not None else np.zeros(m))
     n = 2 * p.n_per_batch
@@ -126,7 +128,8 @@ def main():
         "metric": "traced particles/s/GPU (AR=8, 1e6 particles per batch)",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (device-injected particles, Philox)",
+        "vs_baseline": None, "dtype": "f64" if a.precision == 0 else "f32",
+        "data": "synthetic (device-injected particles, Philox)",
         "config": {"workload": "AR8 trench 120x320, tapers 2/5 deg, conductive floating mask 40, barrier wall, "
                                "sigma_s 1e-15; 1e6 particles per batch; full batch on device (inject, FP64 push, "
                                "deposit, leak, conductor, Green's potential, field)",

@@ -40,6 +40,11 @@ typedef struct th_trace_params {
     int32_t precision;    /* 0: FP64, bit-identical with the reference; 1: FP32 push */
     int32_t refill;       /* wave refill threshold in lanes, 0 = library default */
     int32_t chunk;        /* particles per launch when a stop flag is given, 0 = default */
+    /* Schedule (no effect on results): */
+    int32_t order;        /* 0 = particles in index order, 1 = reversed (electrons first) */
+    int32_t prio_steps;   /* raise a wave's issue priority once a particle is this old, 0 = off */
+    int32_t check_steps;  /* push steps between refill checks, 0 = library default */
+    int32_t reserved;
 } th_trace_params;
 
 /* Scalar tallies of one trace call (trench_cuda.py:119-122). */
@@ -88,6 +93,19 @@ int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, c
              const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
              const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
              int8_t *status /* nullable */, int32_t *hit /* nullable */);
+
+/* Record mode: th_trace plus, per particle, the end point (clipped to the
+ * face it crossed) and final velocity ([n][2] each), and the paths of the
+ * probe particles: slots[n] holds each particle's probe index or -1,
+ * paths is [n_probe][max_steps+1][2], path_lengths[n_probe].  Replaces the
+ * record branch of CudaTracer._trace (trench_cuda.py:81-97, :123-137;
+ * kernel trench_trace.cu:149-183, :189-192).  status and hit are required.
+ * Untraced particles (after a stop) keep their initial point and velocity. */
+int th_trace_record(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                    const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                    const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
+                    int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
+                    const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths);
 
 /* Device-resident particle pool, so a batch can be traced with its states
  * already in HBM.  th_particles_reserve sizes the pool (contents are lost);

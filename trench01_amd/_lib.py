@@ -20,7 +20,8 @@ class TraceParams(ctypes.Structure):
     """th_trace_params."""
     _fields_ = [("dx", ctypes.c_double), ("cfl", ctypes.c_double), ("max_steps", ctypes.c_int32),
                 ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32)]
+                ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Tally(ctypes.Structure):
@@ -67,6 +68,9 @@ SIGNATURES = {
     "th_set_fields": (ctypes.c_int, [vp, dp, dp, dp, dp]),
     "th_trace": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                 i32p, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_trace_record": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                       i32p, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p, dp, dp, i32p,
+                                       ctypes.c_int32, dp, i32p]),
     "th_particles_reserve": (ctypes.c_int, [vp, ctypes.c_int64]),
     "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
     "th_trace_resident": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), i64p,

@@ -170,6 +170,15 @@ class DeviceCharging:
         self.prm = self.tracer.params_for(p, p.wall_model == "barrier")
         self.capacity = 0
 
+    def set_schedule(self, **kw):
+        """Push options of the device batches: precision (0 FP64, 1 FP32) and the
+        schedule fields of trench01_amd.autotune (order, refill, check_steps,
+        prio_steps)."""
+        for k, v in kw.items():
+            if not hasattr(self.prm, k):
+                raise KeyError(k)
+            setattr(self.prm, k, int(v))
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().th_charging_destroy(self._h)

@@ -134,25 +134,36 @@ th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
 
 // Persistent launch: as many workgroups as can be resident at once (the
 // refill loop keeps them busy), never more than the particles need.
-template <bool B, bool T>
-int launch64(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds) {
+template <typename K>
+int launch_kernel(th_tracer *t, K kernel, const th::Grid &g, const th::Work &w, int64_t want, size_t lds) {
     int per_cu = 0;
-    TH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, th::push64_kernel<B, T>, th::kBlock, lds));
+    TH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, th::kBlock, lds));
     if (per_cu < 1) per_cu = 1;
     int blocks = (int)std::min<int64_t>(want, (int64_t)t->cus * per_cu);
-    hipLaunchKernelGGL((th::push64_kernel<B, T>), dim3(blocks), dim3(th::kBlock), lds, t->stream, g, w);
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(th::kBlock), lds, t->stream, g, w);
     TH_HIP(hipGetLastError());
     return TH_OK;
 }
 
-int launch_push(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered) {
+template <bool B, bool T>
+int launch_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds, int precision,
+                   bool record) {
+    if (precision == 1) return launch_kernel(t, th::push32_kernel<B, T>, g, w, want, lds);
+    if (record) return launch_kernel(t, th::push64_record_kernel<B, T>, g, w, want, lds);
+    return launch_kernel(t, th::push64_kernel<B, T>, g, w, want, lds);
+}
+
+int launch_push(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered, int precision,
+                bool record) {
     int64_t n = w.end - w.begin;
     if (n <= 0) return TH_OK;
     size_t lds = sizeof(uint32_t) * 2 * (size_t)t->m;
     int64_t want = (n + th::kBlock - 1) / th::kBlock;
     if (barrier)
-        return tapered ? launch64<true, true>(t, g, w, want, lds) : launch64<true, false>(t, g, w, want, lds);
-    return tapered ? launch64<false, true>(t, g, w, want, lds) : launch64<false, false>(t, g, w, want, lds);
+        return tapered ? launch_variant<true, true>(t, g, w, want, lds, precision, record)
+                       : launch_variant<true, false>(t, g, w, want, lds, precision, record);
+    return tapered ? launch_variant<false, true>(t, g, w, want, lds, precision, record)
+                   : launch_variant<false, false>(t, g, w, want, lds, precision, record);
 }
 
 }  // namespace
@@ -259,7 +270,7 @@ int th_set_fields(th_tracer *t, const double *Ex, const double *Ey, const double
     // FP32 records hold E/dx so the FP32 push works in cell units; dx is not
     // known here, so they are scaled when the push is launched (1.0 for now).
     hipLaunchKernelGGL(th::pack_cells_kernel, dim3(blocks), dim3(threads), 0, t->stream, t->d_tmp,
-                       t->d_tmp + t->N, phi ? t->d_tmp + 2 * (size_t)t->N : nullptr, t->d_vac, t->N, 1.0,
+                       t->d_tmp + t->N, phi ? t->d_tmp + 2 * (size_t)t->N : nullptr, t->d_vac, t->N,
                        t->d_cells, t->d_cells32);
     TH_HIP(hipGetLastError());
     TH_HIP(hipStreamSynchronize(t->stream));
@@ -276,7 +287,7 @@ int check_params(const th_tracer *t, const th_trace_params *p) {
     if (!(p->cfl > 0.0 && p->cfl <= 1.0) || p->max_steps < 1)
         return fail(TH_EINVAL, "tracing needs 0 < cfl <= 1 and max_steps >= 1");
     if (!(p->dx > 0.0)) return fail(TH_EINVAL, "dx must be positive");
-    if (p->precision != 0) return fail(TH_EINVAL, "precision %d is not available", p->precision);
+    if (p->precision != 0 && p->precision != 1) return fail(TH_EINVAL, "precision %d is not available", p->precision);
     if (!t->fields) return fail(TH_ESTATE, "th_set_fields must precede tracing");
     if (p->barrier && !t->barrier_fields) return fail(TH_ESTATE, "barrier wall model needs phi and wface");
     return TH_OK;
@@ -300,7 +311,8 @@ int upload(th_tracer *t, int64_t off, const double *x, const double *y, const do
 // tracer's stream.  With a stop flag the range is cut into chunks and the flag
 // is read between them; *traced is the end of the traced prefix.
 int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
-                 bool want_status, bool want_hit, int64_t *traced, th_tally *tally) {
+                 bool want_status, bool want_hit, int64_t *traced, th_tally *tally,
+                 const th::Record *rec = nullptr) {
     TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
     TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
     TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
@@ -319,6 +331,10 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
         w.scal = t->d_scal;
         w.max_steps = p->max_steps;
         w.refill = p->refill > 0 ? std::min(p->refill, 64) : 16;
+        w.order = p->order != 0;
+        w.prio_steps = p->prio_steps > 0 ? p->prio_steps : 0;
+        w.check_steps = p->check_steps > 0 ? std::min(p->check_steps, 256) : th::kStepsPerCheck;
+        w.rec = rec ? *rec : th::Record{};
         int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
         int rc;
         for (int64_t b = off; b < off + n; b += chunk) {
@@ -329,7 +345,8 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
             w.begin = b;
             w.end = std::min(off + n, b + chunk);
             TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-            if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0))) return rc;
+            if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, rec != nullptr)))
+                return rc;
             if (stop) TH_HIP(hipStreamSynchronize(t->stream));
             *traced = w.end;
         }
@@ -583,7 +600,7 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
     d.leak_M = c->leak_M; d.leak_c = c->leak_c; d.region = c->region; d.cond_s = c->cond_s;
     d.g_cond_s = c->g_cond_s; d.vac = t->d_vac; d.wface = t->d_wface;
     d.rho = c->rho; d.rho_prev = c->rho_prev; d.qcol = c->qcol; d.phi = c->phi; d.diag = c->diag;
-    d.cells = t->d_cells; d.counts = t->d_counts;
+    d.cells = t->d_cells; d.cells32 = t->d_cells32; d.counts = t->d_counts;
     d.q_macro = s->q_macro; d.dx = s->dx; d.C_cond = s->C_cond; d.Te = s->Te; d.dt_batch = s->dt_batch;
     d.top_voltage = s->top_voltage; d.substrate_voltage = s->substrate_voltage;
     d.c_idx = s->c_idx; d.floating = s->floating; d.driven = s->driven; d.has_leak = s->leak_M != nullptr;
@@ -740,5 +757,117 @@ extern "C" int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, doubl
     (void)hipFree(b);
     if (e != hipSuccess) return fail(TH_EHIP, "copy probe: %s", hipGetErrorString(e));
     *ms = f / repeats;
+    return TH_OK;
+}
+
+extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                               const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                               const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
+                               int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
+                               const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths) {
+    if (!t || !p || !cnt_i || !cnt_e || !tally || !status || !hit || !endpoints || !final_velocity)
+        return fail(TH_EINVAL, "invalid argument");
+    if (n < 0 || n_probe < 0 || (n_probe > 0 && (!slots || !paths || !path_lengths)))
+        return fail(TH_EINVAL, "invalid probe arguments");
+    if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    if (p->precision != 0) return fail(TH_EINVAL, "record mode is FP64 only");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; ++k) {
+        if (spc[k] != 0 && spc[k] != 1) return fail(TH_EINVAL, "species must be 0 (ion) or 1 (electron)");
+        if (slots && (slots[k] < -1 || slots[k] >= n_probe)) return fail(TH_EINVAL, "probe slot out of range");
+    }
+    const int stride = p->max_steps + 1;
+    // Untraced particles (after a stop) keep their initial point and velocity,
+    // and probe paths start with the initial point (trench_cuda.py:89-97).
+    for (int64_t k = 0; k < n; ++k) {
+        endpoints[2 * k] = x[k];
+        endpoints[2 * k + 1] = y[k];
+        final_velocity[2 * k] = vx[k];
+        final_velocity[2 * k + 1] = vy[k];
+        status[k] = 3;
+        hit[k] = -1;
+        if (slots && slots[k] >= 0) {
+            paths[2 * (int64_t)slots[k] * stride] = x[k];
+            paths[2 * (int64_t)slots[k] * stride + 1] = y[k];
+            path_lengths[slots[k]] = 1;
+        }
+    }
+    memset(tally, 0, sizeof *tally);
+    memset(cnt_i, 0, sizeof(int64_t) * (size_t)t->N);
+    memset(cnt_e, 0, sizeof(int64_t) * (size_t)t->N);
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    if ((rc = reserve_particles(t, n))) return rc;
+    if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
+    double *d_end = nullptr, *d_vel = nullptr, *d_paths = nullptr;
+    int32_t *d_slots = nullptr, *d_len = nullptr;
+    size_t path_doubles = 2 * (size_t)std::max(n_probe, 1) * stride;
+    auto release = [&]() {
+        for (void *q : {(void *)d_end, (void *)d_vel, (void *)d_paths, (void *)d_slots, (void *)d_len})
+            if (q) (void)hipFree(q);
+    };
+    if ((rc = dev_alloc(&d_end, 2 * (size_t)n)) || (rc = dev_alloc(&d_vel, 2 * (size_t)n)) ||
+        (rc = dev_alloc(&d_paths, path_doubles)) || (rc = dev_alloc(&d_slots, (size_t)n)) ||
+        (rc = dev_alloc(&d_len, (size_t)std::max(n_probe, 1)))) {
+        release();
+        return rc;
+    }
+    std::vector<int32_t> sl((size_t)n, -1);
+    if (slots) memcpy(sl.data(), slots, sizeof(int32_t) * (size_t)n);
+    hipError_t e = hipMemcpyAsync(d_end, endpoints, sizeof(double) * 2 * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_vel, final_velocity, sizeof(double) * 2 * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_slots, sl.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && n_probe > 0)
+        e = hipMemcpyAsync(d_paths, paths, sizeof(double) * 2 * (size_t)n_probe * stride, hipMemcpyHostToDevice,
+                           t->stream);
+    if (e == hipSuccess && n_probe > 0)
+        e = hipMemcpyAsync(d_len, path_lengths, sizeof(int32_t) * n_probe, hipMemcpyHostToDevice, t->stream);
+    if (e != hipSuccess) {
+        release();
+        return fail(TH_EHIP, "record upload: %s", hipGetErrorString(e));
+    }
+    th::Record rec{d_end, d_vel, d_slots, d_paths, d_len, stride};
+    int64_t traced = 0;
+    rc = launch_trace(t, 0, n, p, stop, true, true, &traced, tally, &rec);
+    if (rc) {
+        (void)hipStreamSynchronize(t->stream);
+        release();
+        return rc;
+    }
+    std::vector<uint32_t> counts(2 * (size_t)std::max(t->m, 1));
+    unsigned long long scal[th::kScalCount];
+    e = hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts.data(), t->d_counts, sizeof(uint32_t) * counts.size(), hipMemcpyDeviceToHost,
+                           t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(status, t->d_status, (size_t)n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hit, t->d_hit, sizeof(int32_t) * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(endpoints, d_end, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(final_velocity, d_vel, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && n_probe > 0)
+        e = hipMemcpyAsync(paths, d_paths, sizeof(double) * 2 * (size_t)n_probe * stride, hipMemcpyDeviceToHost,
+                           t->stream);
+    if (e == hipSuccess && n_probe > 0)
+        e = hipMemcpyAsync(path_lengths, d_len, sizeof(int32_t) * n_probe, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "record readback: %s", hipGetErrorString(e));
+    for (int k = 0; k < t->m; ++k) {
+        cnt_i[t->surf[k]] += counts[k];
+        cnt_e[t->surf[k]] += counts[t->m + k];
+    }
+    if (scal[th::kScalOff]) {
+        std::vector<uint32_t> over(2 * (size_t)t->N);
+        TH_HIP(hipMemcpy(over.data(), t->d_overflow, sizeof(uint32_t) * over.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < t->N; ++c) {
+            cnt_i[c] += over[c];
+            cnt_e[c] += over[t->N + c];
+        }
+    }
+    fill_tally(scal, n - traced, tally);
     return TH_OK;
 }

@@ -42,6 +42,7 @@ struct ChargeDev {
     double *phi;                // [N] potential (kept rows; others stay 0)
     double *diag;               // per-batch scalars, see kDiag*
     double4 *cells;             // push records {Ex*vac, Ey*vac, vac, phi}
+    float4 *cells32;            // the same records in float for the FP32 push
     const uint32_t *counts;     // [2][m] deposit tallies of the batch
     double q_macro, dx, C_cond, Te, dt_batch, top_voltage, substrate_voltage;
     int c_idx, floating, driven, has_leak, top_dirichlet, leak_has_c;
@@ -205,6 +206,7 @@ __global__ void field_pack_kernel(ChargeDev d) {
         ey = -g;
     }
     d.cells[c] = make_double4(v ? ex : 0.0, v ? ey : 0.0, v ? 1.0 : 0.0, pc);
+    d.cells32[c] = make_float4(v ? (float)ex : 0.0f, v ? (float)ey : 0.0f, v ? 1.0f : 0.0f, (float)pc);
 }
 
 }  // namespace th

@@ -28,22 +28,37 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
+
 #include "th_math.h"
 
 namespace th {
 
 constexpr int kBlock = 256;          // 4 waves per workgroup
-constexpr int kStepsPerCheck = 8;    // steps between refill checks
+constexpr int kStepsPerCheck = 4;    // default steps between refill checks
 enum : int { kScalQueue = 0, kScalEsc0, kScalEsc1, kScalLost, kScalSteps, kScalOff, kScalCount };
 
 struct Grid {
     const double4 *__restrict__ cells;   // {Ex*vac, Ey*vac, vac, phi} per cell
-    const float4 *__restrict__ cells32;  // FP32 push: {Ex*vac/dx, Ey*vac/dx, vac, 0}
+    const float4 *__restrict__ cells32;  // FP32 push: {Ex*vac, Ey*vac, vac, phi} in float
     const double *__restrict__ wface;    // barrier face weight per cell
     const uint8_t *__restrict__ vac;     // 1 vacuum, 0 solid
     const int32_t *__restrict__ smap;    // cell -> surface index, -1 elsewhere
     int nx, ny, m;
     double dx, inv_dx, cfl_dx, two_dx, lx, ly;   // 1/dx, cfl*dx, 2*dx, nx*dx, ny*dx formed on the host
+};
+
+// Record-mode outputs (trench_trace.cu:149-183, :189-192): the end point of
+// every particle (clipped to the face it crossed), its final velocity, and
+// the path of the probe particles (slots[pid] >= 0).
+struct Record {
+    double *end;             // [n][2]
+    double *vel;             // [n][2]
+    const int32_t *slots;    // [n] probe index or -1
+    double *paths;           // [n_probe][stride][2]
+    int32_t *path_len;       // [n_probe]
+    int stride;              // max_steps + 1
 };
 
 struct Work {
@@ -56,6 +71,8 @@ struct Work {
     uint32_t *overflow;                  // [2][nx*ny] tallies for non-surface cells
     unsigned long long *scal;            // kScalCount counters
     int max_steps, refill;
+    int order, prio_steps, check_steps;
+    Record rec;                          // record mode only
 };
 
 __device__ __forceinline__ int wrap_cell(int i, int n) {
@@ -85,14 +102,18 @@ __device__ __forceinline__ double div_dx(double a, double dx, double inv_dx) {
     return fma(r, inv_dx, q);
 }
 
-struct State64 {
-    double x, y, vx, vy, qm;
-    double q, r;   // x/dx, y/dx of the current position
+template <typename Real>
+struct StateT {
+    Real x, y, vx, vy, qm;
+    Real q, r;   // x/dx, y/dx of the current position
 };
+using State64 = StateT<double>;
+using State32 = StateT<float>;
 
-// One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2 (escaped).
-template <bool BARRIER, bool TAPERED>
-__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
+// One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
+// (escaped).  With RECORD, (xp, yp) is the step's recorded point.
+template <bool BARRIER, bool TAPERED, bool RECORD>
+__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, double &xp, double &yp) {
     const int nx = g.nx, ny = g.ny;
     // Vacuum-weighted bilinear field at the cell-centred lattice.
     double fx = s.q - 0.5, fy = s.r - 0.5;
@@ -132,7 +153,9 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
     bool corner = false;
     if (TAPERED && !hit && !escaped)
         corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
-    if ((hit && (BARRIER || TAPERED)) || corner) {
+    int cx = ix, cy = iy;
+    bool face_x = false;
+    if ((hit && (BARRIER || TAPERED || RECORD)) || corner) {
         // First solid cell crossed during the step, and the vacuum cell before it.
         int ix0 = clip_cell((int)q0, nx), iy0 = clip_cell((int)r0, ny);
         int dix = (int)floor(xu / g.dx) - ix0;
@@ -143,16 +166,118 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell) {
         int ax_ = x_first ? wrap_cell(ix0 + dix, nx) : ix0;
         int ay_ = x_first ? iy0 : iy0 + diy;
         bool last = dix != 0 && diy != 0 && g.vac[ax_ * ny + ay_];
-        int cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
-        int cy = last ? iy0 + diy : ay_;
+        cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
+        cy = last ? iy0 + diy : ay_;
         int vcx = last ? ax_ : ix0, vcy = last ? ay_ : iy0;
-        bool face_x = last ? !x_first : x_first;
+        face_x = last ? !x_first : x_first;
         hit = !g.vac[cx * ny + cy];
         if (BARRIER && hit) {
             double vn = face_x ? s.vx : s.vy;
             int c = cx * ny + cy, v = vcx * ny + vcy;
             double dphi = g.wface[c] * (g.cells[c].w - g.cells[v].w);
             if (0.5 * vn * vn < s.qm * dphi) {   // cannot climb the wall barrier: reflect
+                if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
+                s.x = x0; s.y = y0; s.q = q0; s.r = r0;
+                hit = false;
+            }
+        }
+        ix = cx;
+        iy = cy;
+    }
+    if (RECORD) {
+        xp = s.x;
+        yp = s.y;
+        if (hit) {
+            double frac;
+            if (face_x) {
+                double xb = (double)(cx + (sx < 0.0)) * g.dx;
+                xb += nearbyint((x0 - xb) / g.lx) * g.lx;
+                frac = (xb - x0) / sx;
+            } else {
+                frac = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
+            }
+            frac = fmin(fmax(frac, 0.0), 1.0);
+            xp = wrap_pos(x0 + frac * sx, g.lx);
+            yp = y0 + frac * sy;
+        }
+        if (escaped) {
+            double frac = fmin(fmax((g.ly - y0) / sy, 0.0), 1.0);
+            xp = wrap_pos(x0 + frac * sx, g.lx);
+            yp = g.ly;
+        }
+    }
+    if (hit) { cell = ix * ny + iy; return 1; }
+    return escaped ? 2 : 0;
+}
+
+__device__ __forceinline__ float wrap_posf(float a, float L) {
+    if (a >= 0.0f && a < L) return a;
+    if (a >= L && a < 2.0f * L) return a - L;
+    if (a < 0.0f && a > -L) return a + L;
+    float r = fmodf(a, L);
+    return r < 0.0f ? r + L : r;
+}
+
+// One FP32 push step: the same algorithm in single precision (SI units,
+// float field records, x/dx as a multiply by 1/dx, library hypotf).  Not
+// bit-compatible with the reference; its tallies agree statistically.
+template <bool BARRIER, bool TAPERED>
+__device__ __forceinline__ int step32(const Grid &g, State32 &s, int &cell) {
+    const int nx = g.nx, ny = g.ny;
+    const float dx = (float)g.dx, inv_dx = (float)g.inv_dx, lx = (float)g.lx, ly = (float)g.ly;
+    float fx = s.q - 0.5f, fy = s.r - 0.5f;
+    float fi = floorf(fx), fj = floorf(fy);
+    int i = (int)fi, j = (int)fj;
+    float tx = fx - fi, ty = fy - fj;
+    int i0 = wrap_cell(i, nx), i1 = wrap_cell(i + 1, nx);
+    int j0 = clip_cell(j, ny), j1 = clip_cell(j + 1, ny);
+    float4 c00 = g.cells32[i0 * ny + j0], c10 = g.cells32[i1 * ny + j0];
+    float4 c01 = g.cells32[i0 * ny + j1], c11 = g.cells32[i1 * ny + j1];
+    float w00 = (1.0f - tx) * (1.0f - ty), w10 = tx * (1.0f - ty), w01 = (1.0f - tx) * ty, w11 = tx * ty;
+    float ex = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
+    float ey = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
+    float sw = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
+    float inv_sw = 1.0f / fmaxf(sw, 1e-12f);
+    ex *= inv_sw;
+    ey *= inv_sw;
+    float ax = s.qm * ex, ay = s.qm * ey;
+    float dt = (float)g.cfl_dx / (hypotf(s.vx, s.vy) + sqrtf((float)g.two_dx * hypotf(ax, ay)) + 1.0f);
+    float vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
+    const float x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
+    float sx = 0.5f * (s.vx + vxn) * dt, sy = 0.5f * (s.vy + vyn) * dt;
+    float xu = x0 + sx;
+    s.x = wrap_posf(xu, lx);
+    s.y = y0 + sy;
+    s.vx = vxn;
+    s.vy = vyn;
+    s.q = s.x * inv_dx;
+    s.r = s.y * inv_dx;
+    bool escaped = s.y >= ly;
+    int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
+    bool hit = !escaped && !g.vac[ix * ny + iy];
+    bool corner = false;
+    if (TAPERED && !hit && !escaped)
+        corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
+    if ((hit && (BARRIER || TAPERED)) || corner) {
+        int ix0 = clip_cell((int)q0, nx), iy0 = clip_cell((int)r0, ny);
+        int dix = (int)floorf(xu * inv_dx) - ix0;
+        int diy = clip_cell((int)floorf(s.r), ny) - iy0;
+        float tfx = dix != 0 ? ((float)(ix0 + (dix > 0)) * dx - x0) / sx : INFINITY;
+        float tfy = diy != 0 ? ((float)(iy0 + (diy > 0)) * dx - y0) / sy : INFINITY;
+        bool x_first = tfx < tfy;
+        int ax_ = x_first ? wrap_cell(ix0 + dix, nx) : ix0;
+        int ay_ = x_first ? iy0 : iy0 + diy;
+        bool last = dix != 0 && diy != 0 && g.vac[ax_ * ny + ay_];
+        int cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
+        int cy = last ? iy0 + diy : ay_;
+        int vcx = last ? ax_ : ix0, vcy = last ? ay_ : iy0;
+        bool face_x = last ? !x_first : x_first;
+        hit = !g.vac[cx * ny + cy];
+        if (BARRIER && hit) {
+            float vn = face_x ? s.vx : s.vy;
+            int c = cx * ny + cy, v = vcx * ny + vcy;
+            float dphi = (float)g.wface[c] * (g.cells32[c].w - g.cells32[v].w);
+            if (0.5f * vn * vn < s.qm * dphi) {
                 if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
                 s.x = x0; s.y = y0; s.q = q0; s.r = r0;
                 hit = false;
@@ -186,8 +311,13 @@ __device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *l
     }
 }
 
-template <bool BARRIER, bool TAPERED>
-__global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
+enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2 };
+
+// Persistent refill loop shared by every push variant.
+template <int MODE, bool BARRIER, bool TAPERED>
+__device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
+    using Real = typename std::conditional<MODE == kModeF32, float, double>::type;
+    constexpr bool RECORD = MODE == kModeF64Record;
     extern __shared__ uint32_t lds_counts[];   // [2][m]
     __shared__ unsigned long long lds_scal[kScalCount];
     for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
@@ -197,8 +327,10 @@ __global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
     int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
-    State64 s;
+    StateT<Real> s;
     int sp = 0, k = 0;
+    int slot = -1, plen = 0;      // record mode: probe slot and path length
+    double xp = 0.0, yp = 0.0;    // record mode: last recorded point
     unsigned long long my_steps = 0;
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
@@ -209,13 +341,33 @@ __global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
             if (lane == leader) base = atomicAdd(&w.scal[kScalQueue], (unsigned long long)__popcll(idle));
             base = __shfl(base, leader);
             if (pid == -1) {
-                int64_t q = w.begin + (int64_t)(base + (unsigned long long)__popcll(idle & below));
-                if (q < w.end) {
+                int64_t q = (int64_t)(base + (unsigned long long)__popcll(idle & below));
+                q = w.order ? w.end - 1 - q : w.begin + q;
+                if (q >= w.begin && q < w.end) {
                     pid = q;
-                    s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q]; s.qm = w.qm[q];
-                    s.q = div_dx(s.x, g.dx, g.inv_dx); s.r = div_dx(s.y, g.dx, g.inv_dx);
+                    s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
+                    s.qm = (Real)w.qm[q];
+                    if constexpr (MODE == kModeF32) {
+                        s.q = s.x * (float)g.inv_dx;
+                        s.r = s.y * (float)g.inv_dx;
+                    } else {
+                        s.q = div_dx(s.x, g.dx, g.inv_dx);
+                        s.r = div_dx(s.y, g.dx, g.inv_dx);
+                    }
                     sp = w.spc[q];
                     k = 0;
+                    if constexpr (RECORD) {
+                        xp = s.x;
+                        yp = s.y;
+                        slot = w.rec.slots[q];
+                        plen = 0;
+                        if (slot >= 0) {
+                            double *path = w.rec.paths + 2 * (int64_t)slot * w.rec.stride;
+                            path[0] = s.x;
+                            path[1] = s.y;
+                            plen = 1;
+                        }
+                    }
                 } else {
                     pid = -2;
                 }
@@ -223,13 +375,39 @@ __global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
             busy = __ballot(pid >= 0);
         }
         if (busy == 0ull) break;
+        if (w.prio_steps > 0) {
+            // Waves carrying long-lived particles hold the launch's critical path:
+            // let them issue first.
+            if (__ballot(pid >= 0 && k >= w.prio_steps) != 0ull) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         if (pid >= 0) {
-            for (int t = 0; t < kStepsPerCheck; ++t) {
+            for (int t = 0; t < w.check_steps; ++t) {
                 int cell = -1;
-                int st = step64<BARRIER, TAPERED>(g, s, cell);
+                int st;
+                if constexpr (MODE == kModeF32) {
+                    st = step32<BARRIER, TAPERED>(g, s, cell);
+                } else {
+                    st = step64<BARRIER, TAPERED, RECORD>(g, s, cell, xp, yp);
+                    if constexpr (RECORD) {
+                        if (slot >= 0) {
+                            double *path = w.rec.paths + 2 * ((int64_t)slot * w.rec.stride + plen);
+                            path[0] = xp;
+                            path[1] = yp;
+                            ++plen;
+                        }
+                    }
+                }
                 ++k;
                 if (st != 0 || k >= w.max_steps) {
                     finish(g, w, lds_counts, lds_scal, pid, sp, st != 0 ? st : 3, cell);
+                    if constexpr (RECORD) {
+                        w.rec.end[2 * pid] = xp;
+                        w.rec.end[2 * pid + 1] = yp;
+                        w.rec.vel[2 * pid] = s.vx;
+                        w.rec.vel[2 * pid + 1] = s.vy;
+                        if (slot >= 0) w.rec.path_len[slot] = plen;
+                    }
                     my_steps += (unsigned long long)k;
                     pid = -1;
                     break;
@@ -245,17 +423,35 @@ __global__ void __launch_bounds__(kBlock) push64_kernel(Grid g, Work w) {
         atomicAdd(&w.scal[threadIdx.x], lds_scal[threadIdx.x]);
 }
 
+// FP64 push.  94 VGPRs without spills fits 5 waves per SIMD (6 would spill).
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) push64_kernel(Grid g, Work w) {
+    push_body<kModeF64, BARRIER, TAPERED>(g, w);
+}
+
+// FP64 push with record mode (diagnostics; its extra registers stay unspilled).
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_record_kernel(Grid g, Work w) {
+    push_body<kModeF64Record, BARRIER, TAPERED>(g, w);
+}
+
+// FP32 push.
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push32_kernel(Grid g, Work w) {
+    push_body<kModeF32, BARRIER, TAPERED>(g, w);
+}
+
 // Packs the frozen field of one batch into per-cell records.
 __global__ void pack_cells_kernel(const double *__restrict__ ex, const double *__restrict__ ey,
                                   const double *__restrict__ phi, const uint8_t *__restrict__ vac,
-                                  int n, double inv_dx, double4 *__restrict__ cells,
+                                  int n, double4 *__restrict__ cells,
                                   float4 *__restrict__ cells32) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     bool v = vac[c] != 0;
     double e_x = v ? ex[c] : 0.0, e_y = v ? ey[c] : 0.0;
     cells[c] = make_double4(e_x, e_y, v ? 1.0 : 0.0, phi ? phi[c] : 0.0);
-    cells32[c] = make_float4((float)(e_x * inv_dx), (float)(e_y * inv_dx), v ? 1.0f : 0.0f, 0.0f);
+    cells32[c] = make_float4((float)e_x, (float)e_y, v ? 1.0f : 0.0f, phi ? (float)phi[c] : 0.0f);
 }
 
 __global__ void selftest_math_kernel(const double *a, const double *b, int64_t n, double *out) {

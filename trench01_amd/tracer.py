@@ -29,8 +29,11 @@ class HipTracer:
 
     #: 0 = FP64 push, bit-identical with the reference; 1 = FP32 push.
     precision = 0
-    #: wave refill threshold (lanes); 0 = library default
+    #: push schedule (see trench01_amd.autotune); 0 = library default
     refill = 0
+    order = 1
+    prio_steps = 0
+    check_steps = 0
 
     def __init__(self, vac, device=0):
         L = _lib.load()
@@ -79,13 +82,11 @@ class HipTracer:
             raise ValueError("the HIP tracer was built for another geometry; create a new tracer")
         if not (0 < p.cfl <= 1) or p.max_steps < 1:
             raise ValueError("HIP tracing needs 0 < cfl <= 1 and max_steps >= 1")
-        if record is not None:
-            raise NotImplementedError("record mode is not available in the HIP tracer yet")
         if rf is not None:
             raise NotImplementedError("RF particle clocks are not available in the HIP tracer yet")
-        return self._trace(x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, stop, keep_status)
+        return self._trace(x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status)
 
-    def _trace(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, stop, keep_status):
+    def _trace(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status):
         L = _lib.load()
         n = len(x)
         nx, ny = self.vac.shape
@@ -107,14 +108,27 @@ class HipTracer:
         _lib.check(L.th_set_fields(self._h, _lib.ptr(ex, _lib.dp), _lib.ptr(ey, _lib.dp),
                                    _lib.ptr(phi, _lib.dp), _lib.ptr(wface, _lib.dp)))
         xs = [f64(v) for v in (x, y, vx, vy, qm)]
-        prm = _lib.TraceParams(dx=float(p.dx), cfl=float(p.cfl), max_steps=int(p.max_steps),
-                               tapered=int(tapered_of(p)), barrier=int(barrier is not None),
-                               precision=int(self.precision), refill=int(self.refill), chunk=0)
+        prm = self.params_for(p, barrier is not None)
         cnt_i = np.empty((nx, ny), dtype=np.int64)
         cnt_e = np.empty((nx, ny), dtype=np.int64)
         tally = _lib.Tally()
-        status = np.empty(n, dtype=np.int8) if keep_status else None
-        hit = np.empty(n, dtype=np.int32) if keep_status else None
+        recording = record is not None
+        status = np.empty(n, dtype=np.int8) if (keep_status or recording) else None
+        hit = np.empty(n, dtype=np.int32) if (keep_status or recording) else None
+        if recording:
+            if p.probe_trajectories < 1:
+                raise ValueError("record mode needs probe_trajectories >= 1")
+            if prm.precision != 0:
+                raise ValueError("record mode runs the FP64 push only")
+            # probe particles: trench_cuda.py:85-87
+            selected = np.unique(np.linspace(0, n - 1, min(p.probe_trajectories, n), dtype=int))
+            slots = np.full(n, -1, dtype=np.int32)
+            slots[selected] = np.arange(selected.size, dtype=np.int32)
+            stride = p.max_steps + 1
+            ends = np.empty((n, 2))
+            vel = np.empty((n, 2))
+            paths = np.empty((max(selected.size, 1), stride, 2))
+            lengths = np.ones(max(selected.size, 1), dtype=np.int32)
         flag = None
         watcher = None
         done = threading.Event()
@@ -132,10 +146,16 @@ class HipTracer:
             watcher = threading.Thread(target=watch, daemon=True)
             watcher.start()
         try:
-            _lib.check(L.th_trace(self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n,
-                                  ctypes.byref(prm), ctypes.byref(flag) if flag is not None else None,
-                                  _lib.ptr(cnt_i, _lib.i64p), _lib.ptr(cnt_e, _lib.i64p), ctypes.byref(tally),
-                                  _lib.ptr(status, _lib.i8p), _lib.ptr(hit, _lib.i32p)))
+            args = (self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n, ctypes.byref(prm),
+                    ctypes.byref(flag) if flag is not None else None, _lib.ptr(cnt_i, _lib.i64p),
+                    _lib.ptr(cnt_e, _lib.i64p), ctypes.byref(tally), _lib.ptr(status, _lib.i8p),
+                    _lib.ptr(hit, _lib.i32p))
+            if recording:
+                _lib.check(L.th_trace_record(*args, _lib.ptr(ends, _lib.dp), _lib.ptr(vel, _lib.dp),
+                                             _lib.ptr(slots, _lib.i32p), int(selected.size),
+                                             _lib.ptr(paths, _lib.dp), _lib.ptr(lengths, _lib.i32p)))
+            else:
+                _lib.check(L.th_trace(*args))
         finally:
             done.set()
             if watcher is not None:
@@ -144,13 +164,24 @@ class HipTracer:
         self.last_tally = dict(steps=int(tally.steps), off_surface=int(tally.off_surface),
                                cancelled=bool(tally.cancelled))
         self.last_status, self.last_hit = status, hit
+        if recording:
+            # the record dictionary of trench_cuda.py:123-133
+            cells = np.full((n, 2), -1, dtype=int)
+            absorbed = status == 1
+            cells[absorbed] = np.column_stack((hit[absorbed] // ny, hit[absorbed] % ny))
+            record.update(selected=selected, status=status, hit_cells=cells, initial_x=np.asarray(x).copy(),
+                          endpoints=ends, final_velocity=vel,
+                          paths=[paths[k, :lengths[k]].copy() for k in range(selected.size)],
+                          cancelled=bool(tally.cancelled) or (stop is not None and stop.is_set()))
         return cnt_i, cnt_e, escaped, int(tally.lost)
 
     # ---- device-resident batches (no reference counterpart; for batch loops and benchmarks)
     def params_for(self, p, barrier):
         return _lib.TraceParams(dx=float(p.dx), cfl=float(p.cfl), max_steps=int(p.max_steps),
                                 tapered=int(tapered_of(p)), barrier=int(bool(barrier)),
-                                precision=int(self.precision), refill=int(self.refill), chunk=0)
+                                precision=int(self.precision), refill=int(self.refill), chunk=0,
+                                order=int(self.order), prio_steps=int(self.prio_steps),
+                                check_steps=int(self.check_steps))
 
     def set_fields(self, Ex, Ey, barrier=None):
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
