@@ -44,7 +44,8 @@ typedef struct th_trace_params {
     int32_t order;        /* 0 = particles in index order, 1 = reversed (electrons first) */
     int32_t prio_steps;   /* raise a wave's issue priority once a particle is this old, 0 = off */
     int32_t check_steps;  /* push steps between refill checks, 0 = library default */
-    int32_t reserved;
+    int32_t pass_steps;   /* multi-pass schedule: first pass stops particles at this many steps and
+                             later passes continue the survivors with doubling caps; 0 = one pass */
 } th_trace_params;
 
 /* Scalar tallies of one trace call (trench_cuda.py:119-122). */
@@ -203,6 +204,18 @@ int th_charging_set_rho(th_charging *c, const double *rho_s);
 /* One batch: traces pool[offset, offset+n) through the current field, then
  * deposit, leak, conductor, potential and field update, on the device. */
 int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, th_batch_diag *out);
+
+/* Per-batch monitors on the device (trench_charging_2d.py:1272-1285): after
+ * every th_charging_step the library gathers the potential at n_probes cells,
+ * the integrated charge sum(rho) dx^2 of n_regions regions (CSR lists of
+ * surface indices), sum(|rho|) dx^2 of region abs_region, and the ion and
+ * electron counts on the n_bottom floor cells (surface indices), and keeps
+ * them on the host: th_charging_monitors copies the last batch's
+ * n_probes + n_regions + 3 values (no device access). */
+int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *probe_cells, int32_t n_regions,
+                             const int32_t *region_ptr, const int32_t *region_idx, int32_t abs_region,
+                             int32_t n_bottom, const int32_t *bottom_s);
+int th_charging_monitors(const th_charging *c, double *out);
 
 /* Reads the state: rho (m), potential on the grid (nx*ny; cells outside the
  * kept rows read 0), and the last batch's per-surface-cell counts (2m). */

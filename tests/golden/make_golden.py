@@ -17,7 +17,9 @@ vector pins a reference function that the reference's own fixtures
 * ``run_ar5.npz``   a whole run() (:1045-1369) of the plain AR 5 workload
   (80x200, conductive mask, sigma_s 0, barrier wall, 100 eV ions with 3 degree
   polar spread, Te 4 eV), 50 batches of 1e4 particles per species, seed 0;
-  ``trace_ar5_marched.npz``: a fresh record-mode batch through its final field.
+  ``trace_ar5_marched.npz``: a fresh record-mode batch through its final field;
+  ``result_ar5.npz``: the reference's own result file of that run (save_results
+  :1687-1734), for the on-disk format.
 
 Usage:  python tests/golden/make_golden.py [--reference /root/reference] [--only NAME]
 """
@@ -173,6 +175,13 @@ def write_run_ar5(T):
     arrays.update({f"charge_{k}": v for k, v in res["charge_monitor_hist"].items()})
     np.savez_compressed(HERE / "run_ar5.npz", **arrays)
     print(f"run_ar5: {time.time() - t0:.0f}s, bottom centre {res['hist']['bottom center'][-1]:.3f} V")
+    # The reference's own result file of this run (save_results writes plots too;
+    # only the NPZ is kept, as result_ar5.npz).
+    import shutil
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        T.save_results(res, p, str(Path(tmp) / "out"))
+        shutil.copy(Path(tmp) / "out.npz", HERE / "result_ar5.npz")
     # A fresh batch traced in record mode through the charged field the run ends with.
     solid = res["solid"]
     fields = T.setup_fields(p, solid, res["mask"])

@@ -1,0 +1,91 @@
+"""Device batch driver: on-device monitors, the result file format, and resume.
+
+The reference checks that a resumed run equals the continuous one bit for bit
+(tests/test_resume.py:35-49) and that its NPZ round-trips
+(tests/test_taper.py:159-187); the same checks here run on the device path."""
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests import cases
+from trench01_amd import charging, model, results
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def small(**kw):
+    rp = cases.small_params()
+    base = replace(model.Params(), **{k: getattr(rp, k) for k in model.Params.__dataclass_fields__
+                                      if hasattr(rp, k)})
+    return replace(base, sigma_s=1e-14, n_per_batch=400, n_batches=12, **kw)
+
+
+def same_result(a, b):
+    for key in ("rho_raw", "rho", "phi", "Ex", "Ey"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    for family in ("hist", "charge_hist", "charge_monitor_hist"):
+        for key, values in b[family].items():
+            np.testing.assert_array_equal(a[family][key], values, err_msg=f"{family}/{key}")
+
+
+def test_device_monitors_match_host_monitors():
+    _case("device monitors")
+    p = small()
+    dev = charging.run(p, log=None, device_monitors=True)
+    host = charging.run(p, log=None, device_monitors=False)
+    for key in host["hist"]:
+        np.testing.assert_array_equal(dev["hist"][key], host["hist"][key], err_msg=key)
+    for key, v in host["charge_monitor_hist"].items():
+        np.testing.assert_allclose(dev["charge_monitor_hist"][key], v, rtol=1e-12,
+                                   atol=1e-12 * max(np.abs(v).max(), 1e-30), err_msg=key)
+    np.testing.assert_array_equal(dev["rho_raw"], host["rho_raw"])
+
+
+def test_resume_equals_continuous_run(tmp_path):
+    _case("resume")
+    p = small()
+    full = charging.run(p, log=None)
+    part = charging.run(replace(p, n_batches=5), log=None)
+    path = results.save_results(part, str(tmp_path / "part"))
+    loaded, lp = results.load_results(path)
+    resumed = charging.run(replace(p, n_batches=7), log=None, resume=loaded)
+    assert resumed["start_batch"] == 5 and resumed["checkpoint"]["batches_completed"] == 12
+    same_result(resumed, full)
+
+
+def test_result_file_round_trip(tmp_path):
+    _case("npz round trip")
+    p = small(n_batches=6)
+    res = charging.run(p, log=None)
+    path = results.save_results(res, str(tmp_path / "r"))
+    loaded, lp = results.load_results(path)
+    for key in ("phi", "rho", "rho_raw", "Ex", "Ey", "external_phi", "solid", "mask"):
+        np.testing.assert_array_equal(loaded[key], res[key], err_msg=key)
+    for key in res["hist"]:
+        np.testing.assert_array_equal(loaded["hist"][key], res["hist"][key])
+    assert loaded["backend_used"] == "hip" and lp.n_per_batch == p.n_per_batch
+    with np.load(path, allow_pickle=False) as d, np.load(GOLDEN / "result_ar5.npz", allow_pickle=False) as ref:
+        # the same keys as the reference's own result file (plus params_extra_json)
+        assert set(d.files) - {"params_extra_json"} == set(ref.files) - {"hist_mask sidewall"}
+
+
+def test_reference_result_file_loads_and_resumes():
+    """The reference's result of the AR 5 run (written by its save_results) is
+    read without unpickling and continued on the device with its stored random
+    state."""
+    _case("reference npz resume")
+    res, p = results.load_results(GOLDEN / "result_ar5.npz")
+    assert res["rho_raw"].shape == (80, 200) and len(res["hist"]["t"]) == 50
+    from trench01_amd import configs
+    sampler = configs.ion_velocity_sampler(p.ion_mass_amu, **configs.AR5_ION)
+    more = charging.run(replace(p, n_batches=3), log=None, resume=res, ion_velocity=sampler)
+    assert more["start_batch"] == 50 and len(more["hist"]["t"]) == 53
+    np.testing.assert_array_equal(more["hist"]["t"][:50], res["hist"]["t"])
+    assert np.isfinite(more["hist"]["bottom center"]).all()

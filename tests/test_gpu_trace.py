@@ -137,3 +137,28 @@ def test_stop_before_start_reports_every_particle_lost():
     ci, ce, esc, lost = t(*parts, z, z, ~solid, p, stop=stop)
     assert ci.sum() == ce.sum() == esc.sum() == 0 and lost == 40
     assert t.last_tally["cancelled"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multi_pass_schedule_does_not_change_results(precision):
+    """Stopping particles at a cap and continuing them in later passes (with
+    their full state) gives the single-pass results bit for bit: per-particle
+    status and hit cell, tallies and the total step count."""
+    _print_case(f"multi-pass precision {precision}")
+    c = CASES["ar8-marched"]()
+    p = c["p"]
+    parts = c["particles"]
+    runs = []
+    for pass_steps in (0, 37, 100, 333, 1000):
+        t = HipTracer(c["vac"])
+        t.precision = precision
+        t.pass_steps = pass_steps
+        r = t(*parts, c["Ex"], c["Ey"], c["vac"], p, c["barrier"], keep_status=True)
+        runs.append((r, t.last_status.copy(), t.last_hit.copy(), t.last_tally["steps"]))
+    (r0, s0, h0, n0) = runs[0]
+    for r, st, h, nsteps in runs[1:]:
+        for a, b in zip(r, r0):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(st, s0)
+        np.testing.assert_array_equal(h, h0)
+        assert nsteps == n0

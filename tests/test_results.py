@@ -1,0 +1,36 @@
+"""Result-file format on the CPU: the reference's own result file of the AR 5
+run (tests/golden/result_ar5.npz, written by its save_results) is read without
+unpickling, re-saved in the same format and read back unchanged."""
+from pathlib import Path
+
+import numpy as np
+
+from trench01_amd import results
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_reference_result_file_reads():
+    res, p = results.load_results(GOLDEN / "result_ar5.npz")
+    assert (p.nx, p.trench_d, p.mask_t) == (80, 100, 20)
+    assert res["rho_raw"].shape == (80, 200)
+    assert len(res["hist"]["t"]) == 50 and res["checkpoint"]["batches_completed"] == 50
+    assert res["checkpoint"]["rng_state"]["bit_generator"] == "PCG64"
+    assert set(res["charge_hist"]) == {"left", "right"}
+    assert "surface absolute" in res["charge_monitor_hist"]
+
+
+def test_save_reproduces_the_reference_keys(tmp_path):
+    res, p = results.load_results(GOLDEN / "result_ar5.npz")
+    res["params"]["backend"] = "hip"
+    path = results.save_results(res, str(tmp_path / "again"))
+    with np.load(path, allow_pickle=False) as a, np.load(GOLDEN / "result_ar5.npz", allow_pickle=False) as b:
+        assert set(a.files) - {"params_extra_json"} == set(b.files)
+        for k in b.files:
+            if k in ("params_json", "backend_used", "checkpoint_json"):
+                continue
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    again, q = results.load_results(path)
+    for k in ("phi", "rho_raw", "Ex", "Ey"):
+        np.testing.assert_array_equal(again[k], res[k])
+    assert q == p

@@ -21,7 +21,7 @@ class TraceParams(ctypes.Structure):
     _fields_ = [("dx", ctypes.c_double), ("cfl", ctypes.c_double), ("max_steps", ctypes.c_int32),
                 ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32), ("order", ctypes.c_int32),
-                ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("pass_steps", ctypes.c_int32)]
 
 
 class Tally(ctypes.Structure):
@@ -89,6 +89,9 @@ SIGNATURES = {
     "th_charging_step": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.POINTER(TraceParams), ctypes.POINTER(BatchDiag)]),
     "th_charging_read": (ctypes.c_int, [vp, dp, dp, i32p]),
+    "th_charging_set_monitors": (ctypes.c_int, [vp, ctypes.c_int32, i32p, ctypes.c_int32, i32p, i32p,
+                                                ctypes.c_int32, ctypes.c_int32, i32p]),
+    "th_charging_monitors": (ctypes.c_int, [vp, dp]),
 }
 
 _lib = None

@@ -14,7 +14,8 @@ import json
 import os
 from pathlib import Path
 
-DEFAULT_SPACE = dict(order=(0, 1), refill=(8, 16, 32), check_steps=(4, 8, 16), prio_steps=(0, 400, 1200))
+DEFAULT_SPACE = dict(order=(0, 1), refill=(16, 32), check_steps=(2, 4, 8), prio_steps=(0,),
+                     pass_steps=(0, 150, 250, 400, 600, 1000))
 CACHE = Path(os.environ.get("TRENCH_HIP_TUNE_CACHE", Path.home() / ".cache" / "trench01_amd" / "push_tune.json"))
 
 

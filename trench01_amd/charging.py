@@ -214,6 +214,40 @@ class DeviceCharging:
                                                 ctypes.byref(d)))
         return d
 
+    def set_monitors(self, probes, regions, abs_key="surface"):
+        """Per-batch device monitors: potential at the probe cells, charge per
+        region (regions: name -> flat cells), |charge| of ``abs_key``, and the
+        counts on the trench floor."""
+        geo = self.setup.geo
+        self.monitor_probes = list(probes)
+        self.monitor_regions = list(regions)
+        cells = np.array([ix * geo.ny + iy for ix, iy in probes.values()], dtype=np.int32)
+        idx, ptr = [], [0]
+        for key in self.monitor_regions:
+            s_idx = geo.smap[np.asarray(regions[key], dtype=np.int64)]
+            s_idx = s_idx[s_idx >= 0]          # charge only ever sits on surface cells
+            idx.append(s_idx.astype(np.int32))
+            ptr.append(ptr[-1] + s_idx.size)
+        idx = np.concatenate(idx) if idx else np.zeros(0, np.int32)
+        ptr = np.asarray(ptr, dtype=np.int32)
+        bottom = np.ascontiguousarray(self.setup.bottom_s, dtype=np.int32)
+        self._mon_keep = (cells, ptr, idx, bottom)
+        _lib.check(_lib.load().th_charging_set_monitors(
+            self._h, cells.size, _lib.ptr(cells, _lib.i32p), len(self.monitor_regions), _lib.ptr(ptr, _lib.i32p),
+            _lib.ptr(idx, _lib.i32p), self.monitor_regions.index(abs_key), bottom.size, _lib.ptr(bottom, _lib.i32p)))
+        self._mon_out = np.empty(cells.size + len(self.monitor_regions) + 3)
+
+    def monitors(self):
+        """The last batch's monitors: (probe potentials dict, region charges dict
+        incl. 'surface absolute', floor ion count, floor electron count)."""
+        out = self._mon_out
+        _lib.check(_lib.load().th_charging_monitors(self._h, _lib.ptr(out, _lib.dp)))
+        npb, nr = len(self.monitor_probes), len(self.monitor_regions)
+        probes = dict(zip(self.monitor_probes, out[:npb].tolist()))
+        charges = dict(zip(self.monitor_regions, out[npb:npb + nr].tolist()))
+        charges["surface absolute"] = float(out[npb + nr])
+        return probes, charges, float(out[npb + nr + 1]), float(out[npb + nr + 2])
+
     def read(self, phi=True, counts=True):
         rho = np.empty(self.m)
         ph = np.empty(self.N) if phi else None
@@ -223,12 +257,45 @@ class DeviceCharging:
         return rho, ph, cn
 
 
+# Settings a resumed run may change (trench_charging_2d.py:952-954).
+RESUME_EDITABLE = {"until_steady", "n_batches", "max_batches", "steady_window", "steady_rtol",
+                   "steady_charge_rtol", "steady_charge_atol", "steady_hold", "backend", "cuda_device",
+                   "probe_ions", "probe_energy_eV", "probe_trajectories"}
+
+
+def resume_params(res, p):
+    """The stored run's parameters with the editable settings taken from ``p``;
+    any other difference is an error (trench_charging_2d.py:959-973)."""
+    from dataclasses import fields as dc_fields, replace
+    names = {f.name for f in dc_fields(model.Params)}
+    source = model.Params(**{k: v for k, v in res["params"].items() if k in names})
+    changed = []
+    for k, v in asdict(source).items():
+        entered = getattr(p, k, v)
+        equal = np.isclose(entered, v, rtol=1e-9, atol=0) if isinstance(v, float) else entered == v
+        if k not in RESUME_EDITABLE and not equal and not (k == "backend"):
+            changed.append(k)
+    if changed:
+        raise ValueError("a resumed run must keep the geometry and physics of the stored one: " + ", ".join(changed))
+    return replace(source, **{k: getattr(p, k) for k in RESUME_EDITABLE if hasattr(p, k)})
+
+
 def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=None, ion_velocity=None,
-        device=0):
+        device=0, resume=None, device_monitors=True):
     """Charging run (trench_charging_2d.py:1045-1369) with the batch on the GPU.
-    Returns the reference's result dict (plus ``timing``), or None if stopped
-    before the first batch."""
+    ``resume``: a result of ``run`` or ``results.load_results``; its charge,
+    histories and random state are continued (fixed length: n_batches more;
+    until_steady: max_batches in total).  Returns the reference's result dict
+    (plus ``timing``), or None if stopped before the first batch."""
     log = log or (lambda s: None)
+    start_batch = 0
+    if resume is not None:
+        p = resume_params(resume, p)
+        start_batch = len(resume["hist"]["t"])
+        if p.until_steady and p.max_batches <= start_batch:
+            raise ValueError(f"max_batches counts all batches; raise it above the {start_batch} already run")
+        if not p.until_steady and p.n_batches < 1:
+            raise ValueError("a resumed fixed-length run needs n_batches >= 1")
     if injection not in ("host", "device"):
         raise ValueError("injection must be 'host' or 'device'")
     if p.until_steady and (p.steady_window < 5 or p.steady_hold < 1 or p.steady_rtol <= 0
@@ -243,8 +310,23 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     log(f"particle tracing: HIP ({dev.tracer.name}, device {device}, float64); grid {nx} x {ny}, "
         f"{geo.surface.size} surface cells, {s.green.shape[1]} Green's columns on {s.rows.size} rows")
     m = geo.surface.size
-    dev.set_rho(np.zeros(m))
     rng = np.random.default_rng(p.seed)
+    rho_full = np.zeros(geo.solid.size)
+    n_ok = 0
+    if resume is not None:
+        if not np.array_equal(resume["solid"], geo.solid) or not np.array_equal(resume["mask"], geo.mask):
+            raise ValueError("the stored geometry does not match the parameters")
+        rho_full[:] = np.asarray(resume["rho_raw"], dtype=float).ravel()
+        off = np.setdiff1d(np.flatnonzero(rho_full), geo.surface)
+        if off.size:
+            raise ValueError("stored charge lies outside the surface cells")
+        checkpoint = resume.get("checkpoint") or {}
+        if injection == "host":
+            if not checkpoint.get("rng_state"):
+                raise ValueError("host injection needs the stored random state to resume")
+            rng.bit_generator.state = checkpoint["rng_state"]
+        n_ok = int(checkpoint.get("steady_n_ok", 0) or 0)
+    dev.set_rho(rho_full[geo.surface])
     n = 2 * p.n_per_batch
     dev.reserve(n)
     t_setup = time.perf_counter() - t_setup
@@ -253,14 +335,22 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     hist = {k: [] for k in ("t", "ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "leak", *probes)}
     charge_hist = {"left": [], "right": []}
     charge_monitor_hist = {k: [] for k in (*s.regions, "surface absolute")}
+    if resume is not None:
+        for key in hist:
+            hist[key] = list(np.asarray(resume["hist"].get(key, np.full(start_batch, np.nan)), dtype=float))
+        for key in charge_hist:
+            charge_hist[key] = list(np.asarray(resume["charge_hist"][key], dtype=float))
+        for key in charge_monitor_hist:
+            charge_monitor_hist[key] = list(np.asarray(resume["charge_monitor_hist"][key], dtype=float))
     frac_open = (s.b1 - s.b0) / nx
-    n_max = p.max_batches if p.until_steady else p.n_batches
-    n_ok, converged_at, stopped = 0, None, False
+    if device_monitors:
+        dev.set_monitors(probes, s.regions)
+    n_max = p.max_batches if p.until_steady else start_batch + p.n_batches
+    converged_at, stopped = None, False
     push_ms = update_ms = 0.0
-    rho_full = np.zeros(geo.solid.size)
     phi = fields.external.ravel().copy()
     t0 = time.time()
-    for b in range(n_max):
+    for b in range(start_batch, n_max):
         if stop is not None and stop.is_set():
             stopped = True
             break
@@ -273,20 +363,26 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
             raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
         push_ms += d.push_ms
         update_ms += d.update_ms
-        rho_s, phi, counts = dev.read()
-        rho_full[:] = 0.0
-        rho_full[geo.surface] = rho_s
         hist["t"].append((b + 1) * p.dt_batch)
-        for k, (ix, iy) in probes.items():
-            hist[k].append(phi[ix * ny + iy])
         norm = p.n_per_batch * frac_open
-        hist["ratio_i_bottom"].append(counts[s.bottom_s].sum() / norm)
-        hist["ratio_e_bottom"].append(counts[m + s.bottom_s].sum() / norm)
+        if device_monitors:
+            values, charges, n_i_floor, n_e_floor = dev.monitors()
+            for k in probes:
+                hist[k].append(values[k])
+        else:
+            rho_s, phi, counts = dev.read()
+            rho_full[:] = 0.0
+            rho_full[geo.surface] = rho_s
+            for k, (ix, iy) in probes.items():
+                hist[k].append(phi[ix * ny + iy])
+            n_i_floor, n_e_floor = counts[s.bottom_s].sum(), counts[m + s.bottom_s].sum()
+            charges = monitors.sample_charges(rho_full, s.regions, p.dx)
+        hist["ratio_i_bottom"].append(n_i_floor / norm)
+        hist["ratio_e_bottom"].append(n_e_floor / norm)
         hist["esc_e"].append(d.tally.escaped[1] / p.n_per_batch)
         hist["lost"].append(d.tally.lost / (2 * p.n_per_batch))
         hist["leak"].append(d.leak_dq / p.dt_batch / (E_CHARGE * p.flux * (s.e1 - s.e0) * p.dx)
                             if s.leak is not None else 0.0)
-        charges = monitors.sample_charges(rho_full, s.regions, p.dx)
         for side in ("left", "right"):
             charge_hist[side].append(charges[side])
         for key, q in charges.items():
@@ -311,16 +407,23 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
             if n_ok >= p.steady_hold:
                 converged_at = b + 1
         if progress is not None:
-            progress(dict(batch=b + 1, start_batch=0, n_max=n_max, elapsed=time.time() - t0,
+            if device_monitors:
+                rho_s, phi, _ = dev.read(counts=False)
+                rho_full[:] = 0.0
+                rho_full[geo.surface] = rho_s
+            progress(dict(batch=b + 1, start_batch=start_batch, n_max=n_max, elapsed=time.time() - t0,
                           phi=phi.reshape(nx, ny).copy(), rho=rho_full.reshape(nx, ny).copy(),
                           hist={k: np.array(v) for k, v in hist.items()},
                           charge_hist={k: np.array(v) for k, v in charge_hist.items()},
                           check=check_info, converged=converged_at is not None, probes=list(probes)))
         if converged_at is not None:
             break
-    if not hist["t"]:
+    if len(hist["t"]) == start_batch:
         return None
     wall = time.time() - t0
+    rho_s, phi, _ = dev.read(counts=False)
+    rho_full[:] = 0.0
+    rho_full[geo.surface] = rho_s
     hist = {k: np.array(v) for k, v in hist.items()}
     Wl = min(p.steady_window, len(hist["t"]))
     steady = {k: float(hist[k][-Wl:].mean()) for k in probes}
@@ -334,16 +437,16 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
         induced = (fields.A @ phi_full.ravel()).reshape(rho.shape) - fields.boundary_rhs
         shown[fields.cond] = (-EPS0 * induced)[fields.cond]
     t_conv = converged_at * p.dt_batch if converged_at else np.nan
-    nb = len(hist["t"])
+    nb = len(hist["t"]) - start_batch
     return dict(phi=phi_full, mean_phi=phi_full, rho=shown, Ex=Ex, Ey=Ey, rho_raw=rho,
                 external_phi=fields.external, voltages=fields.volts,
                 charge_hist={k: np.array(v) for k, v in charge_hist.items()},
                 charge_monitor_hist={k: np.array(v) for k, v in charge_monitor_hist.items()},
                 steady_criterion_version=STEADY_CRITERION_VERSION,
-                checkpoint=dict(version=1, batches_completed=nb,
+                checkpoint=dict(version=1, batches_completed=len(hist["t"]),
                                 rng_state=rng.bit_generator.state if injection == "host" else None,
                                 phase_rng_state=None, steady_n_ok=n_ok),
-                start_batch=0, params=asdict(p), converged=converged_at is not None, backend_used="hip",
+                start_batch=start_batch, params=asdict(p), converged=converged_at is not None, backend_used="hip",
                 solid=geo.solid, mask=geo.mask, i0=geo.i0, i1=geo.i1, hist=hist, probes=probes, steady=steady,
                 t_conv=t_conv, stopped=stopped, circuit=None,
                 timing=dict(setup_s=t_setup, loop_s=wall, batches=nb, push_ms=push_ms, update_ms=update_ms,

@@ -58,6 +58,8 @@ struct th_tracer {
     double *d_x = nullptr, *d_y = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_qm = nullptr;
     int8_t *d_spc = nullptr, *d_status = nullptr;
     int32_t *d_hit = nullptr;
+    th::Survivor *d_surv[2] = {nullptr, nullptr};   // multi-pass carry lists (ping-pong)
+    unsigned *d_surv_count = nullptr;                // per-pass list lengths
     // tallies
     uint32_t *d_counts = nullptr, *d_overflow = nullptr;
     unsigned long long *d_scal = nullptr;
@@ -88,24 +90,32 @@ void surface_cells(const std::vector<uint8_t> &vac, int nx, int ny, std::vector<
         }
 }
 
+constexpr int kMaxPasses = 32;
+
 void free_particles(th_tracer *t) {
     for (void *p : {(void *)t->d_x, (void *)t->d_y, (void *)t->d_vx, (void *)t->d_vy, (void *)t->d_qm,
-                    (void *)t->d_spc, (void *)t->d_status, (void *)t->d_hit})
+                    (void *)t->d_spc, (void *)t->d_status, (void *)t->d_hit, (void *)t->d_surv[0],
+                    (void *)t->d_surv[1], (void *)t->d_surv_count})
         if (p) (void)hipFree(p);
     t->d_x = t->d_y = t->d_vx = t->d_vy = t->d_qm = nullptr;
     t->d_spc = t->d_status = nullptr;
     t->d_hit = nullptr;
+    t->d_surv[0] = t->d_surv[1] = nullptr;
+    t->d_surv_count = nullptr;
     t->cap = 0;
 }
 
 int reserve_particles(th_tracer *t, int64_t n) {
     if (n <= t->cap && n > 0) return TH_OK;
+    if (n > INT32_MAX) return fail(TH_EINVAL, "at most 2^31-1 particles per pool");
     free_particles(t);
     int64_t cap = std::max<int64_t>(n, 1024);
     int rc;
     if ((rc = dev_alloc(&t->d_x, cap)) || (rc = dev_alloc(&t->d_y, cap)) || (rc = dev_alloc(&t->d_vx, cap)) ||
         (rc = dev_alloc(&t->d_vy, cap)) || (rc = dev_alloc(&t->d_qm, cap)) || (rc = dev_alloc(&t->d_spc, cap)) ||
-        (rc = dev_alloc(&t->d_status, cap)) || (rc = dev_alloc(&t->d_hit, cap))) {
+        (rc = dev_alloc(&t->d_status, cap)) || (rc = dev_alloc(&t->d_hit, cap)) ||
+        (rc = dev_alloc(&t->d_surv[0], cap)) || (rc = dev_alloc(&t->d_surv[1], cap)) ||
+        (rc = dev_alloc(&t->d_surv_count, kMaxPasses + 1))) {
         free_particles(t);
         return rc;
     }
@@ -335,20 +345,50 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
         w.prio_steps = p->prio_steps > 0 ? p->prio_steps : 0;
         w.check_steps = p->check_steps > 0 ? std::min(p->check_steps, 256) : th::kStepsPerCheck;
         w.rec = rec ? *rec : th::Record{};
-        int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
+        w.step_cap = p->max_steps;
+        w.in = nullptr;
+        w.in_count = nullptr;
+        w.out = nullptr;
+        w.out_count = nullptr;
         int rc;
-        for (int64_t b = off; b < off + n; b += chunk) {
-            if (stop && *stop) {
-                tally->cancelled = 1;
-                break;
+        const bool passes = !stop && !rec && p->pass_steps > 0 && p->pass_steps < p->max_steps;
+        if (passes) {
+            // Multi-pass: caps pass_steps, 2*pass_steps, ... up to max_steps; pass k
+            // continues the survivors of pass k-1.  The passes are stream-ordered;
+            // each reads its list length from device memory.
+            TH_HIP(hipMemsetAsync(t->d_surv_count, 0, sizeof(unsigned) * (kMaxPasses + 1), t->stream));
+            w.begin = off;
+            w.end = off + n;
+            int64_t cap = p->pass_steps;
+            for (int pass = 0;; ++pass) {
+                bool last = cap >= p->max_steps || pass == kMaxPasses - 1;
+                w.step_cap = last ? p->max_steps : (int)cap;
+                w.in = pass ? t->d_surv[(pass - 1) & 1] : nullptr;
+                w.in_count = pass ? t->d_surv_count + pass - 1 : nullptr;
+                w.out = last ? nullptr : t->d_surv[pass & 1];
+                w.out_count = last ? nullptr : t->d_surv_count + pass;
+                TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
+                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false)))
+                    return rc;
+                if (last) break;
+                cap = std::min<int64_t>(2 * cap, p->max_steps);
             }
-            w.begin = b;
-            w.end = std::min(off + n, b + chunk);
-            TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-            if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, rec != nullptr)))
-                return rc;
-            if (stop) TH_HIP(hipStreamSynchronize(t->stream));
-            *traced = w.end;
+            *traced = off + n;
+        } else {
+            int64_t chunk = stop ? (p->chunk > 0 ? p->chunk : 65536) : n;
+            for (int64_t b = off; b < off + n; b += chunk) {
+                if (stop && *stop) {
+                    tally->cancelled = 1;
+                    break;
+                }
+                w.begin = b;
+                w.end = std::min(off + n, b + chunk);
+                TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
+                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, rec != nullptr)))
+                    return rc;
+                if (stop) TH_HIP(hipStreamSynchronize(t->stream));
+                *traced = w.end;
+            }
         }
     }
     TH_HIP(hipEventRecord(t->ev1, t->stream));
@@ -511,6 +551,11 @@ struct th_charging {
     double *rho = nullptr, *rho_prev = nullptr, *qcol = nullptr, *phi = nullptr, *diag = nullptr;
     hipEvent_t ev2 = nullptr;
     int update_lds = 0;
+    // monitors
+    th::MonitorDev mon{};
+    int32_t *mon_buf = nullptr;     // probe cells, region ptr, region idx, bottom (one allocation)
+    double *mon_out = nullptr;
+    std::vector<double> mon_host;
 };
 
 namespace {
@@ -552,6 +597,8 @@ void th_charging_destroy(th_charging *c) {
                     (void *)c->diag})
         if (p) (void)hipFree(p);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->mon_buf) (void)hipFree(c->mon_buf);
+    if (c->mon_out) (void)hipFree(c->mon_out);
     delete c;
 }
 
@@ -646,11 +693,18 @@ int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_p
     hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
     TH_HIP(hipGetLastError());
     if ((rc = launch_field(c))) return rc;
+    if (c->mon_out) {
+        hipLaunchKernelGGL(th::monitor_kernel, dim3(1), dim3(256), 0, t->stream, c->d, c->mon);
+        TH_HIP(hipGetLastError());
+    }
     TH_HIP(hipEventRecord(c->ev2, t->stream));
     unsigned long long scal[th::kScalCount];
     double diag[th::kDiagCount];
     TH_HIP(hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream));
     TH_HIP(hipMemcpyAsync(diag, c->diag, sizeof diag, hipMemcpyDeviceToHost, t->stream));
+    if (c->mon_out)
+        TH_HIP(hipMemcpyAsync(c->mon_host.data(), c->mon_out, sizeof(double) * c->mon_host.size(),
+                              hipMemcpyDeviceToHost, t->stream));
     TH_HIP(hipStreamSynchronize(t->stream));
     fill_tally(scal, offset + n - traced, &out->tally);
     out->leak_dq = diag[th::kDiagLeakDQ];
@@ -869,5 +923,66 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
         }
     }
     fill_tally(scal, n - traced, tally);
+    return TH_OK;
+}
+
+extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *probe_cells,
+                                        int32_t n_regions, const int32_t *region_ptr, const int32_t *region_idx,
+                                        int32_t abs_region, int32_t n_bottom, const int32_t *bottom_s) {
+    if (!c || n_probes < 0 || n_regions < 0 || n_bottom < 0) return fail(TH_EINVAL, "invalid argument");
+    if ((n_probes && !probe_cells) || !region_ptr || (n_bottom && !bottom_s))
+        return fail(TH_EINVAL, "monitor array is NULL");
+    if (abs_region < -1 || abs_region >= n_regions) return fail(TH_EINVAL, "abs_region out of range");
+    th_tracer *t = c->t;
+    for (int i = 0; i < n_probes; ++i)
+        if (probe_cells[i] < 0 || probe_cells[i] >= t->N) return fail(TH_EINVAL, "probe cell out of range");
+    if (region_ptr[0] != 0) return fail(TH_EINVAL, "region_ptr must start at 0");
+    for (int r = 0; r < n_regions; ++r)
+        if (region_ptr[r + 1] < region_ptr[r]) return fail(TH_EINVAL, "region_ptr must not decrease");
+    int32_t n_idx = region_ptr[n_regions];
+    if (n_idx && !region_idx) return fail(TH_EINVAL, "region_idx is NULL");
+    for (int j = 0; j < n_idx; ++j)
+        if (region_idx[j] < 0 || region_idx[j] >= t->m) return fail(TH_EINVAL, "region index out of range");
+    for (int j = 0; j < n_bottom; ++j)
+        if (bottom_s[j] < 0 || bottom_s[j] >= t->m) return fail(TH_EINVAL, "floor index out of range");
+    TH_HIP(hipSetDevice(t->device));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    if (c->mon_buf) (void)hipFree(c->mon_buf);
+    if (c->mon_out) (void)hipFree(c->mon_out);
+    c->mon_buf = nullptr;
+    c->mon_out = nullptr;
+    size_t total = (size_t)n_probes + n_regions + 1 + n_idx + n_bottom;
+    std::vector<int32_t> buf(std::max<size_t>(total, 1));
+    size_t o = 0;
+    if (n_probes) memcpy(buf.data() + o, probe_cells, sizeof(int32_t) * n_probes);
+    size_t o_ptr = o + n_probes;
+    memcpy(buf.data() + o_ptr, region_ptr, sizeof(int32_t) * (n_regions + 1));
+    size_t o_idx = o_ptr + n_regions + 1;
+    if (n_idx) memcpy(buf.data() + o_idx, region_idx, sizeof(int32_t) * n_idx);
+    size_t o_bot = o_idx + n_idx;
+    if (n_bottom) memcpy(buf.data() + o_bot, bottom_s, sizeof(int32_t) * n_bottom);
+    int rc;
+    if ((rc = dev_alloc(&c->mon_buf, buf.size()))) return rc;
+    size_t n_out = (size_t)n_probes + n_regions + 3;
+    if ((rc = dev_alloc(&c->mon_out, n_out))) return rc;
+    TH_HIP(hipMemcpy(c->mon_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
+    TH_HIP(hipMemset(c->mon_out, 0, sizeof(double) * n_out));
+    c->mon.n_probes = n_probes;
+    c->mon.n_regions = n_regions;
+    c->mon.abs_region = abs_region;
+    c->mon.n_bottom = n_bottom;
+    c->mon.probe_cells = c->mon_buf + o;
+    c->mon.region_ptr = c->mon_buf + o_ptr;
+    c->mon.region_idx = c->mon_buf + o_idx;
+    c->mon.bottom_s = c->mon_buf + o_bot;
+    c->mon.out = c->mon_out;
+    c->mon_host.assign(n_out, 0.0);
+    return TH_OK;
+}
+
+extern "C" int th_charging_monitors(const th_charging *c, double *out) {
+    if (!c || !out) return fail(TH_EINVAL, "invalid argument");
+    if (!c->mon_out) return fail(TH_ESTATE, "no monitors set");
+    memcpy(out, c->mon_host.data(), sizeof(double) * c->mon_host.size());
     return TH_OK;
 }

@@ -210,3 +210,49 @@ __global__ void field_pack_kernel(ChargeDev d) {
 }
 
 }  // namespace th
+
+namespace th {
+
+// Per-batch monitors: probe potentials, integrated charge per region, the
+// absolute charge of one region, and the ion / electron counts on the trench
+// floor (trench_charging_2d.py:1272-1285, :871-874).
+struct MonitorDev {
+    int n_probes, n_regions, abs_region, n_bottom;
+    const int32_t *probe_cells;   // [n_probes] flat cells
+    const int32_t *region_ptr;    // [n_regions + 1] CSR offsets into region_idx
+    const int32_t *region_idx;    // surface indices
+    const int32_t *bottom_s;      // [n_bottom] surface indices of the floor cells
+    double *out;                  // [n_probes + n_regions + 3]
+};
+
+__global__ void __launch_bounds__(256) monitor_kernel(ChargeDev d, MonitorDev mo) {
+    __shared__ double scratch[40];
+    const double dx2 = d.dx * d.dx;
+    for (int i = threadIdx.x; i < mo.n_probes; i += blockDim.x) mo.out[i] = d.phi[mo.probe_cells[i]];
+    for (int r = 0; r < mo.n_regions; ++r) {
+        double part = 0.0, apart = 0.0;
+        for (int j = mo.region_ptr[r] + threadIdx.x; j < mo.region_ptr[r + 1]; j += blockDim.x) {
+            double v = d.rho[mo.region_idx[j]];
+            part += v;
+            apart += fabs(v);
+        }
+        double sum = block_sum(part, scratch);
+        double asum = r == mo.abs_region ? block_sum(apart, scratch) : 0.0;
+        if (threadIdx.x == 0) {
+            mo.out[mo.n_probes + r] = sum * dx2;
+            if (r == mo.abs_region) mo.out[mo.n_probes + mo.n_regions] = asum * dx2;
+        }
+    }
+    double ni = 0.0, ne = 0.0;
+    for (int j = threadIdx.x; j < mo.n_bottom; j += blockDim.x) {
+        ni += (double)d.counts[mo.bottom_s[j]];
+        ne += (double)d.counts[d.m + mo.bottom_s[j]];
+    }
+    double si = block_sum(ni, scratch), se = block_sum(ne, scratch);
+    if (threadIdx.x == 0) {
+        mo.out[mo.n_probes + mo.n_regions + 1] = si;
+        mo.out[mo.n_probes + mo.n_regions + 2] = se;
+    }
+}
+
+}  // namespace th

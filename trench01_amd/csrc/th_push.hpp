@@ -61,6 +61,14 @@ struct Record {
     int stride;              // max_steps + 1
 };
 
+// A particle carried from one pass of the push to the next (multi-pass
+// schedule): its full state and step count, so the next pass continues it
+// bit for bit.
+struct Survivor {
+    double x, y, vx, vy;
+    int32_t pid, k;
+};
+
 struct Work {
     const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
     const int8_t *__restrict__ spc;
@@ -73,6 +81,14 @@ struct Work {
     int max_steps, refill;
     int order, prio_steps, check_steps;
     Record rec;                          // record mode only
+    // Multi-pass schedule: particles still live after step_cap steps are
+    // appended to `out` instead of being counted; a later pass reads them from
+    // `in` (its length in *in_count).  step_cap >= max_steps: single pass.
+    int step_cap;
+    const Survivor *in;
+    const unsigned *in_count;
+    Survivor *out;
+    unsigned *out_count;
 };
 
 __device__ __forceinline__ int wrap_cell(int i, int n) {
@@ -326,9 +342,11 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
 
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
+    const int64_t q_begin = w.in ? 0 : w.begin;
+    const int64_t q_end = w.in ? (int64_t)*w.in_count : w.end;
     int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
     StateT<Real> s;
-    int sp = 0, k = 0;
+    int sp = 0, k = 0, k0 = 0;    // step count, and its value when this pass took the particle
     int slot = -1, plen = 0;      // record mode: probe slot and path length
     double xp = 0.0, yp = 0.0;    // record mode: last recorded point
     unsigned long long my_steps = 0;
@@ -342,11 +360,21 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
             base = __shfl(base, leader);
             if (pid == -1) {
                 int64_t q = (int64_t)(base + (unsigned long long)__popcll(idle & below));
-                q = w.order ? w.end - 1 - q : w.begin + q;
-                if (q >= w.begin && q < w.end) {
-                    pid = q;
-                    s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
-                    s.qm = (Real)w.qm[q];
+                q = w.order ? q_end - 1 - q : q_begin + q;
+                if (q >= q_begin && q < q_end) {
+                    if (w.in) {
+                        const Survivor &v = w.in[q];
+                        pid = v.pid;
+                        s.x = (Real)v.x; s.y = (Real)v.y; s.vx = (Real)v.vx; s.vy = (Real)v.vy;
+                        k = v.k;
+                        k0 = k;
+                    } else {
+                        pid = q;
+                        s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
+                        k = 0;
+                        k0 = 0;
+                    }
+                    s.qm = (Real)w.qm[pid];
                     if constexpr (MODE == kModeF32) {
                         s.q = s.x * (float)g.inv_dx;
                         s.r = s.y * (float)g.inv_dx;
@@ -354,12 +382,11 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         s.q = div_dx(s.x, g.dx, g.inv_dx);
                         s.r = div_dx(s.y, g.dx, g.inv_dx);
                     }
-                    sp = w.spc[q];
-                    k = 0;
+                    sp = w.spc[pid];
                     if constexpr (RECORD) {
                         xp = s.x;
                         yp = s.y;
-                        slot = w.rec.slots[q];
+                        slot = w.rec.slots[pid];
                         plen = 0;
                         if (slot >= 0) {
                             double *path = w.rec.paths + 2 * (int64_t)slot * w.rec.stride;
@@ -408,7 +435,18 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         w.rec.vel[2 * pid + 1] = s.vy;
                         if (slot >= 0) w.rec.path_len[slot] = plen;
                     }
-                    my_steps += (unsigned long long)k;
+                    my_steps += (unsigned long long)(k - k0);
+                    pid = -1;
+                    break;
+                }
+                if (k >= w.step_cap) {   // carry to the next pass
+                    unsigned slot_out = atomicAdd(w.out_count, 1u);
+                    Survivor v;
+                    v.x = (double)s.x; v.y = (double)s.y; v.vx = (double)s.vx; v.vy = (double)s.vy;
+                    v.pid = (int32_t)pid;
+                    v.k = k;
+                    w.out[slot_out] = v;
+                    my_steps += (unsigned long long)(k - k0);
                     pid = -1;
                     break;
                 }

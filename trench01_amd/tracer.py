@@ -29,11 +29,13 @@ class HipTracer:
 
     #: 0 = FP64 push, bit-identical with the reference; 1 = FP32 push.
     precision = 0
-    #: push schedule (see trench01_amd.autotune); 0 = library default
-    refill = 0
+    #: push schedule (see trench01_amd.autotune); the defaults are the fastest
+    #: of the schedule sweep on the AR 8 workload (MI355X)
+    refill = 32
     order = 1
     prio_steps = 0
-    check_steps = 0
+    check_steps = 8
+    pass_steps = 250
 
     def __init__(self, vac, device=0):
         L = _lib.load()
@@ -181,7 +183,7 @@ class HipTracer:
                                 tapered=int(tapered_of(p)), barrier=int(bool(barrier)),
                                 precision=int(self.precision), refill=int(self.refill), chunk=0,
                                 order=int(self.order), prio_steps=int(self.prio_steps),
-                                check_steps=int(self.check_steps))
+                                check_steps=int(self.check_steps), pass_steps=int(self.pass_steps))
 
     def set_fields(self, Ex, Ey, barrier=None):
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
