@@ -24,7 +24,7 @@ def small(**kw):
     rp = cases.small_params()
     base = replace(model.Params(), **{k: getattr(rp, k) for k in model.Params.__dataclass_fields__
                                       if hasattr(rp, k)})
-    return replace(base, sigma_s=1e-14, n_per_batch=400, n_batches=12, **kw)
+    return replace(base, **{**dict(sigma_s=1e-14, n_per_batch=400, n_batches=12), **kw})
 
 
 def same_result(a, b):
