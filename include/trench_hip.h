@@ -46,6 +46,7 @@ typedef struct th_trace_params {
     int32_t check_steps;  /* push steps between refill checks, 0 = library default */
     int32_t pass_steps;   /* multi-pass schedule: first pass stops particles at this many steps and
                              later passes continue the survivors with doubling caps; 0 = one pass */
+    int32_t rf;           /* 1: RF full-cycle push (th_set_rf, launch times per particle), FP64 only */
 } th_trace_params;
 
 /* Scalar tallies of one trace call (trench_cuda.py:119-122). */
@@ -108,6 +109,35 @@ int th_trace_record(th_tracer *t, const double *x, const double *y, const double
                     int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
                     const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths);
 
+/* RF full-cycle push (field_mode "circuit_rf"): the basis of unit substrate
+ * and mask voltage responses -- Ex, Ey and (for the barrier wall model) phi,
+ * each [2][nx*ny] -- the voltage wave [n_wave][2] (deviation from the period
+ * mean, substrate then mask), the period and the step cap dt_max.  Replaces
+ * the rf uploads of CudaTracer._trace (trench_cuda.py:65-71) and validates as
+ * trench_rf.validate (trench_rf.py:21-32).  n_wave = 0 clears the RF state. */
+int th_set_rf(th_tracer *t, const double *Ex, const double *Ey, const double *phi /* nullable */,
+              const double *wave, int64_t n_wave, double period, double dt_max);
+
+/* th_trace with the RF full-cycle push (params.rf = 1): each particle starts
+ * its phase clock at initial_time[k] mod period (trench_cuda.py:69;
+ * trench_charging_2d.py:556-602, :629-632; kernel trench_trace.cu:97-110,
+ * :131-136). */
+int th_trace_rf(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                const double *qm, const int8_t *spc, const double *initial_time, int64_t n,
+                const th_trace_params *p, const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e,
+                th_tally *tally, int8_t *status /* nullable */, int32_t *hit /* nullable */);
+
+/* th_trace_record with the RF push: also the flight time to the recorded
+ * point and the phase clock there, in [0, period), per particle
+ * (trench_charging_2d.py:575-577, :680-682; trench_cuda.py:134-137).
+ * Untraced particles keep flight time 0 and their launch clock. */
+int th_trace_rf_record(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                       const double *qm, const int8_t *spc, const double *initial_time, int64_t n,
+                       const th_trace_params *p, const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e,
+                       th_tally *tally, int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
+                       const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths,
+                       double *flight_time, double *phase_clock);
+
 /* Device-resident particle pool, so a batch can be traced with its states
  * already in HBM.  th_particles_reserve sizes the pool (contents are lost);
  * th_particles_upload copies n host particles to pool[offset, offset+n).
@@ -116,6 +146,11 @@ int th_trace_record(th_tracer *t, const double *x, const double *y, const double
 int th_particles_reserve(th_tracer *t, int64_t capacity);
 int th_particles_upload(th_tracer *t, int64_t offset, const double *x, const double *y, const double *vx,
                         const double *vy, const double *qm, const int8_t *spc, int64_t n);
+
+/* Launch times of pool[offset, offset+n) for RF traces of the resident pool
+ * (th_trace_resident and th_charging_step with params.rf = 1); the reference
+ * draws them per batch (trench_charging_2d.py:1231-1234). */
+int th_particles_upload_times(th_tracer *t, int64_t offset, const double *initial_time, int64_t n);
 
 /* Traces pool[offset, offset+n) with the fields last set.  Same outputs as
  * th_trace; cnt_i / cnt_e may be NULL to leave the tallies on the device

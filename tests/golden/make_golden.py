@@ -14,6 +14,12 @@ vector pins a reference function that the reference's own fixtures
 * ``setup_*.npz``   the conductor-collapsed LU (make_solver :297-309),
   conductor_response / C_cond (:807-812, :1089), leak operator
   (build_leakage :768-803) and conductor_step (:815-830) on small geometries.
+* ``rftrace_*.npz`` trace_cpu with the RF full-cycle push (rf=, :556-602,
+  :629-632, :654-682): circuit_rf fields (setup_fields :361-371, :397-412) of
+  a sinusoidal circuit waveform, launch times from trench_rf.launch_times,
+  charged sidewalls; record-mode outputs incl. flight time and phases.  Run
+  with single-threaded BLAS (the bound's matrix product rounds per position;
+  see oracle/trace_ref.c).
 * ``run_ar5.npz``   a whole run() (:1045-1369) of the plain AR 5 workload
   (80x200, conductive mask, sigma_s 0, barrier wall, 100 eV ions with 3 degree
   polar spread, Te 4 eV), 50 batches of 1e4 particles per species, seed 0;
@@ -97,6 +103,30 @@ TRACE_CASES = {
 }
 
 
+RF_CASES = {
+    # name: (params changes, sidewall charge left/right [C/m^3])
+    "pillar_barrier": (dict(nx=20, trench_w=6, trench_d=12, floor_t=4, mask_t=4, n_vac=8, side_bc="pillar",
+                               wall_model="barrier"), (2e7, -1e7)),
+    "taper_absorb": (dict(nx=40, trench_w=16, trench_d=20, floor_t=4, mask_t=12, n_vac=10,
+                             trench_taper_deg=float(np.degrees(np.arctan(.25))),
+                             mask_taper_deg=float(np.degrees(np.arctan(.5))), wall_model="absorb"), (2e7, -1e7)),
+    "taper_barrier": (dict(nx=40, trench_w=16, trench_d=20, floor_t=4, mask_t=12, n_vac=10,
+                              trench_taper_deg=float(np.degrees(np.arctan(.25))),
+                              mask_taper_deg=float(np.degrees(np.arctan(.5))), wall_model="barrier"), (3e7, -1e7)),
+}
+
+
+def rf_circuit(period=4e-10, n=256, amplitude=45.0):
+    """One period of a sinusoidal bias in the circuit-waveform layout that
+    setup_fields reads (t, uw, um, up, V1; :313-321, :363-371)."""
+    t = np.arange(n) * period / n
+    phase = 2 * np.pi * t / period
+    up = 12 + 3 * np.cos(phase)
+    uw = -70 - amplitude * np.sin(phase) + up
+    um = -30 - 0.5 * amplitude * np.sin(phase + 0.3) + up
+    return dict(t=t, uw=uw, um=um, up=up, V1=up - um)
+
+
 def params_json(p):
     """The run's Params as JSON (stored with every vector)."""
     return np.array(json.dumps(asdict(p)))
@@ -129,6 +159,44 @@ def write_trace_case(T, name):
                         path_lengths=np.array([len(v) for v in record["paths"]]),
                         paths=np.concatenate(record["paths"]) if record["paths"] else np.zeros((0, 2)))
     print(f"trace_{name}: n={x.size} absorbed={int((record['status'] == 1).sum())} escaped={esc} lost={lost}")
+
+
+def write_rf_case(T, name):
+    from threadpoolctl import threadpool_limits
+    import trench_rf as RF
+    changes, (ql, qr) = RF_CASES[name]
+    p = replace(T.Params(), field_mode="circuit_rf", rf_freq=2.5e9, bias="rf", mask_type="conductor",
+                sigma_s=0.0, n_per_batch=300, max_steps=1000, probe_trajectories=12, **changes)
+    solid, mask, i0, i1 = T.build_geometry(p)
+    fields = T.setup_fields(p, solid, mask, rf_circuit())
+    rho = np.zeros(solid.shape)
+    rho[i0 - 1, p.floor_t:p.floor_t + p.trench_d] = ql
+    rho[i1, p.floor_t:p.floor_t + p.trench_d] = qr
+    phi = fields["solve"]((-rho / T.EPS0).ravel()).reshape(solid.shape) + fields["external"]
+    ex, ey = T.compute_field(phi, ~solid, p)
+    wface = np.where(fields["cond"], 1.0, fields["eps"] / (1 + fields["eps"]))
+    barrier = (phi, wface) if p.wall_model == "barrier" else None
+    sampler = ion_sampler(T, kind="mono", energy_eV=60.0, sigma_deg=3.0)
+    rng = np.random.default_rng(22)
+    x, y, vx, vy, qm, spc = T.inject(p, p.n_per_batch, solid.shape[1], rng, ion_velocity=sampler)
+    rf = {**fields["rf"], "initial_time": np.concatenate(
+        [RF.launch_times(rng, p.n_per_batch, fields["rf"]["period"]) for _ in range(2)])}
+    record = {}
+    with threadpool_limits(1):
+        ci, ce, esc, lost = T.trace_cpu(x.copy(), y.copy(), vx.copy(), vy.copy(), qm.copy(), spc.copy(), ex,
+                                        ey, ~solid, p, barrier, record, rf=rf)
+    np.savez_compressed(HERE / f"rftrace_{name}.npz", params=params_json(p), solid=solid, mask=mask, rho=rho,
+                        phi=phi, Ex=ex, Ey=ey, wface=wface, x=x, y=y, vx=vx, vy=vy, qm=qm, spc=spc,
+                        rf_period=np.float64(rf["period"]), rf_dt_max=np.float64(rf["dt_max"]),
+                        rf_wave=rf["wave"], rf_Ex=rf["Ex"], rf_Ey=rf["Ey"], rf_phi=rf["phi"],
+                        rf_initial_time=rf["initial_time"], cnt_i=ci, cnt_e=ce, escaped=esc,
+                        lost=np.int64(lost), status=record["status"], hit_cells=record["hit_cells"],
+                        endpoints=record["endpoints"], final_velocity=record["final_velocity"],
+                        flight_time=record["flight_time"], final_phase=record["final_phase"],
+                        launch_phase=record["launch_phase"], selected=record["selected"],
+                        path_lengths=np.array([len(v) for v in record["paths"]]),
+                        paths=np.concatenate(record["paths"]) if record["paths"] else np.zeros((0, 2)))
+    print(f"rftrace_{name}: n={x.size} absorbed={int((record['status'] == 1).sum())} escaped={esc} lost={lost}")
 
 
 def write_setup_case(T, name):
@@ -208,11 +276,14 @@ def main():
     ap.add_argument("--only", default=None)
     a = ap.parse_args()
     T = load_reference(a.reference)
-    jobs = [*(f"trace_{k}" for k in TRACE_CASES), "setup_small", "setup_taper", "run_ar5"]
+    jobs = [*(f"trace_{k}" for k in TRACE_CASES), *(f"rftrace_{k}" for k in RF_CASES), "setup_small",
+            "setup_taper", "run_ar5"]
     for job in jobs:
         if a.only and a.only != job:
             continue
-        if job.startswith("trace_"):
+        if job.startswith("rftrace_"):
+            write_rf_case(T, job[8:])
+        elif job.startswith("trace_"):
             write_trace_case(T, job[6:])
         elif job.startswith("setup_"):
             write_setup_case(T, job[6:])

@@ -21,7 +21,8 @@ class TraceParams(ctypes.Structure):
     _fields_ = [("dx", ctypes.c_double), ("cfl", ctypes.c_double), ("max_steps", ctypes.c_int32),
                 ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32), ("order", ctypes.c_int32),
-                ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("pass_steps", ctypes.c_int32)]
+                ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("pass_steps", ctypes.c_int32),
+                ("rf", ctypes.c_int32)]
 
 
 class Tally(ctypes.Structure):
@@ -71,6 +72,13 @@ SIGNATURES = {
     "th_trace_record": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                        i32p, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p, dp, dp, i32p,
                                        ctypes.c_int32, dp, i32p]),
+    "th_set_rf": (ctypes.c_int, [vp, dp, dp, dp, dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double]),
+    "th_trace_rf": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, dp, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                   i32p, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_trace_rf_record": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, dp, ctypes.c_int64,
+                                          ctypes.POINTER(TraceParams), i32p, i64p, i64p, ctypes.POINTER(Tally),
+                                          i8p, i32p, dp, dp, i32p, ctypes.c_int32, dp, i32p, dp, dp]),
+    "th_particles_upload_times": (ctypes.c_int, [vp, ctypes.c_int64, dp, ctypes.c_int64]),
     "th_particles_reserve": (ctypes.c_int, [vp, ctypes.c_int64]),
     "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
     "th_trace_resident": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), i64p,

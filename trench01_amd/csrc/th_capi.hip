@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/trench_hip.h"
@@ -53,11 +54,18 @@ struct th_tracer {
     float4 *d_cells32 = nullptr;
     double *d_wface = nullptr, *d_tmp = nullptr;   // d_tmp: 3*N upload staging
     bool fields = false, barrier_fields = false;
+    // RF full-cycle push: basis records, voltage wave, clock parameters
+    double4 *d_rfcells = nullptr;
+    double2 *d_rfphi = nullptr, *d_wave = nullptr;
+    int n_wave = 0;
+    double period = 0.0, dt_max = 0.0, bound[2] = {0.0, 0.0};
+    bool rf = false, rf_phi = false;
     // particle buffers
     int64_t cap = 0;
     double *d_x = nullptr, *d_y = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_qm = nullptr;
     int8_t *d_spc = nullptr, *d_status = nullptr;
     int32_t *d_hit = nullptr;
+    double *d_t0 = nullptr;                          // RF launch times
     th::Survivor *d_surv[2] = {nullptr, nullptr};   // multi-pass carry lists (ping-pong)
     unsigned *d_surv_count = nullptr;                // per-pass list lengths
     // tallies
@@ -95,9 +103,9 @@ constexpr int kMaxPasses = 32;
 void free_particles(th_tracer *t) {
     for (void *p : {(void *)t->d_x, (void *)t->d_y, (void *)t->d_vx, (void *)t->d_vy, (void *)t->d_qm,
                     (void *)t->d_spc, (void *)t->d_status, (void *)t->d_hit, (void *)t->d_surv[0],
-                    (void *)t->d_surv[1], (void *)t->d_surv_count})
+                    (void *)t->d_surv[1], (void *)t->d_surv_count, (void *)t->d_t0})
         if (p) (void)hipFree(p);
-    t->d_x = t->d_y = t->d_vx = t->d_vy = t->d_qm = nullptr;
+    t->d_x = t->d_y = t->d_vx = t->d_vy = t->d_qm = t->d_t0 = nullptr;
     t->d_spc = t->d_status = nullptr;
     t->d_hit = nullptr;
     t->d_surv[0] = t->d_surv[1] = nullptr;
@@ -113,7 +121,7 @@ int reserve_particles(th_tracer *t, int64_t n) {
     int rc;
     if ((rc = dev_alloc(&t->d_x, cap)) || (rc = dev_alloc(&t->d_y, cap)) || (rc = dev_alloc(&t->d_vx, cap)) ||
         (rc = dev_alloc(&t->d_vy, cap)) || (rc = dev_alloc(&t->d_qm, cap)) || (rc = dev_alloc(&t->d_spc, cap)) ||
-        (rc = dev_alloc(&t->d_status, cap)) || (rc = dev_alloc(&t->d_hit, cap)) ||
+        (rc = dev_alloc(&t->d_status, cap)) || (rc = dev_alloc(&t->d_hit, cap)) || (rc = dev_alloc(&t->d_t0, cap)) ||
         (rc = dev_alloc(&t->d_surv[0], cap)) || (rc = dev_alloc(&t->d_surv[1], cap)) ||
         (rc = dev_alloc(&t->d_surv_count, kMaxPasses + 1))) {
         free_particles(t);
@@ -139,6 +147,14 @@ th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
     g.two_dx = 2.0 * p->dx;
     g.lx = t->nx * p->dx;
     g.ly = t->ny * p->dx;
+    g.rfcells = t->d_rfcells;
+    g.rfphi = t->d_rfphi;
+    g.wave = t->d_wave;
+    g.n_wave = t->n_wave;
+    g.period = t->period;
+    g.dt_max = t->dt_max;
+    g.bound0 = t->bound[0];
+    g.bound1 = t->bound[1];
     return g;
 }
 
@@ -157,23 +173,26 @@ int launch_kernel(th_tracer *t, K kernel, const th::Grid &g, const th::Work &w, 
 
 template <bool B, bool T>
 int launch_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds, int precision,
-                   bool record) {
+                   bool record, bool rf) {
+    if (rf)
+        return record ? launch_kernel(t, th::push64_rf_record_kernel<B, T>, g, w, want, lds)
+                      : launch_kernel(t, th::push64_rf_kernel<B, T>, g, w, want, lds);
     if (precision == 1) return launch_kernel(t, th::push32_kernel<B, T>, g, w, want, lds);
     if (record) return launch_kernel(t, th::push64_record_kernel<B, T>, g, w, want, lds);
     return launch_kernel(t, th::push64_kernel<B, T>, g, w, want, lds);
 }
 
 int launch_push(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered, int precision,
-                bool record) {
+                bool record, bool rf) {
     int64_t n = w.end - w.begin;
     if (n <= 0) return TH_OK;
     size_t lds = sizeof(uint32_t) * 2 * (size_t)t->m;
     int64_t want = (n + th::kBlock - 1) / th::kBlock;
     if (barrier)
-        return tapered ? launch_variant<true, true>(t, g, w, want, lds, precision, record)
-                       : launch_variant<true, false>(t, g, w, want, lds, precision, record);
-    return tapered ? launch_variant<false, true>(t, g, w, want, lds, precision, record)
-                   : launch_variant<false, false>(t, g, w, want, lds, precision, record);
+        return tapered ? launch_variant<true, true>(t, g, w, want, lds, precision, record, rf)
+                       : launch_variant<true, false>(t, g, w, want, lds, precision, record, rf);
+    return tapered ? launch_variant<false, true>(t, g, w, want, lds, precision, record, rf)
+                   : launch_variant<false, false>(t, g, w, want, lds, precision, record, rf);
 }
 
 }  // namespace
@@ -244,7 +263,7 @@ void th_tracer_destroy(th_tracer *t) {
     free_particles(t);
     for (void *p : {(void *)t->d_vac, (void *)t->d_smap, (void *)t->d_cells, (void *)t->d_cells32,
                     (void *)t->d_wface, (void *)t->d_tmp, (void *)t->d_counts, (void *)t->d_overflow,
-                    (void *)t->d_scal})
+                    (void *)t->d_scal, (void *)t->d_rfcells, (void *)t->d_rfphi, (void *)t->d_wave})
         if (p) (void)hipFree(p);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
@@ -300,6 +319,11 @@ int check_params(const th_tracer *t, const th_trace_params *p) {
     if (p->precision != 0 && p->precision != 1) return fail(TH_EINVAL, "precision %d is not available", p->precision);
     if (!t->fields) return fail(TH_ESTATE, "th_set_fields must precede tracing");
     if (p->barrier && !t->barrier_fields) return fail(TH_ESTATE, "barrier wall model needs phi and wface");
+    if (p->rf) {
+        if (!t->rf) return fail(TH_ESTATE, "th_set_rf must precede an RF trace");
+        if (p->precision != 0) return fail(TH_EINVAL, "the RF push is FP64 only");
+        if (p->barrier && !t->rf_phi) return fail(TH_ESTATE, "an RF barrier trace needs the basis potentials");
+    }
     return TH_OK;
 }
 
@@ -334,6 +358,7 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
         th::Grid g = make_grid(t, p);
         th::Work w;
         w.x = t->d_x; w.y = t->d_y; w.vx = t->d_vx; w.vy = t->d_vy; w.qm = t->d_qm; w.spc = t->d_spc;
+        w.t0 = t->d_t0;
         w.status = want_status ? t->d_status : nullptr;
         w.hit = want_hit ? t->d_hit : nullptr;
         w.counts = t->d_counts;
@@ -368,7 +393,7 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
                 w.out = last ? nullptr : t->d_surv[pass & 1];
                 w.out_count = last ? nullptr : t->d_surv_count + pass;
                 TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false)))
+                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false, p->rf != 0)))
                     return rc;
                 if (last) break;
                 cap = std::min<int64_t>(2 * cap, p->max_steps);
@@ -384,7 +409,8 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
                 w.begin = b;
                 w.end = std::min(off + n, b + chunk);
                 TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, rec != nullptr)))
+                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, rec != nullptr,
+                                      p->rf != 0)))
                     return rc;
                 if (stop) TH_HIP(hipStreamSynchronize(t->stream));
                 *traced = w.end;
@@ -447,13 +473,33 @@ int trace_range(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, 
 
 extern "C" {
 
-int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
-             const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
-             const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
-             int32_t *hit) {
+}  // extern "C"
+
+namespace {
+
+// NumPy's float remainder for a divisor L > 0 (npy_divmod).
+double host_py_mod(double a, double L) {
+    double r = fmod(a, L);
+    if (r != 0.0) return r < 0.0 ? r + L : r;
+    return 0.0;
+}
+
+int upload_times(th_tracer *t, int64_t off, const double *t0, int64_t n) {
+    for (int64_t k = 0; k < n; ++k)
+        if (!std::isfinite(t0[k])) return fail(TH_EINVAL, "launch times must be finite");
+    if (n) TH_HIP(hipMemcpyAsync(t->d_t0 + off, t0, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, t->stream));
+    return TH_OK;
+}
+
+// th_trace / th_trace_rf on host arrays (t0: launch times, RF only).
+int trace_host(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+               const double *qm, const int8_t *spc, const double *t0, int64_t n, const th_trace_params *p,
+               const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
+               int32_t *hit) {
     if (!t || !p || !cnt_i || !cnt_e || !tally) return fail(TH_EINVAL, "invalid argument");
     if (n < 0) return fail(TH_EINVAL, "negative particle count");
     if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    if (p->rf && n > 0 && !t0) return fail(TH_EINVAL, "an RF trace needs the launch times");
     int rc = check_params(t, p);
     if (rc) return rc;
     for (int64_t k = 0; k < n; ++k)
@@ -466,8 +512,96 @@ int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, c
     if (n > 0) {
         if ((rc = reserve_particles(t, n))) return rc;
         if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
+        if (p->rf && (rc = upload_times(t, 0, t0, n))) return rc;
     }
     return trace_range(t, 0, n, p, stop, cnt_i, cnt_e, tally, status, hit);
+}
+
+}  // namespace
+
+extern "C" {
+
+int th_trace(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+             const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+             const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
+             int32_t *hit) {
+    if (p && p->rf) return fail(TH_EINVAL, "an RF trace takes launch times: use th_trace_rf");
+    return trace_host(t, x, y, vx, vy, qm, spc, nullptr, n, p, stop, cnt_i, cnt_e, tally, status, hit);
+}
+
+int th_trace_rf(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                const double *qm, const int8_t *spc, const double *initial_time, int64_t n,
+                const th_trace_params *p, const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e,
+                th_tally *tally, int8_t *status, int32_t *hit) {
+    if (p && !p->rf) return fail(TH_EINVAL, "th_trace_rf needs params.rf = 1");
+    return trace_host(t, x, y, vx, vy, qm, spc, initial_time, n, p, stop, cnt_i, cnt_e, tally, status, hit);
+}
+
+int th_set_rf(th_tracer *t, const double *Ex, const double *Ey, const double *phi, const double *wave,
+              int64_t n_wave, double period, double dt_max) {
+    if (!t) return fail(TH_EINVAL, "invalid argument");
+    if (n_wave == 0) {
+        t->rf = false;
+        return TH_OK;
+    }
+    if (!Ex || !Ey || !wave || n_wave < 2 || n_wave > (1 << 24))
+        return fail(TH_EINVAL, "RF needs the basis fields and a wave of 2 to 2^24 phases");
+    if (!(std::isfinite(period) && period > 0.0 && std::isfinite(dt_max) && dt_max > 0.0))
+        return fail(TH_EINVAL, "RF period and dt_max must be positive and finite");
+    const size_t N = (size_t)t->N;
+    double bound[2] = {0.0, 0.0};
+    for (int64_t k = 0; k < 2 * n_wave; ++k) {
+        if (!std::isfinite(wave[k])) return fail(TH_EINVAL, "RF wave must be finite");
+        bound[k & 1] = std::max(bound[k & 1], std::fabs(wave[k]));
+    }
+    for (size_t c = 0; c < 2 * N; ++c)
+        if (!std::isfinite(Ex[c]) || !std::isfinite(Ey[c]) || (phi && !std::isfinite(phi[c])))
+            return fail(TH_EINVAL, "RF basis must be finite");
+    TH_HIP(hipSetDevice(t->device));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    int rc;
+    if (!t->d_rfcells && ((rc = dev_alloc(&t->d_rfcells, N)) || (rc = dev_alloc(&t->d_rfphi, N)))) return rc;
+    if (n_wave > t->n_wave || !t->d_wave) {
+        if (t->d_wave) (void)hipFree(t->d_wave);
+        t->d_wave = nullptr;
+        t->n_wave = 0;
+        if ((rc = dev_alloc(&t->d_wave, (size_t)n_wave))) return rc;
+    }
+    double *stage = nullptr;
+    if ((rc = dev_alloc(&stage, 6 * N))) return rc;
+    hipError_t e = hipMemcpyAsync(stage, Ex, sizeof(double) * 2 * N, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(stage + 2 * N, Ey, sizeof(double) * 2 * N, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && phi)
+        e = hipMemcpyAsync(stage + 4 * N, phi, sizeof(double) * 2 * N, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t->d_wave, wave, sizeof(double) * 2 * (size_t)n_wave, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) {
+        int threads = 256, blocks = (t->N + threads - 1) / threads;
+        hipLaunchKernelGGL(th::pack_rf_kernel, dim3(blocks), dim3(threads), 0, t->stream, stage, stage + 2 * N,
+                           phi ? stage + 4 * N : nullptr, t->d_vac, t->N, t->d_rfcells, t->d_rfphi);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(TH_EHIP, "RF upload: %s", hipGetErrorString(e));
+    t->n_wave = (int)n_wave;
+    t->period = period;
+    t->dt_max = dt_max;
+    t->bound[0] = bound[0];
+    t->bound[1] = bound[1];
+    t->rf = true;
+    t->rf_phi = phi != nullptr;
+    return TH_OK;
+}
+
+int th_particles_upload_times(th_tracer *t, int64_t offset, const double *initial_time, int64_t n) {
+    if (!t || offset < 0 || n < 0 || (n > 0 && !initial_time)) return fail(TH_EINVAL, "invalid argument");
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    TH_HIP(hipSetDevice(t->device));
+    int rc = upload_times(t, offset, initial_time, n);
+    if (rc) return rc;
+    TH_HIP(hipStreamSynchronize(t->stream));
+    return TH_OK;
 }
 
 int th_particles_reserve(th_tracer *t, int64_t capacity) {
@@ -814,13 +948,19 @@ extern "C" int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, doubl
     return TH_OK;
 }
 
-extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
-                               const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
-                               const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
-                               int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
-                               const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths) {
+namespace {
+
+// th_trace_record / th_trace_rf_record (t0, flight, clock: RF only).
+int trace_record_host(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                      const double *qm, const int8_t *spc, const double *t0, int64_t n, const th_trace_params *p,
+                      const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
+                      int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
+                      const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths,
+                      double *flight, double *clock) {
     if (!t || !p || !cnt_i || !cnt_e || !tally || !status || !hit || !endpoints || !final_velocity)
         return fail(TH_EINVAL, "invalid argument");
+    if (p->rf && (!flight || !clock || (n > 0 && !t0)))
+        return fail(TH_EINVAL, "an RF record trace needs launch times, flight_time and phase_clock");
     if (n < 0 || n_probe < 0 || (n_probe > 0 && (!slots || !paths || !path_lengths)))
         return fail(TH_EINVAL, "invalid probe arguments");
     if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
@@ -841,6 +981,10 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
         final_velocity[2 * k + 1] = vy[k];
         status[k] = 3;
         hit[k] = -1;
+        if (p->rf) {
+            flight[k] = 0.0;
+            clock[k] = host_py_mod(t0[k], t->period);
+        }
         if (slots && slots[k] >= 0) {
             paths[2 * (int64_t)slots[k] * stride] = x[k];
             paths[2 * (int64_t)slots[k] * stride + 1] = y[k];
@@ -854,16 +998,19 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
     TH_HIP(hipSetDevice(t->device));
     if ((rc = reserve_particles(t, n))) return rc;
     if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
-    double *d_end = nullptr, *d_vel = nullptr, *d_paths = nullptr;
+    if (p->rf && (rc = upload_times(t, 0, t0, n))) return rc;
+    double *d_end = nullptr, *d_vel = nullptr, *d_paths = nullptr, *d_flight = nullptr, *d_clock = nullptr;
     int32_t *d_slots = nullptr, *d_len = nullptr;
     size_t path_doubles = 2 * (size_t)std::max(n_probe, 1) * stride;
     auto release = [&]() {
-        for (void *q : {(void *)d_end, (void *)d_vel, (void *)d_paths, (void *)d_slots, (void *)d_len})
+        for (void *q : {(void *)d_end, (void *)d_vel, (void *)d_paths, (void *)d_slots, (void *)d_len,
+                        (void *)d_flight, (void *)d_clock})
             if (q) (void)hipFree(q);
     };
     if ((rc = dev_alloc(&d_end, 2 * (size_t)n)) || (rc = dev_alloc(&d_vel, 2 * (size_t)n)) ||
         (rc = dev_alloc(&d_paths, path_doubles)) || (rc = dev_alloc(&d_slots, (size_t)n)) ||
-        (rc = dev_alloc(&d_len, (size_t)std::max(n_probe, 1)))) {
+        (rc = dev_alloc(&d_len, (size_t)std::max(n_probe, 1))) ||
+        (p->rf && ((rc = dev_alloc(&d_flight, (size_t)n)) || (rc = dev_alloc(&d_clock, (size_t)n))))) {
         release();
         return rc;
     }
@@ -879,11 +1026,15 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
                            t->stream);
     if (e == hipSuccess && n_probe > 0)
         e = hipMemcpyAsync(d_len, path_lengths, sizeof(int32_t) * n_probe, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && p->rf)
+        e = hipMemcpyAsync(d_flight, flight, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && p->rf)
+        e = hipMemcpyAsync(d_clock, clock, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
     if (e != hipSuccess) {
         release();
         return fail(TH_EHIP, "record upload: %s", hipGetErrorString(e));
     }
-    th::Record rec{d_end, d_vel, d_slots, d_paths, d_len, stride};
+    th::Record rec{d_end, d_vel, d_slots, d_paths, d_len, stride, d_flight, d_clock};
     int64_t traced = 0;
     rc = launch_trace(t, 0, n, p, stop, true, true, &traced, tally, &rec);
     if (rc) {
@@ -907,6 +1058,10 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
                            t->stream);
     if (e == hipSuccess && n_probe > 0)
         e = hipMemcpyAsync(path_lengths, d_len, sizeof(int32_t) * n_probe, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && p->rf)
+        e = hipMemcpyAsync(flight, d_flight, sizeof(double) * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && p->rf)
+        e = hipMemcpyAsync(clock, d_clock, sizeof(double) * n, hipMemcpyDeviceToHost, t->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
     release();
     if (e != hipSuccess) return fail(TH_EHIP, "record readback: %s", hipGetErrorString(e));
@@ -924,6 +1079,30 @@ extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, c
     }
     fill_tally(scal, n - traced, tally);
     return TH_OK;
+}
+
+}  // namespace
+
+extern "C" int th_trace_record(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                               const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                               const volatile int32_t *stop, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally,
+                               int8_t *status, int32_t *hit, double *endpoints, double *final_velocity,
+                               const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths) {
+    if (p && p->rf) return fail(TH_EINVAL, "an RF trace takes launch times: use th_trace_rf_record");
+    return trace_record_host(t, x, y, vx, vy, qm, spc, nullptr, n, p, stop, cnt_i, cnt_e, tally, status, hit,
+                             endpoints, final_velocity, slots, n_probe, paths, path_lengths, nullptr, nullptr);
+}
+
+extern "C" int th_trace_rf_record(th_tracer *t, const double *x, const double *y, const double *vx,
+                                  const double *vy, const double *qm, const int8_t *spc, const double *initial_time,
+                                  int64_t n, const th_trace_params *p, const volatile int32_t *stop, int64_t *cnt_i,
+                                  int64_t *cnt_e, th_tally *tally, int8_t *status, int32_t *hit, double *endpoints,
+                                  double *final_velocity, const int32_t *slots, int32_t n_probe, double *paths,
+                                  int32_t *path_lengths, double *flight_time, double *phase_clock) {
+    if (p && !p->rf) return fail(TH_EINVAL, "th_trace_rf_record needs params.rf = 1");
+    return trace_record_host(t, x, y, vx, vy, qm, spc, initial_time, n, p, stop, cnt_i, cnt_e, tally, status, hit,
+                             endpoints, final_velocity, slots, n_probe, paths, path_lengths, flight_time,
+                             phase_clock);
 }
 
 extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *probe_cells,

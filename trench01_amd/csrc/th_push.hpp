@@ -18,13 +18,22 @@
 //     for the next step's interpolation (x/dx - 0.5) and for the previous-cell
 //     indices of wall entry; a reflected particle restores them with x0, y0;
 //   * the periodic wrap takes a - L or a + L where that equals the reference's
-//     fmod form exactly (Sterbenz), else falls back to fmod;
+//     fmod form exactly (Sterbenz), else falls back to fmod (the RF clock
+//     wraps the same way);
 //   * cfl*dx, 2*dx, nx*dx and ny*dx are formed once on the host (same products);
 //   * x/dx and y/dx use Markstein's correction (P. Markstein, "Computation of
 //     elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
 //     Dev. 34(1), 1990): with y = RN(1/dx) and a faithful quotient q, one fused
 //     residual step RN(q + (x - q dx) y) is the correctly rounded x/dx, i.e. the
 //     same double as the IEEE division (positions never over- or underflow).
+//
+// RF full-cycle push (trench_charging_2d.py:556-602, :629-632, :654-682;
+// trench_rf.py:5-11): each particle carries a phase clock and its flight
+// time; the field is the frozen field plus the two unit-voltage basis fields
+// weighted by the voltage wave at mid-step, and the step is capped by dt_max
+// and by an acceleration bound over the whole period.  The bound's two-term
+// product is formed as fma(b0, |e0|, b1 |e1|), the rounding NumPy's matrix
+// product gives all but the last n % 4 live particles of a step.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -47,6 +56,12 @@ struct Grid {
     const int32_t *__restrict__ smap;    // cell -> surface index, -1 elsewhere
     int nx, ny, m;
     double dx, inv_dx, cfl_dx, two_dx, lx, ly;   // 1/dx, cfl*dx, 2*dx, nx*dx, ny*dx formed on the host
+    // RF full-cycle push only.
+    const double4 *__restrict__ rfcells;  // {E0x*vac, E0y*vac, E1x*vac, E1y*vac}: substrate / mask basis
+    const double2 *__restrict__ rfphi;    // {phi0, phi1} basis potentials (barrier term)
+    const double2 *__restrict__ wave;     // [n_wave] {substrate, mask} deviation from the period mean
+    int n_wave;
+    double period, dt_max, bound0, bound1;   // bound = max |wave| per column
 };
 
 // Record-mode outputs (trench_trace.cu:149-183, :189-192): the end point of
@@ -59,6 +74,8 @@ struct Record {
     double *paths;           // [n_probe][stride][2]
     int32_t *path_len;       // [n_probe]
     int stride;              // max_steps + 1
+    double *flight;          // RF: [n] flight time to the recorded point
+    double *clock;           // RF: [n] phase clock at the recorded point, in [0, period)
 };
 
 // A particle carried from one pass of the push to the next (multi-pass
@@ -66,12 +83,14 @@ struct Record {
 // bit for bit.
 struct Survivor {
     double x, y, vx, vy;
+    double clock, elapsed;   // RF phase clock and flight time
     int32_t pid, k;
 };
 
 struct Work {
     const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
     const int8_t *__restrict__ spc;
+    const double *__restrict__ t0;       // RF: launch time per particle
     int64_t begin, end;                  // particles [begin, end)
     int8_t *status;                      // nullable
     int32_t *hit;                        // nullable
@@ -109,6 +128,35 @@ __device__ __forceinline__ double wrap_pos(double a, double L) {
     return r < 0.0 ? r + L : r;
 }
 
+// NumPy's float remainder for a divisor L > 0 (npy_divmod): fmod, then a
+// negative remainder is shifted by L and a zero one is +0.
+__device__ __forceinline__ double py_mod(double a, double L) {
+    if (a >= 0.0 && a < L) return a;
+    if (a >= L && a < 2.0 * L) return a - L;   // exact: L <= a < 2L
+    double r = fmod(a, L);
+    if (r != 0.0) return r < 0.0 ? r + L : r;
+    return 0.0;
+}
+
+// Voltage deviations at time `at`: periodic linear interpolation of the wave
+// (trench_rf.py:5-11).
+__device__ __forceinline__ void rf_coefficients(const Grid &g, double at, double &vs, double &vm) {
+    double f = py_mod(at, g.period) / g.period * (double)g.n_wave;
+    double fl = floor(f);
+    int i = (int)fl % g.n_wave;
+    int j = i + 1 == g.n_wave ? 0 : i + 1;
+    double w = f - fl;
+    double2 a = g.wave[i], b = g.wave[j];
+    vs = a.x * (1.0 - w) + b.x * w;
+    vm = a.y * (1.0 - w) + b.y * w;
+}
+
+// Per-particle RF state.
+struct RfClock {
+    double clock, elapsed;   // phase clock in [0, period), flight time
+    double ft, ck;           // record mode: flight time and clock at the recorded point
+};
+
 // x / dx, correctly rounded, from the host-rounded reciprocal (see above).
 __device__ __forceinline__ double div_dx(double a, double dx, double inv_dx) {
     double q = a * inv_dx;
@@ -127,9 +175,10 @@ using State64 = StateT<double>;
 using State32 = StateT<float>;
 
 // One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
-// (escaped).  With RECORD, (xp, yp) is the step's recorded point.
-template <bool BARRIER, bool TAPERED, bool RECORD>
-__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, double &xp, double &yp) {
+// (escaped).  With RECORD, (xp, yp) is the step's recorded point.  With RF the
+// voltage wave acts during the step and rc advances.
+template <bool BARRIER, bool TAPERED, bool RECORD, bool RF>
+__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc) {
     const int nx = g.nx, ny = g.ny;
     // Vacuum-weighted bilinear field at the cell-centred lattice.
     double fx = s.q - 0.5, fy = s.r - 0.5;
@@ -150,8 +199,35 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
     ex = ex / sw;
     ey = ey / sw;
     // CFL-limited trapezoidal push.
-    double ax = s.qm * ex, ay = s.qm * ey;
-    double dt = g.cfl_dx / (th_hypot(s.vx, s.vy) + sqrt(g.two_dx * th_hypot(ax, ay)) + 1.0);
+    double ax, ay, dt;
+    if constexpr (RF) {
+        double4 b00 = g.rfcells[i0 * ny + j0], b10 = g.rfcells[i1 * ny + j0];
+        double4 b01 = g.rfcells[i0 * ny + j1], b11 = g.rfcells[i1 * ny + j1];
+        double e0x = 0.0, e0y = 0.0, e1x = 0.0, e1y = 0.0;
+        e0x += w00 * b00.x; e0y += w00 * b00.y; e1x += w00 * b00.z; e1y += w00 * b00.w;
+        e0x += w10 * b10.x; e0y += w10 * b10.y; e1x += w10 * b10.z; e1y += w10 * b10.w;
+        e0x += w01 * b01.x; e0y += w01 * b01.y; e1x += w01 * b01.z; e1y += w01 * b01.w;
+        e0x += w11 * b11.x; e0y += w11 * b11.y; e1x += w11 * b11.z; e1y += w11 * b11.w;
+        e0x = e0x / sw; e0y = e0y / sw; e1x = e1x / sw; e1y = e1y / sw;
+        // Acceleration bound over the period, then the field at mid-step.
+        double bx = fabs(ex) + fma(g.bound0, fabs(e0x), g.bound1 * fabs(e1x));
+        double by = fabs(ey) + fma(g.bound0, fabs(e0y), g.bound1 * fabs(e1y));
+        double accel = fabs(s.qm) * th_hypot(bx, by);
+        dt = g.cfl_dx / (th_hypot(s.vx, s.vy) + sqrt(g.two_dx * accel) + 1.0);
+        if (!(dt < g.dt_max) && !isnan(dt)) dt = g.dt_max;
+        double vs, vm;
+        rf_coefficients(g, rc.clock + dt / 2.0, vs, vm);
+        ex = ex + (vs * e0x + vm * e1x);
+        ey = ey + (vs * e0y + vm * e1y);
+        ax = s.qm * ex;
+        ay = s.qm * ey;
+        rc.clock = py_mod(rc.clock + dt, g.period);
+        rc.elapsed = rc.elapsed + dt;
+    } else {
+        ax = s.qm * ex;
+        ay = s.qm * ey;
+        dt = g.cfl_dx / (th_hypot(s.vx, s.vy) + sqrt(g.two_dx * th_hypot(ax, ay)) + 1.0);
+    }
     double vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
     const double x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
     double sx = 0.5 * (s.vx + vxn) * dt, sy = 0.5 * (s.vy + vyn) * dt;
@@ -191,6 +267,12 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
             double vn = face_x ? s.vx : s.vy;
             int c = cx * ny + cy, v = vcx * ny + vcy;
             double dphi = g.wface[c] * (g.cells[c].w - g.cells[v].w);
+            if constexpr (RF) {   // the wall potential at the step's end time
+                double cs, cm;
+                rf_coefficients(g, rc.clock, cs, cm);
+                double2 pc = g.rfphi[c], pv = g.rfphi[v];
+                dphi = dphi + g.wface[c] * (cs * (pc.x - pv.x) + cm * (pc.y - pv.y));
+            }
             if (0.5 * vn * vn < s.qm * dphi) {   // cannot climb the wall barrier: reflect
                 if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
                 s.x = x0; s.y = y0; s.q = q0; s.r = r0;
@@ -203,6 +285,7 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
     if (RECORD) {
         xp = s.x;
         yp = s.y;
+        double tf = 1.0;   // fraction of the step to the recorded point
         if (hit) {
             double frac;
             if (face_x) {
@@ -213,13 +296,19 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
                 frac = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
             }
             frac = fmin(fmax(frac, 0.0), 1.0);
+            tf = frac;
             xp = wrap_pos(x0 + frac * sx, g.lx);
             yp = y0 + frac * sy;
         }
         if (escaped) {
             double frac = fmin(fmax((g.ly - y0) / sy, 0.0), 1.0);
+            tf = frac;
             xp = wrap_pos(x0 + frac * sx, g.lx);
             yp = g.ly;
+        }
+        if constexpr (RF) {
+            rc.ft = rc.elapsed - (1.0 - tf) * dt;
+            rc.ck = py_mod(rc.clock - (1.0 - tf) * dt, g.period);
         }
     }
     if (hit) { cell = ix * ny + iy; return 1; }
@@ -327,13 +416,14 @@ __device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *l
     }
 }
 
-enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2 };
+enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4 };
 
 // Persistent refill loop shared by every push variant.
 template <int MODE, bool BARRIER, bool TAPERED>
 __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     using Real = typename std::conditional<MODE == kModeF32, float, double>::type;
-    constexpr bool RECORD = MODE == kModeF64Record;
+    constexpr bool RECORD = MODE == kModeF64Record || MODE == kModeF64RfRecord;
+    constexpr bool RF = MODE == kModeF64Rf || MODE == kModeF64RfRecord;
     extern __shared__ uint32_t lds_counts[];   // [2][m]
     __shared__ unsigned long long lds_scal[kScalCount];
     for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
@@ -349,6 +439,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     int sp = 0, k = 0, k0 = 0;    // step count, and its value when this pass took the particle
     int slot = -1, plen = 0;      // record mode: probe slot and path length
     double xp = 0.0, yp = 0.0;    // record mode: last recorded point
+    RfClock rc{0.0, 0.0, 0.0, 0.0};
     unsigned long long my_steps = 0;
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
@@ -368,11 +459,17 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         s.x = (Real)v.x; s.y = (Real)v.y; s.vx = (Real)v.vx; s.vy = (Real)v.vy;
                         k = v.k;
                         k0 = k;
+                        rc.clock = v.clock;
+                        rc.elapsed = v.elapsed;
                     } else {
                         pid = q;
                         s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
                         k = 0;
                         k0 = 0;
+                        if constexpr (RF) {
+                            rc.clock = py_mod(w.t0[q], g.period);
+                            rc.elapsed = 0.0;
+                        }
                     }
                     s.qm = (Real)w.qm[pid];
                     if constexpr (MODE == kModeF32) {
@@ -415,7 +512,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 if constexpr (MODE == kModeF32) {
                     st = step32<BARRIER, TAPERED>(g, s, cell);
                 } else {
-                    st = step64<BARRIER, TAPERED, RECORD>(g, s, cell, xp, yp);
+                    st = step64<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc);
                     if constexpr (RECORD) {
                         if (slot >= 0) {
                             double *path = w.rec.paths + 2 * ((int64_t)slot * w.rec.stride + plen);
@@ -434,6 +531,10 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         w.rec.vel[2 * pid] = s.vx;
                         w.rec.vel[2 * pid + 1] = s.vy;
                         if (slot >= 0) w.rec.path_len[slot] = plen;
+                        if constexpr (RF) {
+                            w.rec.flight[pid] = rc.ft;
+                            w.rec.clock[pid] = rc.ck;
+                        }
                     }
                     my_steps += (unsigned long long)(k - k0);
                     pid = -1;
@@ -443,6 +544,8 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                     unsigned slot_out = atomicAdd(w.out_count, 1u);
                     Survivor v;
                     v.x = (double)s.x; v.y = (double)s.y; v.vx = (double)s.vx; v.vy = (double)s.vy;
+                    v.clock = rc.clock;
+                    v.elapsed = rc.elapsed;
                     v.pid = (int32_t)pid;
                     v.k = k;
                     w.out[slot_out] = v;
@@ -473,10 +576,31 @@ __global__ void __launch_bounds__(kBlock) push64_record_kernel(Grid g, Work w) {
     push_body<kModeF64Record, BARRIER, TAPERED>(g, w);
 }
 
+// FP64 RF full-cycle push, without and with record mode.
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_rf_kernel(Grid g, Work w) {
+    push_body<kModeF64Rf, BARRIER, TAPERED>(g, w);
+}
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_rf_record_kernel(Grid g, Work w) {
+    push_body<kModeF64RfRecord, BARRIER, TAPERED>(g, w);
+}
+
 // FP32 push.
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(kBlock) push32_kernel(Grid g, Work w) {
     push_body<kModeF32, BARRIER, TAPERED>(g, w);
+}
+
+// Packs the RF basis (two unit-voltage responses) into per-cell records.
+__global__ void pack_rf_kernel(const double *__restrict__ ex, const double *__restrict__ ey,
+                               const double *__restrict__ phi, const uint8_t *__restrict__ vac, int n,
+                               double4 *__restrict__ rfcells, double2 *__restrict__ rfphi) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    bool v = vac[c] != 0;
+    rfcells[c] = make_double4(v ? ex[c] : 0.0, v ? ey[c] : 0.0, v ? ex[n + c] : 0.0, v ? ey[n + c] : 0.0);
+    rfphi[c] = make_double2(phi ? phi[c] : 0.0, phi ? phi[n + c] : 0.0);
 }
 
 // Packs the frozen field of one batch into per-cell records.

@@ -5,7 +5,10 @@ constructed once per geometry with the vacuum map, then called once per batch
 as ``tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p, barrier=None, record=None,
 stop=None, rf=None) -> (cnt_i, cnt_e, escaped, lost)`` (trench_cuda.py:41,
 trench_charging_2d.py:542, :702).  Argument checks and their exception types
-follow trench_cuda.py:42-59.
+follow trench_cuda.py:42-59.  ``rf=`` runs the RF full-cycle push
+(trench_charging_2d.py:556-602; trench_cuda.py:65-71, :134-137): the rf dict
+holds ``period``, ``dt_max``, ``wave`` (n_phase, 2), the basis ``Ex``, ``Ey``,
+``phi`` (2, nx, ny) and ``initial_time`` (n,).
 """
 import ctypes
 import threading
@@ -84,11 +87,35 @@ class HipTracer:
             raise ValueError("the HIP tracer was built for another geometry; create a new tracer")
         if not (0 < p.cfl <= 1) or p.max_steps < 1:
             raise ValueError("HIP tracing needs 0 < cfl <= 1 and max_steps >= 1")
-        if rf is not None:
-            raise NotImplementedError("RF particle clocks are not available in the HIP tracer yet")
-        return self._trace(x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status)
+        return self._trace(x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status, rf)
 
-    def _trace(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status):
+    def set_rf(self, rf, n=None):
+        """Upload the RF basis and wave (checked as trench_rf.validate,
+        trench_rf.py:21-32); returns the launch times as a float64 array."""
+        nx, ny = self.vac.shape
+        period, dt_max = float(rf["period"]), float(rf["dt_max"])
+        if not (np.isfinite(period) and period > 0 and np.isfinite(dt_max) and dt_max > 0):
+            raise ValueError("RF period and dt_max must be positive and finite")
+        basis = {}
+        for key in ("phi", "Ex", "Ey"):
+            a = np.ascontiguousarray(rf[key], dtype=np.float64)
+            if a.shape != (2, nx, ny) or not np.isfinite(a).all():
+                raise ValueError(f"RF basis {key} must be finite with shape (2, nx, ny)")
+            basis[key] = a
+        wave = np.ascontiguousarray(rf["wave"], dtype=np.float64)
+        if wave.ndim != 2 or wave.shape[1] != 2 or len(wave) < 2 or not np.isfinite(wave).all():
+            raise ValueError("RF wave must be a finite (n_phase >= 2, 2) array")
+        t0 = None
+        if n is not None:
+            t0 = np.ascontiguousarray(rf["initial_time"], dtype=np.float64)
+            if t0.shape != (n,) or not np.isfinite(t0).all():
+                raise ValueError("RF initial_time must be finite with one entry per particle")
+        _lib.check(_lib.load().th_set_rf(self._h, _lib.ptr(basis["Ex"], _lib.dp), _lib.ptr(basis["Ey"], _lib.dp),
+                                         _lib.ptr(basis["phi"], _lib.dp), _lib.ptr(wave, _lib.dp), len(wave),
+                                         period, dt_max))
+        return t0
+
+    def _trace(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, record, stop, keep_status, rf=None):
         L = _lib.load()
         n = len(x)
         nx, ny = self.vac.shape
@@ -110,7 +137,12 @@ class HipTracer:
         _lib.check(L.th_set_fields(self._h, _lib.ptr(ex, _lib.dp), _lib.ptr(ey, _lib.dp),
                                    _lib.ptr(phi, _lib.dp), _lib.ptr(wface, _lib.dp)))
         xs = [f64(v) for v in (x, y, vx, vy, qm)]
-        prm = self.params_for(p, barrier is not None)
+        prm = self.params_for(p, barrier is not None, rf is not None)
+        t0 = None
+        if rf is not None:
+            if self.precision != 0:
+                raise ValueError("the RF push runs in FP64 only")
+            t0 = self.set_rf(rf, n)
         cnt_i = np.empty((nx, ny), dtype=np.int64)
         cnt_e = np.empty((nx, ny), dtype=np.int64)
         tally = _lib.Tally()
@@ -131,6 +163,8 @@ class HipTracer:
             vel = np.empty((n, 2))
             paths = np.empty((max(selected.size, 1), stride, 2))
             lengths = np.ones(max(selected.size, 1), dtype=np.int32)
+            flight = np.empty(n) if rf is not None else None
+            clock = np.empty(n) if rf is not None else None
         flag = None
         watcher = None
         done = threading.Event()
@@ -148,14 +182,23 @@ class HipTracer:
             watcher = threading.Thread(target=watch, daemon=True)
             watcher.start()
         try:
-            args = (self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n, ctypes.byref(prm),
+            head = (self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p))
+            if rf is not None:
+                head = head + (_lib.ptr(t0, _lib.dp),)
+            args = (*head, n, ctypes.byref(prm),
                     ctypes.byref(flag) if flag is not None else None, _lib.ptr(cnt_i, _lib.i64p),
                     _lib.ptr(cnt_e, _lib.i64p), ctypes.byref(tally), _lib.ptr(status, _lib.i8p),
                     _lib.ptr(hit, _lib.i32p))
             if recording:
-                _lib.check(L.th_trace_record(*args, _lib.ptr(ends, _lib.dp), _lib.ptr(vel, _lib.dp),
-                                             _lib.ptr(slots, _lib.i32p), int(selected.size),
-                                             _lib.ptr(paths, _lib.dp), _lib.ptr(lengths, _lib.i32p)))
+                rec_args = (_lib.ptr(ends, _lib.dp), _lib.ptr(vel, _lib.dp), _lib.ptr(slots, _lib.i32p),
+                            int(selected.size), _lib.ptr(paths, _lib.dp), _lib.ptr(lengths, _lib.i32p))
+                if rf is not None:
+                    _lib.check(L.th_trace_rf_record(*args, *rec_args, _lib.ptr(flight, _lib.dp),
+                                                    _lib.ptr(clock, _lib.dp)))
+                else:
+                    _lib.check(L.th_trace_record(*args, *rec_args))
+            elif rf is not None:
+                _lib.check(L.th_trace_rf(*args))
             else:
                 _lib.check(L.th_trace(*args))
         finally:
@@ -175,15 +218,25 @@ class HipTracer:
                           endpoints=ends, final_velocity=vel,
                           paths=[paths[k, :lengths[k]].copy() for k in range(selected.size)],
                           cancelled=bool(tally.cancelled) or (stop is not None and stop.is_set()))
+            if rf is not None:
+                # trench_charging_2d.py:575-577, :680-682; trench_cuda.py:134-137
+                period = float(rf["period"])
+                record.update(launch_phase=t0 % period / period * 360, final_phase=clock / period * 360,
+                              flight_time=flight)
         return cnt_i, cnt_e, escaped, int(tally.lost)
 
     # ---- device-resident batches (no reference counterpart; for batch loops and benchmarks)
-    def params_for(self, p, barrier):
+    def params_for(self, p, barrier, rf=False):
         return _lib.TraceParams(dx=float(p.dx), cfl=float(p.cfl), max_steps=int(p.max_steps),
                                 tapered=int(tapered_of(p)), barrier=int(bool(barrier)),
-                                precision=int(self.precision), refill=int(self.refill), chunk=0,
+                                precision=0 if rf else int(self.precision), refill=int(self.refill), chunk=0,
                                 order=int(self.order), prio_steps=int(self.prio_steps),
-                                check_steps=int(self.check_steps), pass_steps=int(self.pass_steps))
+                                check_steps=int(self.check_steps), pass_steps=int(self.pass_steps), rf=int(bool(rf)))
+
+    def upload_times(self, offset, initial_time):
+        """Launch times of pool[offset, offset+n) for resident RF traces."""
+        t0 = np.ascontiguousarray(initial_time, dtype=np.float64)
+        _lib.check(_lib.load().th_particles_upload_times(self._h, int(offset), _lib.ptr(t0, _lib.dp), t0.size))
 
     def set_fields(self, Ex, Ey, barrier=None):
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
