@@ -17,8 +17,12 @@ batch), so path lengths are those of a charged trench.
 Prints one JSON line: the metric, the roofline of the push kernel (FP64
 FLOP at 69 per particle-step over its HIP-event time), and the CPU baseline
 (the oracle's C push on this host's cores over one whole batch, same states,
-same field).  ``--steady`` adds the wall time of plain time-marching to the
-steady-state criterion at this workload.
+same field).  The second half of the metric, time-to-steady at the same
+workload, is measured too (``--no-steady`` skips it): plain time-marching from
+zero charge to the reference's steady criterion, and the chosen method
+(trench01_amd.steady.anderson: a Krylov solve of the batch map's fixed point
+with common random numbers, then plain batches to the same criterion), with
+the bottom-centre potential of each.
 """
 import argparse
 import json
@@ -37,6 +41,12 @@ BYTES_PER_PARTICLE = 78     # algorithmic HBM bytes per particle (state in/out, 
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector (= FP64 matrix) peak
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector peak
 HBM_PEAK_GBS = 8000.0
+CPU_SAMPLE_S = 10.0         # minimum CPU work behind the cpu_baseline figure
+# HBM bytes of the push per batch at this workload, measured with rocprofv3
+# (FETCH_SIZE / WRITE_SIZE per dispatch, summed over the passes of one batch,
+# scaled by the copy-kernel calibration); the command that made it is stored
+# with it.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "push64_ar8_1e6_traffic.json")
 
 
 def parse():
@@ -48,9 +58,11 @@ def parse():
     ap.add_argument("--precharge", type=int, default=600, help="pre-charge batches (2e4 per species)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores this process may use")
-    ap.add_argument("--steady", action="store_true", help="also time plain time-marching to steady state")
+    ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady measurement")
     ap.add_argument("--max-batches", type=int, default=5000)
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
+    ap.add_argument("--pass-steps", type=int, default=None,
+                    help="multi-pass schedule's first cap (0: one pass); default: the library's tuned value")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -101,6 +113,8 @@ def main():
     setup = charging.ChargingSetup(p)
     dev = charging.DeviceCharging(setup)
     dev.set_schedule(precision=a.precision)
+    if a.pass_steps is not None:
+        dev.set_schedule(pass_steps=a.pass_steps)
     m = dev.m
     dev.set_rho(rho0 if rho0 is not None else np.zeros(m))
     n = 2 * p.n_per_batch
@@ -126,8 +140,9 @@ def main():
     value = a.steps * n / wall
     achieved = FLOP_PER_STEP * steps_total / (push_ms * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if a.precision == 0 else FP32_PEAK_TFLOPS
+    traffic, traffic_src = measured_traffic(a.precision)
     out = {
-        "metric": "traced particles/s/GPU (AR=8, 1e6 particles per batch)",
+        "metric": "traced particles/s/GPU + time-to-steady-charge at AR=8 (plain vs chosen)",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if a.precision == 0 else "f32",
@@ -140,53 +155,93 @@ def main():
                    "max_steps": p.max_steps, "mean_steps_per_particle": steps_total / (a.steps * n),
                    "push_ms_per_batch": push_ms / a.steps, "update_ms_per_batch": update_ms / a.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "push64_kernel" if a.precision == 0 else "push32_kernel",
-                     "kernel_ms_per_launch": push_ms / a.steps,
+                     "kernel_ms_per_batch": push_ms / a.steps,
+                     "launches_per_batch": len(pass_caps(dev.prm.pass_steps, p.max_steps)),
                      "flop_per_particle_step": FLOP_PER_STEP,
                      "note": "vector-ALU roof of the push's precision (FP64 78.6 TFLOP/s = FP64 MFMA peak; FP32 "
                              "157.3); the push is ALU/latency bound, not HBM: algorithmic HBM traffic is 78 B/particle; "
-                             "kernel time covers all passes of the multi-pass schedule",
+                             "one batch's push is launches_per_batch stream-ordered launches (multi-pass schedule), "
+                             "timed together with HIP events on the library stream; traffic is HBM bytes per batch",
                      "hbm_gbs_algorithmic": BYTES_PER_PARTICLE * n / (push_ms / a.steps * 1e-3) / 1e9},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
-    if a.steady:
+    if not a.no_steady:
         out["time_to_steady"] = time_to_steady(p, a.max_batches, log)
     print(json.dumps(out), flush=True)
 
 
+def pass_caps(pass_steps, max_steps):
+    """Step caps of the multi-pass schedule (one launch each)."""
+    if pass_steps <= 0 or pass_steps >= max_steps:
+        return [max_steps]
+    caps, cap = [], pass_steps
+    while cap < max_steps and len(caps) < 31:
+        caps.append(cap)
+        cap = min(2 * cap, max_steps)
+    return caps + [max_steps]
+
+
+def measured_traffic(precision):
+    """HBM bytes per batch of the FP64 push from the committed rocprofv3
+    counter summary (None for the FP32 push, which has no such summary)."""
+    if precision != 0 or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t["fetch_bytes_per_batch"] + t["write_bytes_per_batch"], os.path.relpath(TRAFFIC_FILE, ROOT)
+
+
 def cpu_baseline(p, setup, parts, field, phi, threads):
-    """The oracle's C push (OpenMP) over one whole batch of the timed workload:
-    same particle states and field as the GPU's first timed batch."""
+    """The oracle's C push (OpenMP) over whole batches of the timed workload
+    (same particle states and field as the GPU's first timed batch), repeated
+    until at least CPU_SAMPLE_S seconds of work."""
     from oracle import trace_ref
     geo = setup.geo
     cores = threads or len(os.sched_getaffinity(0))
     trace_ref.lib()
     ex, ey = field
     x, y, vx, vy, qm, spc = parts
+    reps, steps = 0, 0
     t0 = time.perf_counter()
-    r = trace_ref.trace_arrays(x, y, vx, vy, qm, spc, ex, ey, geo.vac, p.dx, p.cfl, p.max_steps,
-                               True, (phi.reshape(geo.solid.shape), setup.fields.wface), threads=cores)
+    while reps == 0 or time.perf_counter() - t0 < CPU_SAMPLE_S:
+        r = trace_ref.trace_arrays(x, y, vx, vy, qm, spc, ex, ey, geo.vac, p.dx, p.cfl, p.max_steps,
+                                   True, (phi.reshape(geo.solid.shape), setup.fields.wface), threads=cores)
+        reps += 1
+        steps += r["total_steps"]
     wall = time.perf_counter() - t0
-    return {"value": x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": f"one whole batch ({x.size} particles, {r['total_steps']} particle-steps) of the timed "
-                      f"workload, push only, in {wall:.2f} s"}
+    return {"value": reps * x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} pass(es) over one whole batch ({x.size} particles, {r['total_steps']} "
+                      f"particle-steps each) of the timed workload, push only, {wall:.1f} s"}
 
 
 def time_to_steady(p, max_batches, log):
-    """Plain time-marching from zero charge until the reference's steady
-    criterion passes (W = 30, hold = 2), device injection, 1e6 per batch."""
-    from trench01_amd import charging
+    """Time-to-steady from zero charge at this workload (device injection,
+    1e6 particles per batch): plain time-marching until the reference's
+    criterion passes (W = 30, hold = 2), and the chosen method (Anderson
+    solve of the batch map's fixed point, then plain batches to the same
+    criterion).  Wall times include each method's host set-up."""
+    from trench01_amd import charging, steady
     ps = replace(p, until_steady=True, max_batches=max_batches)
     t0 = time.perf_counter()
-    res = charging.run(ps, log=lambda s: log(s) if "steady" in s else None, injection="device")
+    res = charging.run(ps, log=None, injection="device")
     wall = time.perf_counter() - t0
     nb = res["timing"]["batches"]
-    log(f"time-to-steady: converged={res['converged']} after {nb} batches in {wall:.1f}s")
-    return {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb,
-            "wall_s": wall, "setup_s": res["timing"]["setup_s"],
-            "bottom_center_V": res["steady"]["bottom center"], "t_conv_s": res["t_conv"]}
+    plain = {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb, "wall_s": wall,
+             "setup_s": res["timing"]["setup_s"], "bottom_center_V": res["steady"]["bottom center"],
+             "t_conv_s": res["t_conv"]}
+    log(f"time-to-steady, plain: converged={res['converged']} after {nb} batches in {wall:.1f}s, "
+        f"bottom centre {plain['bottom_center_V']:.3f} V")
+    r = steady.anderson(ps)
+    chosen = {"method": r["method"], "converged": bool(r["converged"]), "batches": r["batches"],
+              "wall_s": r["wall_s"], "setup_s": r["setup_s"], "phases": r["phases"],
+              "bottom_center_V": r["steady"]["bottom center"]}
+    log(f"time-to-steady, chosen: converged={r['converged']} after {r['batches']} batches in {r['wall_s']:.1f}s, "
+        f"bottom centre {chosen['bottom_center_V']:.3f} V")
+    return {"plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
+            "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]}
 
 
 if __name__ == "__main__":

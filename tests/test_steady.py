@@ -1,0 +1,94 @@
+"""The steady-state solver's host parts on the CPU: the Anderson step is
+pinned by what Anderson acceleration must do on linear maps (Walker & Ni,
+SIAM J. Numer. Anal. 49(4), 2011: type II with full memory is GMRES on
+(I - A) x = b, so it reaches the fixed point of an n-dimensional affine map
+in at most n + 1 evaluations), and the potential coordinates by round trips
+through the Green's response of a real geometry."""
+import numpy as np
+import pytest
+
+from tests import cases
+from trench01_amd import charging, model, steady
+
+
+def affine(n, rho, seed):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = Q @ np.diag(rng.uniform(-rho, rho, n)) @ Q.T
+    b = rng.normal(size=n)
+    return A, b, np.linalg.solve(np.eye(n) - A, b)
+
+
+def test_full_memory_anderson_solves_affine_map_in_n_plus_one_steps():
+    """GMRES terminates in n steps on an n-dimensional system, so the (n+1)-th
+    evaluation of full-memory Anderson sits at the fixed point up to the
+    rounding of the normal equations."""
+    n = 12
+    A, b, xs = affine(n, 0.99, 1)
+    res = []
+    steady.anderson_iterate(lambda x: (None, A @ x + b), np.zeros(n), [], tol=0.0, memory=n + 1,
+                            max_evals=n + 1, log=lambda k, r: res.append(r))
+    assert len(res) == n + 1
+    assert res[n] < 1e-5 * res[0]
+    assert res[n] < 1e-2 * res[n - 1]          # the drop comes at the n+1-th evaluation
+
+
+def test_anderson_beats_plain_iteration_on_slow_map():
+    n = 40
+    A, b, xs = affine(n, 0.995, 2)                # plain iteration: ~ 5000 steps for 1e-10
+    _, gx, evals = steady.anderson_iterate(lambda x: (None, A @ x + b), np.zeros(n), [], tol=1e-9, memory=10,
+                                           max_evals=400)
+    np.testing.assert_allclose(gx, xs, atol=1e-6)
+    assert evals < 200
+
+
+def test_anderson_update_without_pairs_is_the_plain_step():
+    gx = np.arange(5.0)
+    out = steady.anderson_update([], gx - 1.0, gx)
+    np.testing.assert_array_equal(out, gx)
+    assert out is not gx
+
+
+def test_secant_pairs_carry_over_to_a_perturbed_map():
+    """A second map with the same Jacobian and a shifted fixed point (a new
+    particle sample) converges in fewer evaluations with the first map's
+    secant pairs than from an empty memory."""
+    n = 30
+    A, b, xs = affine(n, 0.99, 3)
+    b2 = b + 1e-3 * np.random.default_rng(4).normal(size=n)
+    pairs = []
+    _, x1, _ = steady.anderson_iterate(lambda x: (None, A @ x + b), np.zeros(n), pairs, tol=1e-9, memory=n)
+    warm = steady.anderson_iterate(lambda x: (None, A @ x + b2), x1, pairs, tol=1e-9, memory=n)[2]
+    cold = steady.anderson_iterate(lambda x: (None, A @ x + b2), x1, [], tol=1e-9, memory=n)[2]
+    assert warm < cold
+
+
+@pytest.mark.parametrize("name", ["taper", "ar5"])
+def test_potential_coordinates_round_trip(name):
+    rp = {"taper": cases.taper_params, "ar5": cases.ar5_params}[name]()
+    names = set(model.Params.__dataclass_fields__)
+    p = model.Params(**{k: getattr(rp, k) for k in names if hasattr(rp, k)})
+    s = charging.ChargingSetup(p)
+    to_u, to_rho = steady.potential_map(s)
+    rng = np.random.default_rng(0)
+    rho = rng.normal(0, 1e6, s.geo.surface.size) * (1 - s.cond_s)
+    if s.fields.floating:
+        rho[s.c_idx] = -4e8
+        rho[np.flatnonzero(s.cond_s)[-1]] += 1e8     # a conductor's charge may sit on several cells
+    u = to_u(rho)
+    # u is the surface potential of the charge: the Green's response times the column totals
+    q = np.zeros(s.green.shape[1])
+    np.add.at(q, s.col_of[s.col_of >= 0], rho[s.col_of >= 0] * p.dx ** 2)
+    P = s.surface_response
+    cols = [np.flatnonzero(s.col_of == c)[0] for c in range(q.size)]
+    np.testing.assert_allclose(u, P[cols] @ rho * p.dx ** 2, rtol=1e-10, atol=1e-10 * np.abs(u).max())
+    back = to_rho(u, rho)
+    np.testing.assert_allclose(back, rho, rtol=1e-9, atol=1e-9 * np.abs(rho).max())
+    u2 = u + rng.normal(0, 1.0, u.size)
+    np.testing.assert_allclose(to_u(to_rho(u2, rho)), u2, atol=1e-9 * np.abs(u2).max())
+    # the within-conductor distribution of the template is kept apart from the representative cell
+    if s.fields.floating:
+        moved = to_rho(u2, rho)
+        keep = s.cond_s.astype(bool)
+        keep[cols] = False
+        np.testing.assert_array_equal(moved[keep], rho[keep])

@@ -72,6 +72,7 @@ class ChargingSetup:
         P = np.zeros((m, m))
         has = col_of >= 0
         P[:, has] = G_full[surf][:, col_of[has]]
+        self.surface_response = P      # surface potentials per unit surface charge [V per C/m]
         self.leak = self._leakage(P) if p.sigma_s > 0 else None
         self.g_cond = G_full[:, cond_col].reshape(geo.solid.shape) if fields.floating else None
         if fields.floating:
