@@ -31,3 +31,29 @@ def test_binding_covers_header():
 
 def test_device_count_without_gpu_is_not_an_error():
     assert _lib.device_count() >= 0
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes struct of the binding has the size and field offsets the C
+    compiler gives the header's struct (a short Python struct would let the
+    library read past it)."""
+    import subprocess
+    structs = {"th_trace_params": _lib.TraceParams, "th_tally": _lib.Tally,
+               "th_charging_setup": _lib.ChargingSetupC, "th_batch_diag": _lib.BatchDiag,
+               "th_inject_params": _lib.InjectParams}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
