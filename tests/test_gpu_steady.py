@@ -1,11 +1,12 @@
 """The chosen steady-state method on the GPU against plain time-marching.
 
-Both start from zero charge at the AR 5 workload (conductive floating mask,
-100 eV ions with a 3 degree spread, 2e4 particles per species per batch) and
-stop at the reference's steady criterion (trench_charging_2d.py:893-947,
-:1299-1316).  The Krylov (Anderson) solve of the batch map's fixed point must
-land on the time-marched bottom-centre potential within 0.5 V and trace
-fewer batches."""
+Both start from zero charge on the AR 8 geometry (120x320, tapers, conductive
+floating mask, surface leak; arcsine 40-260 eV ions with a 4 degree spread)
+at 1.25e5 particles per species per batch, and stop at the reference's steady
+criterion (trench_charging_2d.py:893-947, :1299-1316).  The Krylov (Anderson)
+solve of the batch map's fixed point must land on the time-marched
+bottom-centre potential within 0.5 V and trace at most a third of the
+batches."""
 from dataclasses import replace
 
 import numpy as np
@@ -17,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_anderson_steady_state_matches_time_marching():
-    print("[gpu case] steady AR5 plain vs anderson", flush=True)
-    p, _ = configs.product_params("AR5", n_per_batch=20000)
+    print("[gpu case] steady AR8 (1.25e5 per species) plain vs anderson", flush=True)
+    p, _ = configs.product_params("AR8", n_per_batch=125_000)
     p = replace(p, until_steady=True, max_batches=4000)
     plain = charging.run(p, log=None, injection="device")
     chosen = steady.anderson(p)
@@ -27,6 +28,6 @@ def test_anderson_steady_state_matches_time_marching():
           f"bottom centre {chosen['steady']['bottom center']:.3f} V", flush=True)
     assert plain["converged"] and chosen["converged"]
     assert abs(chosen["steady"]["bottom center"] - plain["steady"]["bottom center"]) < 0.5
-    assert chosen["batches"] < plain["timing"]["batches"]
+    assert 3 * chosen["batches"] < plain["timing"]["batches"]
     for key, v in plain["steady"].items():
         assert np.isfinite(chosen["steady"][key]), key

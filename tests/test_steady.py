@@ -92,3 +92,19 @@ def test_potential_coordinates_round_trip(name):
         keep = s.cond_s.astype(bool)
         keep[cols] = False
         np.testing.assert_array_equal(moved[keep], rho[keep])
+
+
+def test_patience_stops_a_noisy_map():
+    """A map whose residual cannot fall below its noise stops after `patience`
+    evaluations without progress instead of running to max_evals."""
+    rng = np.random.default_rng(5)
+    n = 8
+    A, b, xs = affine(n, 0.5, 6)
+    calls = []
+
+    def g(x):
+        calls.append(1)
+        return None, A @ x + b + rng.normal(0, 1e-3, n)
+
+    steady.anderson_iterate(g, np.zeros(n), [], tol=1e-12, memory=5, max_evals=200, patience=6)
+    assert len(calls) < 60

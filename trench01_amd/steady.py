@@ -62,12 +62,14 @@ def anderson_update(pairs, f, gx, reg=1e-12):
     return gx - dG @ gamma
 
 
-def anderson_iterate(g, x, pairs, tol, memory=10, max_evals=40, log=None):
-    """Iterate x <- Anderson(g) until max |g(x) - x| < tol.  ``g(x)`` returns
-    (state, image); ``pairs`` (secant pairs, newest last) is updated in place
-    and may come from an earlier map with the same Jacobian.  Returns
+def anderson_iterate(g, x, pairs, tol, memory=10, max_evals=40, patience=None, log=None):
+    """Iterate x <- Anderson(g) until max |g(x) - x| < tol, ``max_evals``
+    evaluations, or ``patience`` evaluations without a new smallest residual
+    (the sampling noise of the map then dominates the residual).  ``g(x)``
+    returns (state, image); ``pairs`` (secant pairs, newest last) is updated in
+    place and may come from an earlier map with the same Jacobian.  Returns
     (state at the last evaluation, its image, evaluations)."""
-    last, evals, best = None, 0, np.inf
+    last, evals, best, since_best = None, 0, np.inf, 0
     while True:
         state, gx = g(x)
         evals += 1
@@ -75,7 +77,8 @@ def anderson_iterate(g, x, pairs, tol, memory=10, max_evals=40, log=None):
         res = float(np.max(np.abs(f)))
         if log:
             log(evals, res)
-        if res < tol or evals >= max_evals:
+        since_best = 0 if res < best else since_best + 1
+        if res < tol or evals >= max_evals or (patience is not None and since_best >= patience):
             return state, gx, evals
         if res > 4 * best and pairs:        # safeguard: drop the history, take a plain step
             pairs.clear()
@@ -126,11 +129,12 @@ def potential_map(setup):
     return to_u, to_rho
 
 
-def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, log=None, device=0, setup=None):
+def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None):
     """Steady state by multilevel Anderson acceleration of the batch map with
     common random numbers, then plain batches to the reference criterion.
     ``stages`` lists (K, R, tol): K batches per replica, R replicas averaged
-    per evaluation, stop when no surface potential moves by tol volts.
+    per evaluation, stop when no surface potential moves by tol volts (or
+    after ``patience`` evaluations without progress).
     Returns a dict: converged, batches (all traced), phases, steady (probe ->
     last-window mean), hist (probe histories of the plain phase), rho_s,
     wall_s and setup_s."""
@@ -174,7 +178,7 @@ def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, log=None, device
             return rho_g, gx
 
         rho_g, x, e = anderson_iterate(
-            g, x, pairs, tol, memory, max_evals,
+            g, x, pairs, tol, memory, max_evals, patience,
             log=lambda k, res, K=K, R=R: log(f"  [anderson K={K} R={R}] eval {k}: max |g(x) - x| = {res:.4g} V"))
         rho = to_rho(x, rho_g)
         evals_per_stage.append(e)
