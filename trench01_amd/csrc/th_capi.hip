@@ -52,6 +52,8 @@ struct th_tracer {
     int32_t *d_smap = nullptr;
     double4 *d_cells = nullptr;
     float4 *d_cells32 = nullptr;
+    double2 *d_corners = nullptr;      // interpolation corners per lattice key
+    uint8_t *d_corner_vac = nullptr;
     double *d_wface = nullptr, *d_tmp = nullptr;   // d_tmp: 3*N upload staging
     bool fields = false, barrier_fields = false;
     // RF full-cycle push: basis records, voltage wave, clock parameters
@@ -131,10 +133,21 @@ int reserve_particles(th_tracer *t, int64_t n) {
     return TH_OK;
 }
 
+// Rebuilds the interpolation corners after the cell records changed.
+int pack_corners(th_tracer *t) {
+    int keys = (t->nx + 1) * (t->ny + 1);
+    hipLaunchKernelGGL(th::pack_corners_kernel, dim3((keys + 255) / 256), dim3(256), 0, t->stream, t->d_cells,
+                       t->d_vac, t->nx, t->ny, t->d_corners, t->d_corner_vac);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
     th::Grid g;
     g.cells = t->d_cells;
     g.cells32 = t->d_cells32;
+    g.corners = t->d_corners;
+    g.corner_vac = t->d_corner_vac;
     g.wface = t->d_wface;
     g.vac = t->d_vac;
     g.smap = t->d_smap;
@@ -245,6 +258,8 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
         return bad(fail(TH_EHIP, "hipEventCreate failed"));
     if ((rc = dev_alloc(&t->d_vac, t->N)) || (rc = dev_alloc(&t->d_smap, t->N)) ||
         (rc = dev_alloc(&t->d_cells, t->N)) || (rc = dev_alloc(&t->d_cells32, t->N)) ||
+        (rc = dev_alloc(&t->d_corners, 4 * (size_t)(nx + 1) * (ny + 1))) ||
+        (rc = dev_alloc(&t->d_corner_vac, (size_t)(nx + 1) * (ny + 1))) ||
         (rc = dev_alloc(&t->d_wface, t->N)) || (rc = dev_alloc(&t->d_tmp, 3 * (size_t)t->N)) ||
         (rc = dev_alloc(&t->d_counts, 2 * (size_t)std::max(t->m, 1))) ||
         (rc = dev_alloc(&t->d_overflow, 2 * (size_t)t->N)) || (rc = dev_alloc(&t->d_scal, th::kScalCount)))
@@ -262,6 +277,7 @@ void th_tracer_destroy(th_tracer *t) {
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     free_particles(t);
     for (void *p : {(void *)t->d_vac, (void *)t->d_smap, (void *)t->d_cells, (void *)t->d_cells32,
+                    (void *)t->d_corners, (void *)t->d_corner_vac,
                     (void *)t->d_wface, (void *)t->d_tmp, (void *)t->d_counts, (void *)t->d_overflow,
                     (void *)t->d_scal, (void *)t->d_rfcells, (void *)t->d_rfphi, (void *)t->d_wave})
         if (p) (void)hipFree(p);
@@ -302,6 +318,8 @@ int th_set_fields(th_tracer *t, const double *Ex, const double *Ey, const double
                        t->d_tmp + t->N, phi ? t->d_tmp + 2 * (size_t)t->N : nullptr, t->d_vac, t->N,
                        t->d_cells, t->d_cells32);
     TH_HIP(hipGetLastError());
+    int rc = pack_corners(t);
+    if (rc) return rc;
     TH_HIP(hipStreamSynchronize(t->stream));
     t->fields = true;
     t->barrier_fields = phi != nullptr;
@@ -714,7 +732,7 @@ int launch_field(th_charging *c) {
     TH_HIP(hipGetLastError());
     hipLaunchKernelGGL(th::field_pack_kernel, dim3((t->N + 255) / 256), dim3(256), 0, t->stream, c->d);
     TH_HIP(hipGetLastError());
-    return TH_OK;
+    return pack_corners(t);
 }
 
 }  // namespace

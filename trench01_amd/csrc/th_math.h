@@ -34,7 +34,7 @@ TH_HD double th_hypot_core(double ax, double ay) {
 // Full hypot: ordering, widely-separated operands, and power-of-two rescaling
 // for very large (> 2^511) or very small (< 2^-459) operands, where the
 // correction terms would overflow or lose bits to underflow (exact scalings).
-TH_HD double th_hypot(double x, double y) {
+TH_HD double th_hypot_full(double x, double y) {
     if (!isfinite(x) || !isfinite(y)) {
         if (isinf(x) || isinf(y)) return INFINITY;
         return x + y;
@@ -58,3 +58,16 @@ TH_HD double th_hypot(double x, double y) {
     }
     return th_hypot_core(ax * down, ay * down) * up;
 }
+
+// The same function with the common case first: operands in the core's range
+// (no rescaling, not widely separated, finite, non-zero) go straight to the
+// core, which is what th_hypot_full computes for them (its scalings are then
+// by 1.0).  NaN fails every comparison and takes the full path.
+TH_HD double th_hypot(double x, double y) {
+    double fx = fabs(x), fy = fabs(y);
+    double ax = fx < fy ? fy : fx, ay = fx < fy ? fx : fy;
+    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-459 && ay > ax * 0x1p-54, 1))
+        return th_hypot_core(ax, ay);
+    return th_hypot_full(x, y);
+}
+

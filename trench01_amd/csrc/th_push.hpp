@@ -50,6 +50,11 @@ enum : int { kScalQueue = 0, kScalEsc0, kScalEsc1, kScalLost, kScalSteps, kScalO
 
 struct Grid {
     const double4 *__restrict__ cells;   // {Ex*vac, Ey*vac, vac, phi} per cell
+    // Interpolation corners per lattice key (i+1, j+1), i in [-1, nx-1],
+    // j in [-1, ny-1]: {Ex*vac, Ey*vac} of cells (i0,j0), (i1,j0), (i0,j1),
+    // (i1,j1) after the periodic wrap / clip, and their vacuum flags (bits 0-3).
+    const double2 *__restrict__ corners;  // [(nx+1)(ny+1)][4]
+    const uint8_t *__restrict__ corner_vac;   // [(nx+1)(ny+1)]
     const float4 *__restrict__ cells32;  // FP32 push: {Ex*vac, Ey*vac, vac, phi} in float
     const double *__restrict__ wface;    // barrier face weight per cell
     const uint8_t *__restrict__ vac;     // 1 vacuum, 0 solid
@@ -119,6 +124,18 @@ __device__ __forceinline__ int wrap_cell(int i, int n) {
 }
 __device__ __forceinline__ int clip_cell(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
+// wrap_cell(i, n) and wrap_cell(i + 1, n): branch-free for the lattice
+// indices of positions inside the period (i in [-1, n-1]).
+__device__ __forceinline__ void wrap_pair(int i, int n, int &i0, int &i1) {
+    if (__builtin_expect((unsigned)(i + 1) <= (unsigned)n, 1)) {
+        i0 = i < 0 ? n - 1 : i;
+        i1 = i + 1 == n ? 0 : i + 1;
+    } else {
+        i0 = wrap_cell(i, n);
+        i1 = wrap_cell(i + 1, n);
+    }
+}
+
 // NumPy's float %, i.e. r = fmod(a, L); r < 0 ? r + L : r, for L > 0.
 __device__ __forceinline__ double wrap_pos(double a, double L) {
     if (a >= 0.0 && a < L) return a;
@@ -185,16 +202,37 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
     double fi = floor(fx), fj = floor(fy);
     int i = (int)fi, j = (int)fj;
     double tx = fx - (double)i, ty = fy - (double)j;
-    int i0 = wrap_cell(i, nx), i1 = wrap_cell(i + 1, nx);
-    int j0 = clip_cell(j, ny), j1 = clip_cell(j + 1, ny);
-    double4 c00 = g.cells[i0 * ny + j0], c10 = g.cells[i1 * ny + j0];
-    double4 c01 = g.cells[i0 * ny + j1], c11 = g.cells[i1 * ny + j1];
     double w00 = (1.0 - tx) * (1.0 - ty), w10 = tx * (1.0 - ty), w01 = (1.0 - tx) * ty, w11 = tx * ty;
+    double2 e00, e10, e01, e11;
+    double v00, v10, v01, v11;              // vacuum weights 1.0 / 0.0
+    int i0, i1, j0, j1;
+    if (__builtin_expect((unsigned)(i + 1) <= (unsigned)nx && (unsigned)(j + 1) <= (unsigned)ny, 1)) {
+        // one 64-byte corner record and one flag byte per step
+        const int key = (i + 1) * (ny + 1) + (j + 1);
+        const double2 *c = g.corners + 4 * (int64_t)key;
+        e00 = c[0]; e10 = c[1]; e01 = c[2]; e11 = c[3];
+        const unsigned f = g.corner_vac[key];
+        v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
+        v01 = (f & 4u) ? 1.0 : 0.0; v11 = (f & 8u) ? 1.0 : 0.0;
+        i0 = i < 0 ? nx - 1 : i;
+        i1 = i + 1 == nx ? 0 : i + 1;
+        j0 = j < 0 ? 0 : j;
+        j1 = j + 1 >= ny ? ny - 1 : j + 1;
+    } else {                                // positions outside the period or grid
+        wrap_pair(i, nx, i0, i1);
+        j0 = clip_cell(j, ny);
+        j1 = clip_cell(j + 1, ny);
+        double4 c00 = g.cells[i0 * ny + j0], c10 = g.cells[i1 * ny + j0];
+        double4 c01 = g.cells[i0 * ny + j1], c11 = g.cells[i1 * ny + j1];
+        e00 = make_double2(c00.x, c00.y); e10 = make_double2(c10.x, c10.y);
+        e01 = make_double2(c01.x, c01.y); e11 = make_double2(c11.x, c11.y);
+        v00 = c00.z; v10 = c10.z; v01 = c01.z; v11 = c11.z;
+    }
     double ex = 0.0, ey = 0.0, sw = 0.0;
-    ex += w00 * c00.x; ey += w00 * c00.y; sw += w00 * c00.z;
-    ex += w10 * c10.x; ey += w10 * c10.y; sw += w10 * c10.z;
-    ex += w01 * c01.x; ey += w01 * c01.y; sw += w01 * c01.z;
-    ex += w11 * c11.x; ey += w11 * c11.y; sw += w11 * c11.z;
+    ex += w00 * e00.x; ey += w00 * e00.y; sw += w00 * v00;
+    ex += w10 * e10.x; ey += w10 * e10.y; sw += w10 * v10;
+    ex += w01 * e01.x; ey += w01 * e01.y; sw += w01 * v01;
+    ex += w11 * e11.x; ey += w11 * e11.y; sw += w11 * v11;
     sw = sw < 1e-12 ? 1e-12 : sw;
     ex = ex / sw;
     ey = ey / sw;
@@ -334,7 +372,8 @@ __device__ __forceinline__ int step32(const Grid &g, State32 &s, int &cell) {
     float fi = floorf(fx), fj = floorf(fy);
     int i = (int)fi, j = (int)fj;
     float tx = fx - fi, ty = fy - fj;
-    int i0 = wrap_cell(i, nx), i1 = wrap_cell(i + 1, nx);
+    int i0, i1;
+    wrap_pair(i, nx, i0, i1);
     int j0 = clip_cell(j, ny), j1 = clip_cell(j + 1, ny);
     float4 c00 = g.cells32[i0 * ny + j0], c10 = g.cells32[i1 * ny + j0];
     float4 c01 = g.cells32[i0 * ny + j1], c11 = g.cells32[i1 * ny + j1];
@@ -590,6 +629,24 @@ __global__ void __launch_bounds__(kBlock) push64_rf_record_kernel(Grid g, Work w
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(kBlock) push32_kernel(Grid g, Work w) {
     push_body<kModeF32, BARRIER, TAPERED>(g, w);
+}
+
+// Corner records of every lattice key (see Grid::corners) from the cell records.
+__global__ void pack_corners_kernel(const double4 *__restrict__ cells, const uint8_t *__restrict__ vac, int nx,
+                                    int ny, double2 *__restrict__ corners, uint8_t *__restrict__ corner_vac) {
+    int key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= (nx + 1) * (ny + 1)) return;
+    int i = key / (ny + 1) - 1, j = key % (ny + 1) - 1;
+    int i0 = i < 0 ? nx - 1 : i, i1 = i + 1 == nx ? 0 : i + 1;
+    int j0 = j < 0 ? 0 : j, j1 = j + 1 >= ny ? ny - 1 : j + 1;
+    int c[4] = {i0 * ny + j0, i1 * ny + j0, i0 * ny + j1, i1 * ny + j1};
+    unsigned f = 0;
+    for (int k = 0; k < 4; ++k) {
+        double4 r = cells[c[k]];
+        corners[4 * (int64_t)key + k] = make_double2(r.x, r.y);
+        f |= (vac[c[k]] ? 1u : 0u) << k;
+    }
+    corner_vac[key] = (uint8_t)f;
 }
 
 // Packs the RF basis (two unit-voltage responses) into per-cell records.
