@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady measurement")
     ap.add_argument("--max-batches", type=int, default=5000)
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
+    ap.add_argument("--green", type=int, default=0, choices=(0, 1, 2),
+                    help="potential update: 0 FP64 Green's matrix, 1 BF16 hi/lo split, 2 FP16 hi/lo split")
     ap.add_argument("--pass-steps", type=int, default=None,
                     help="multi-pass schedule's first cap (0: one pass); default: the library's tuned value")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -117,6 +119,7 @@ def main():
         dev.set_schedule(pass_steps=a.pass_steps)
     m = dev.m
     dev.set_rho(rho0 if rho0 is not None else np.zeros(m))
+    precision = precision_errors(dev, setup, a.green) if (a.green or a.precision) else None
     n = 2 * p.n_per_batch
     dev.reserve(n)
     for b in range(a.warmup):
@@ -166,11 +169,32 @@ def main():
                              "timed together with HIP events on the library stream; traffic is HBM bytes per batch",
                      "hbm_gbs_algorithmic": BYTES_PER_PARTICLE * n / (push_ms / a.steps * 1e-3) / 1e9},
     }
+    if precision is not None:
+        out["precision"] = precision
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
     if not a.no_steady:
-        out["time_to_steady"] = time_to_steady(p, a.max_batches, log)
+        out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green)
     print(json.dumps(out), flush=True)
+
+
+GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums"}
+
+
+def precision_errors(dev, setup, green):
+    """Potential and field of the pre-charged state with the FP64 Green's
+    matrix against the selected split (left selected)."""
+    dev.set_green_precision(0)
+    _, phi64, _ = dev.read(counts=False)
+    ex64, ey64 = dev.tracer.read_field()
+    dev.set_green_precision(green)
+    _, phi, _ = dev.read(counts=False)
+    ex, ey = dev.tracer.read_field()
+    rows = setup.rows
+    emax = max(np.abs(ex64).max(), np.abs(ey64).max())
+    return {"green": GREEN_NAMES[green], "potential_max_abs_err_V": float(np.abs(phi[rows] - phi64[rows]).max()),
+            "potential_max_V": float(np.abs(phi64[rows]).max()),
+            "field_max_rel_err": float(max(np.abs(ex - ex64).max(), np.abs(ey - ey64).max()) / emax)}
 
 
 def pass_caps(pass_steps, max_steps):
@@ -217,7 +241,7 @@ def cpu_baseline(p, setup, parts, field, phi, threads):
                       f"particle-steps each) of the timed workload, push only, {wall:.1f} s"}
 
 
-def time_to_steady(p, max_batches, log):
+def time_to_steady(p, max_batches, log, push_precision=0, green=0):
     """Time-to-steady from zero charge at this workload (device injection,
     1e6 particles per batch): plain time-marching until the reference's
     criterion passes (W = 30, hold = 2), and the chosen method (Anderson
@@ -240,8 +264,19 @@ def time_to_steady(p, max_batches, log):
               "bottom_center_V": r["steady"]["bottom center"]}
     log(f"time-to-steady, chosen: converged={r['converged']} after {r['batches']} batches in {r['wall_s']:.1f}s, "
         f"bottom centre {chosen['bottom_center_V']:.3f} V")
-    return {"plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
-            "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]}
+    out = {"plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
+           "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]}
+    if push_precision or green:
+        rv = steady.anderson(ps, push_precision=push_precision, green_precision=green)
+        out["chosen_precision_variant"] = {
+            "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green],
+            "converged": bool(rv["converged"]), "batches": rv["batches"], "wall_s": rv["wall_s"],
+            "bottom_center_V": rv["steady"]["bottom center"],
+            "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain["bottom_center_V"],
+            "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen["bottom_center_V"]}
+        log(f"time-to-steady, chosen with precision variants: {rv['wall_s']:.1f}s, bottom centre "
+            f"{rv['steady']['bottom center']:.3f} V")
+    return out
 
 
 if __name__ == "__main__":

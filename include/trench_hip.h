@@ -232,6 +232,14 @@ typedef struct th_batch_diag {
 int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **out);
 void th_charging_destroy(th_charging *c);
 
+/* Precision of the potential update (no reference counterpart; the
+ * reference solves with its FP64 LU, trench_charging_2d.py:1268): 0 FP64
+ * Green's matrix; 1 the matrix split into BF16 hi + lo halves; 2 split into
+ * FP16 hi + lo halves of the matrix scaled into FP16 range.  The split modes
+ * form products and sums in FP32 and add the external potential in FP64.
+ * Recomputes the potential and field of the current charge. */
+int th_charging_set_green(th_charging *c, int32_t mode);
+
 /* Sets the surface charge density rho (m values, C/m^3) and recomputes the
  * potential and the push's field records from it. */
 int th_charging_set_rho(th_charging *c, const double *rho_s);

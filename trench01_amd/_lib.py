@@ -94,6 +94,7 @@ SIGNATURES = {
     "th_charging_create": (ctypes.c_int, [vp, ctypes.POINTER(ChargingSetupC), ctypes.POINTER(vp)]),
     "th_charging_destroy": (None, [vp]),
     "th_charging_set_rho": (ctypes.c_int, [vp, dp]),
+    "th_charging_set_green": (ctypes.c_int, [vp, ctypes.c_int32]),
     "th_charging_step": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.POINTER(TraceParams), ctypes.POINTER(BatchDiag)]),
     "th_charging_read": (ctypes.c_int, [vp, dp, dp, i32p]),

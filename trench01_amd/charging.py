@@ -191,6 +191,13 @@ class DeviceCharging:
         except Exception:
             pass
 
+    def set_green_precision(self, mode):
+        """Potential update in FP64 (0) or with the Green's matrix split into
+        BF16 (1) or scaled FP16 (2) halves, FP32 products and sums
+        (th_charging_set_green)."""
+        _lib.check(_lib.load().th_charging_set_green(self._h, int(mode)))
+        self.green_precision = int(mode)
+
     def set_rho(self, rho_s):
         rho_s = np.ascontiguousarray(rho_s, dtype=np.float64)
         _lib.check(_lib.load().th_charging_set_rho(self._h, _lib.ptr(rho_s, _lib.dp)))

@@ -703,6 +703,10 @@ struct th_charging {
     double *rho = nullptr, *rho_prev = nullptr, *qcol = nullptr, *phi = nullptr, *diag = nullptr;
     hipEvent_t ev2 = nullptr;
     int update_lds = 0;
+    // split Green's matrix (precision variant), built on first use
+    uint16_t *g_hi = nullptr, *g_lo = nullptr;
+    int green_built = 0;
+    double green_absmax = 0.0;
     // monitors
     th::MonitorDev mon{};
     int32_t *mon_buf = nullptr;     // probe cells, region ptr, region idx, bottom (one allocation)
@@ -724,11 +728,24 @@ int upload_new(T **dst, const T *src, size_t count, hipStream_t s) {
 // Potential from the column charges, then the push's field records.
 int launch_field(th_charging *c) {
     th_tracer *t = c->t;
-    int waves = 4, rows_per_block = waves;
-    int blocks = (int)std::min<int64_t>((c->d.n_rows + rows_per_block - 1) / rows_per_block, (int64_t)t->cus * 8);
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(th::green_gemv_kernel, dim3(blocks), dim3(64 * waves), sizeof(double) * c->d.n_cols,
-                       t->stream, c->d);
+    if (c->d.green_mode == th::kGreenF64) {
+        int waves = 4, rows_per_block = waves;
+        int blocks = (int)std::min<int64_t>((c->d.n_rows + rows_per_block - 1) / rows_per_block, (int64_t)t->cus * 8);
+        if (blocks < 1) blocks = 1;
+        hipLaunchKernelGGL(th::green_gemv_kernel, dim3(blocks), dim3(64 * waves), sizeof(double) * c->d.n_cols,
+                           t->stream, c->d);
+    } else {
+        int rows_per_block = 256 / 16;
+        int blocks = (int)std::min<int64_t>((c->d.n_rows + rows_per_block - 1) / rows_per_block, (int64_t)t->cus * 8);
+        if (blocks < 1) blocks = 1;
+        size_t lds = sizeof(float) * (size_t)c->d.n_pad;
+        if (c->d.green_mode == th::kGreenBF16)
+            hipLaunchKernelGGL(th::green_gemv_split_kernel<th::kGreenBF16>, dim3(blocks), dim3(256), lds, t->stream,
+                               c->d);
+        else
+            hipLaunchKernelGGL(th::green_gemv_split_kernel<th::kGreenF16>, dim3(blocks), dim3(256), lds, t->stream,
+                               c->d);
+    }
     TH_HIP(hipGetLastError());
     hipLaunchKernelGGL(th::field_pack_kernel, dim3((t->N + 255) / 256), dim3(256), 0, t->stream, c->d);
     TH_HIP(hipGetLastError());
@@ -746,7 +763,7 @@ void th_charging_destroy(th_charging *c) {
     for (void *p : {(void *)c->col_of, (void *)c->rows, (void *)c->surf, (void *)c->green, (void *)c->external,
                     (void *)c->leak_M, (void *)c->leak_c, (void *)c->g_cond_s, (void *)c->region,
                     (void *)c->cond_s, (void *)c->rho, (void *)c->rho_prev, (void *)c->qcol, (void *)c->phi,
-                    (void *)c->diag})
+                    (void *)c->diag, (void *)c->g_hi, (void *)c->g_lo})
         if (p) (void)hipFree(p);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->mon_buf) (void)hipFree(c->mon_buf);
@@ -801,12 +818,57 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
     d.rho = c->rho; d.rho_prev = c->rho_prev; d.qcol = c->qcol; d.phi = c->phi; d.diag = c->diag;
     d.cells = t->d_cells; d.cells32 = t->d_cells32; d.counts = t->d_counts;
     d.q_macro = s->q_macro; d.dx = s->dx; d.C_cond = s->C_cond; d.Te = s->Te; d.dt_batch = s->dt_batch;
+    d.green_mode = th::kGreenF64;
+    d.n_pad = (s->n_cols + 7) / 8 * 8;
+    d.lo_scale = 1.0f;
+    d.inv_scale = 1.0;
+    for (size_t k = 0; k < (size_t)s->n_rows * s->n_cols; ++k)
+        c->green_absmax = std::max(c->green_absmax, std::fabs(s->green[k]));
     d.top_voltage = s->top_voltage; d.substrate_voltage = s->substrate_voltage;
     d.c_idx = s->c_idx; d.floating = s->floating; d.driven = s->driven; d.has_leak = s->leak_M != nullptr;
     d.top_dirichlet = s->top_dirichlet; d.leak_has_c = s->leak_c != nullptr;
     c->update_lds = (int)(2 * (size_t)std::max(m, 1) * sizeof(double));
     if (hipStreamSynchronize(st) != hipSuccess) return bad(fail(TH_EHIP, "set-up upload failed"));
     *out = c;
+    return TH_OK;
+}
+
+int th_charging_set_green(th_charging *c, int32_t mode) {
+    if (!c) return fail(TH_EINVAL, "invalid argument");
+    if (mode < th::kGreenF64 || mode > th::kGreenF16) return fail(TH_EINVAL, "Green's precision %d is not available", mode);
+    th_tracer *t = c->t;
+    TH_HIP(hipSetDevice(t->device));
+    th::ChargeDev &d = c->d;
+    double scale = 1.0;
+    float lo_scale = 1.0f;
+    if (mode == th::kGreenF16) {
+        // G * scale spans at most 2^14 (FP16 overflows at 65504); lo carries 11 more bits
+        int e = c->green_absmax > 0.0 ? (int)std::ceil(std::log2(c->green_absmax)) : 0;
+        scale = std::ldexp(1.0, 14 - e);
+        lo_scale = std::ldexp(1.0f, -11);
+    }
+    if (mode != th::kGreenF64 && c->green_built != mode) {
+        size_t count = (size_t)d.n_rows * d.n_pad;
+        int rc;
+        if (!c->g_hi && ((rc = dev_alloc(&c->g_hi, count)) || (rc = dev_alloc(&c->g_lo, count)))) return rc;
+        int blocks = (int)((count + 255) / 256);
+        if (mode == th::kGreenBF16)
+            hipLaunchKernelGGL(th::green_split_kernel<th::kGreenBF16>, dim3(blocks), dim3(256), 0, t->stream, c->green,
+                               d.n_rows, d.n_cols, d.n_pad, scale, lo_scale, c->g_hi, c->g_lo);
+        else
+            hipLaunchKernelGGL(th::green_split_kernel<th::kGreenF16>, dim3(blocks), dim3(256), 0, t->stream, c->green,
+                               d.n_rows, d.n_cols, d.n_pad, scale, lo_scale, c->g_hi, c->g_lo);
+        TH_HIP(hipGetLastError());
+        c->green_built = mode;
+    }
+    d.green_mode = mode;
+    d.g_hi = c->g_hi;
+    d.g_lo = c->g_lo;
+    d.lo_scale = lo_scale;
+    d.inv_scale = 1.0 / scale;      // a power of two
+    int rc = launch_field(c);       // potential and field of the current charge in the new precision
+    if (rc) return rc;
+    TH_HIP(hipStreamSynchronize(t->stream));
     return TH_OK;
 }
 

@@ -8,7 +8,9 @@
 //             rho[c_cell] += C (dV - drift) / dx^2                          (:1260-1266)
 //   potential phi = G Q + phi_ext on the kept rows, G the Green's matrix
 //             (one column per distinct surface response; the reference's
-//             solve(-rho/eps0) restricted to the rows the push reads)       (:1268, :795-797)
+//             solve(-rho/eps0) restricted to the rows the push reads)       (:1268, :795-797);
+//             FP64, or G split into two 16-bit halves (BF16 or scaled FP16)
+//             with FP32 products and accumulation (precision variant)
 //   field     E = -grad(phi) from vacuum cells only, packed with vac and phi
 //             into the push's cell records                                  (:429-454, :1269)
 //
@@ -44,6 +46,11 @@ struct ChargeDev {
     double4 *cells;             // push records {Ex*vac, Ey*vac, vac, phi}
     float4 *cells32;            // the same records in float for the FP32 push
     const uint32_t *counts;     // [2][m] deposit tallies of the batch
+    // split Green's matrix: G ~ (hi + lo * lo_scale) * inv_scale, rows padded to n_pad (multiple of 8)
+    const uint16_t *g_hi, *g_lo;
+    int n_pad, green_mode;      // 0 FP64, 1 BF16 hi/lo, 2 FP16 hi/lo
+    float lo_scale;
+    double inv_scale;
     double q_macro, dx, C_cond, Te, dt_batch, top_voltage, substrate_voltage;
     int c_idx, floating, driven, has_leak, top_dirichlet, leak_has_c;
 };
@@ -172,6 +179,82 @@ __global__ void __launch_bounds__(256) green_gemv_kernel(ChargeDev d) {
         if (lane == 0) {
             int cell = d.rows[r];
             d.phi[cell] = acc + d.external[cell];
+        }
+    }
+}
+
+enum : int { kGreenF64 = 0, kGreenBF16 = 1, kGreenF16 = 2 };
+
+__device__ __forceinline__ float bf16_to_float(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ uint32_t float_to_bf16(float f) {   // round to nearest even (finite f)
+    uint32_t u = __float_as_uint(f);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+template <int MODE>
+__device__ __forceinline__ float half_to_float(uint32_t h) {
+    if constexpr (MODE == kGreenBF16) {
+        return bf16_to_float(h);
+    } else {
+        _Float16 v;
+        uint16_t b = (uint16_t)h;
+        __builtin_memcpy(&v, &b, 2);
+        return (float)v;
+    }
+}
+
+// Splits G (scaled by `scale`) into hi + lo * lo_scale in 16-bit halves.
+template <int MODE>
+__global__ void green_split_kernel(const double *__restrict__ green, int n_rows, int n_cols, int n_pad, double scale,
+                                   float lo_scale, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n_rows * n_pad) return;
+    int r = (int)(idx / n_pad), c = (int)(idx - (int64_t)r * n_pad);
+    double g = c < n_cols ? green[(int64_t)r * n_cols + c] * scale : 0.0;
+    uint16_t h, l;
+    if constexpr (MODE == kGreenBF16) {
+        h = (uint16_t)float_to_bf16((float)g);
+        l = (uint16_t)float_to_bf16((float)((g - (double)bf16_to_float(h)) / (double)lo_scale));
+    } else {
+        _Float16 hv = (_Float16)(float)g;
+        _Float16 lv = (_Float16)(float)((g - (double)(float)hv) / (double)lo_scale);
+        __builtin_memcpy(&h, &hv, 2);
+        __builtin_memcpy(&l, &lv, 2);
+    }
+    hi[idx] = h;
+    lo[idx] = l;
+}
+
+// phi[rows[r]] = inv_scale * sum_c (hi + lo * lo_scale)[r][c] qcol[c] + external:
+// 16 lanes per row, 16-byte loads of eight halves, FP32 products and
+// accumulation, the row sum returned to FP64 for the external potential.
+template <int MODE>
+__global__ void __launch_bounds__(256) green_gemv_split_kernel(ChargeDev d) {
+    extern __shared__ float qf[];
+    for (int c = threadIdx.x; c < d.n_pad; c += blockDim.x) qf[c] = c < d.n_cols ? (float)d.qcol[c] : 0.0f;
+    __syncthreads();
+    const int sub = threadIdx.x & 15;
+    const int rows_per_block = blockDim.x >> 4;
+    const int vecs = d.n_pad >> 3;
+    for (int r = blockIdx.x * rows_per_block + (threadIdx.x >> 4); r < d.n_rows; r += gridDim.x * rows_per_block) {
+        const uint4 *h = reinterpret_cast<const uint4 *>(d.g_hi + (int64_t)r * d.n_pad);
+        const uint4 *l = reinterpret_cast<const uint4 *>(d.g_lo + (int64_t)r * d.n_pad);
+        float acc = 0.0f;
+        for (int v = sub; v < vecs; v += 16) {
+            uint4 hv = h[v], lv = l[v];
+            const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w}, lw[4] = {lv.x, lv.y, lv.z, lv.w};
+            const float *q = qf + 8 * v;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float g0 = half_to_float<MODE>(hw[k] & 0xffffu) + half_to_float<MODE>(lw[k] & 0xffffu) * d.lo_scale;
+                float g1 = half_to_float<MODE>(hw[k] >> 16) + half_to_float<MODE>(lw[k] >> 16) * d.lo_scale;
+                acc = fmaf(g0, q[2 * k], acc);
+                acc = fmaf(g1, q[2 * k + 1], acc);
+            }
+        }
+        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+        if (sub == 0) {
+            int cell = d.rows[r];
+            d.phi[cell] = (double)acc * d.inv_scale + d.external[cell];
         }
     }
 }

@@ -129,12 +129,15 @@ def potential_map(setup):
     return to_u, to_rho
 
 
-def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None):
+def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None,
+             push_precision=0, green_precision=0):
     """Steady state by multilevel Anderson acceleration of the batch map with
     common random numbers, then plain batches to the reference criterion.
     ``stages`` lists (K, R, tol): K batches per replica, R replicas averaged
     per evaluation, stop when no surface potential moves by tol volts (or
-    after ``patience`` evaluations without progress).
+    after ``patience`` evaluations without progress).  ``push_precision`` /
+    ``green_precision`` select the precision variants of the batch (FP32
+    push; BF16 / FP16 split Green's matrix).
     Returns a dict: converged, batches (all traced), phases, steady (probe ->
     last-window mean), hist (probe histories of the plain phase), rho_s,
     wall_s and setup_s."""
@@ -142,6 +145,9 @@ def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log
     t_start = time.perf_counter()
     s = setup or ChargingSetup(p)
     dev = DeviceCharging(s, device=device)
+    dev.set_schedule(precision=push_precision)
+    if green_precision:
+        dev.set_green_precision(green_precision)
     probes = s.probes
     dev.set_monitors(probes, s.regions)
     n = 2 * p.n_per_batch
