@@ -9,8 +9,11 @@ expressions keep the reference's evaluation order so that a whole run can be
 compared bit for bit with the reference's own regression fixtures
 (tests/data/base_*.npz, made by tests/test_regression.py:25-50).
 
-Scope: field_mode floating / fixed, bias dc (optionally with ``ion_energy`` /
-``ion_velocity`` samplers), no RF clocks, no resume.
+Scope: field_mode floating / fixed (bias dc, optionally with ``ion_energy`` /
+``ion_velocity`` samplers) and the circuit modes "circuit" (the circuit's
+voltages frozen at one phase) and "circuit_rf" (the push follows one RF period
+of voltages, launch phases drawn per batch); side boundaries periodic, fixed
+and pillar; no resume.
 """
 from dataclasses import dataclass, replace
 
@@ -60,6 +63,10 @@ class Params:
     ion_temp_eV: float = 0.5
     electron_temp_eV: float = 3.0
     bias: str = "dc"
+    rf_phase: float = 0.0
+    rf_time_steps: int = 256
+    rf_wave: str = "sine"
+    rf_rise: float = 1e-9
     sigma_s: float = 1e-15
     until_steady: bool = False
     max_batches: int = 3000
@@ -189,27 +196,84 @@ def make_solver(A, cond):
     return lambda b: E @ lu.solve(E.T @ b)
 
 
-def boundary_rhs(p, eps, volts):
-    """trench_charging_2d.py:323-337 (periodic / fixed sides; pillar not restated)."""
+def boundary_rhs(p, eps, volts, fixed_sides=True):
+    """trench_charging_2d.py:323-337: Dirichlet faces of the substrate, the top
+    and (side_bc fixed / pillar) the dielectric's side faces; pillar sides
+    ramp linearly from the substrate to the mask voltage over the dielectric's
+    height.  ``fixed_sides`` False: fixed sides at 0 V (the RF basis)."""
     rhs = np.zeros_like(eps)
     rhs[:, 0] -= 2.0 * eps[:, 0] / p.dx ** 2 * volts["substrate"]
     if p.top_bc == "dirichlet":
         rhs[:, -1] -= 2.0 * eps[:, -1] / p.dx ** 2 * volts["top"]
     top_ox = p.floor_t + p.trench_d
-    if p.side_bc == "fixed":
-        rhs[0, :top_ox] -= 2.0 * eps[0, :top_ox] / p.dx ** 2 * p.left_voltage
-        rhs[-1, :top_ox] -= 2.0 * eps[-1, :top_ox] / p.dx ** 2 * p.right_voltage
-    elif p.side_bc != "periodic":
-        raise NotImplementedError("pillar side boundary is not restated")
+    if p.side_bc == "pillar":
+        frac = (np.arange(top_ox) + 0.5) / top_ox
+        left = right = volts["substrate"] + (volts["mask"] - volts["substrate"]) * frac
+    elif fixed_sides:
+        left, right = p.left_voltage, p.right_voltage
+    else:
+        left = right = 0.0
+    if p.side_bc != "periodic":
+        rhs[0, :top_ox] -= 2.0 * eps[0, :top_ox] / p.dx ** 2 * left
+        rhs[-1, :top_ox] -= 2.0 * eps[-1, :top_ox] / p.dx ** 2 * right
     return rhs
 
 
-def setup_fields(p, solid, mask):
-    """trench_charging_2d.py:340-395 for field_mode floating / fixed."""
-    if p.field_mode not in ("floating", "fixed"):
-        raise NotImplementedError("circuit field modes are not restated")
-    volts = dict(substrate=p.substrate_voltage, mask=p.mask_voltage, top=p.top_voltage)
+def circuit_voltages(c, phase):
+    """trench_charging_2d.py:312-320: the circuit's electrode voltages at
+    ``phase`` [deg] by periodic linear interpolation, relative to the plasma
+    potential (top at 0 V)."""
+    t = np.asarray(c["t"])
+    period = (t[1] - t[0]) * t.size
+    at = (phase % 360.0) / 360.0 * period
+    v = {k: float(np.interp(at, np.append(t, period), np.append(c[k], c[k][0]))) for k in ("uw", "um", "up", "V1")}
+    return dict(substrate=v["uw"] - v["up"], mask=v["um"] - v["up"], top=0.0, sheath=v["V1"])
+
+
+def launch_times(rng, n, period):
+    """trench_rf.py:14-18: one time per n-th of the period, uniform inside its
+    stratum, in shuffled order."""
+    t = (np.arange(n) + rng.random(n)) * period / n
+    rng.shuffle(t)
+    return t
+
+
+def rf_coefficients(rf, at):
+    """trench_rf.py:35-44: the wave's deviation from its mean at times ``at``,
+    linear between the n_phase nodes of one period (periodic)."""
+    wave = np.asarray(rf["wave"])
+    n = len(wave)
+    u = np.mod(np.asarray(at, dtype=float), rf["period"]) / rf["period"] * n
+    k = np.floor(u).astype(np.int64) % n
+    f = (u - np.floor(u))[..., None]
+    return wave[k] * (1 - f) + wave[(k + 1) % n] * f
+
+
+def setup_fields(p, solid, mask, circuit=None):
+    """trench_charging_2d.py:340-413: charge-free potential of the electrode
+    voltages and the charge response (floating / fixed / circuit /
+    circuit_rf).  For circuit_rf the external potential is the period mean and
+    ``rf`` holds the unit-voltage responses of substrate and mask (potential,
+    field, boundary terms) with the wave's deviation from its mean."""
     driven = p.field_mode != "floating"
+    if p.field_mode in ("circuit", "circuit_rf"):
+        if p.bias != "rf" or circuit is None or p.top_bc != "dirichlet":
+            raise ValueError("circuit field modes need bias rf, a circuit waveform and a Dirichlet top")
+        volts = circuit_voltages(circuit, p.rf_phase)
+    elif p.field_mode in ("floating", "fixed"):
+        volts = dict(substrate=p.substrate_voltage, mask=p.mask_voltage, top=p.top_voltage)
+    else:
+        raise ValueError(f"unknown field_mode {p.field_mode!r}")
+    if p.side_bc == "pillar" and not driven:
+        raise ValueError("pillar sides need driven electrodes")
+    if p.field_mode == "circuit_rf":
+        t = np.asarray(circuit["t"])
+        period = (t[1] - t[0]) * t.size
+        if p.rf_time_steps < 16 or p.dt_batch < period * (1 - 1e-12):
+            raise ValueError("circuit_rf needs rf_time_steps >= 16 and dt_batch of at least one period")
+        wave = np.column_stack((circuit["uw"] - circuit["up"], circuit["um"] - circuit["up"]))
+        mean = wave.mean(axis=0)
+        volts = dict(substrate=float(mean[0]), mask=float(mean[1]), top=0.0, sheath=float(np.mean(circuit["V1"])))
     cond = mask if p.mask_type == "conductor" else np.zeros_like(mask)
     eps = permittivity(solid, mask, p)
     A = build_poisson(eps, cond, p)
@@ -230,8 +294,39 @@ def setup_fields(p, solid, mask):
         mask_rhs = np.zeros(cond.size)
     rhs = boundary_rhs(p, eps, volts)
     external = (solve(rhs.ravel() + mask_rhs) + phi_cond).reshape(eps.shape)
-    return dict(A=A, solve=solve, cond=cond, eps=eps, external=external, boundary_rhs=rhs,
-                volts=volts, floating=bool(cond.any() and not driven))
+    fields = dict(A=A, solve=solve, cond=cond, eps=eps, external=external, boundary_rhs=rhs,
+                  volts=volts, floating=bool(cond.any() and not driven), rf=None)
+    if p.field_mode == "circuit_rf":
+        # unit responses: substrate at 1 V, then mask at 1 V (:397-406); the
+        # field of each with a 0 V top ghost (:407)
+        phis, rhss = [], []
+        for vs, vm in ((1.0, 0.0), (0.0, 1.0)):
+            b = boundary_rhs(p, eps, dict(substrate=vs, mask=vm, top=0.0), fixed_sides=False)
+            on_mask = cond.ravel().astype(float) * vm
+            phis.append((solve(b.ravel() - A @ on_mask) + on_mask).reshape(eps.shape))
+            rhss.append(b)
+        e = [compute_field(v, ~solid, p, 0.0) for v in phis]
+        dt_max = period / p.rf_time_steps
+        if p.rf_wave == "pulse":
+            dt_max = min(dt_max, p.rf_rise / 8)
+        fields["rf"] = dict(period=period, wave=wave - mean, phi=np.array(phis), Ex=np.array([a for a, _ in e]),
+                            Ey=np.array([b for _, b in e]), dt_max=dt_max, boundary_rhs=np.array(rhss))
+    return fields
+
+
+def field_snapshot(fields, mean_phi, p):
+    """trench_charging_2d.py:416-426: potential, external potential, boundary
+    terms and voltages shown at p.rf_phase (circuit_rf); the period means
+    otherwise."""
+    rf = fields["rf"]
+    if rf is None:
+        return mean_phi, fields["external"], fields["boundary_rhs"], fields["volts"]
+    d = rf_coefficients(rf, (p.rf_phase % 360) / 360 * rf["period"])
+    shift = d[0] * rf["phi"][0] + d[1] * rf["phi"][1]
+    rhs = fields["boundary_rhs"] + (d[0] * rf["boundary_rhs"][0] + d[1] * rf["boundary_rhs"][1])
+    volts = {**fields["volts"], "substrate": fields["volts"]["substrate"] + float(d[0]),
+             "mask": fields["volts"]["mask"] + float(d[1])}
+    return mean_phi + shift, fields["external"] + shift, rhs, volts
 
 
 def _neg_grad(c, m, pl, vm, vp, dx):
@@ -455,10 +550,10 @@ def probes_for(solid, p, cond):
 
 
 # ------------------------------------------------------------------ batch loop
-def batch_context(p):
+def batch_context(p, circuit=None):
     """Set-up of the per-batch update for a geometry (trench_charging_2d.py:1063-1094)."""
     solid, mask, i0, i1 = build_geometry(p)
-    fields = setup_fields(p, solid, mask)
+    fields = setup_fields(p, solid, mask, circuit)
     cond, solve = fields["cond"], fields["solve"]
     ctx = dict(p=p, vac=~solid, fields=fields, cond=cond, conductor=fields["floating"],
                driven=p.field_mode != "floating", solve=solve,
@@ -507,19 +602,31 @@ def batch_update(ctx, rho, cnt_i, cnt_e):
     return rho, phi, Ex, Ey, leak_ratio
 
 
-def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_batch=None):
+def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_batch=None, circuit=None,
+        blas_tail=False):
     """trench_charging_2d.py:1045-1369 (setup :1063-1223, loop :1225-1335,
-    result :1337-1369), restated for dc bias, floating / fixed fields, no resume.
+    result :1337-1369), restated without resume.
 
     ``tracer`` defaults to the C push.  ``on_batch(b, state)`` is called after
-    every batch with the loop's arrays (for per-batch comparisons)."""
-    tracer = tracer or trace_ref.oracle_tracer
+    every batch with the loop's arrays (for per-batch comparisons).
+    ``circuit``: the circuit waveform (t, uw, um, up, V1) of the circuit field
+    modes.  ``blas_tail``: the C push rounds the RF field bound as NumPy's
+    single-threaded matrix product does (for comparisons with the reference)."""
+    if tracer is None:
+        tracer = (lambda *a, **k: trace_ref.oracle_tracer(*a, **k, blas_tail=blas_tail)) if blas_tail \
+            else trace_ref.oracle_tracer
     rng = np.random.default_rng(p.seed)
     solid, mask, i0, i1 = build_geometry(p)
     vac = ~solid
     nx, ny = solid.shape
     dx = p.dx
-    fields = setup_fields(p, solid, mask)
+    fields = setup_fields(p, solid, mask, circuit)
+    # stratified launch phases on a stream of their own (:1079)
+    phase_rng = np.random.default_rng(np.random.SeedSequence([p.seed, 0x5246])) if fields["rf"] else None
+    if p.field_mode in ("circuit", "circuit_rf"):
+        # ions enter at the Bohm energy Te/2 and are accelerated in the domain (:1131-1136)
+        ion_velocity = None
+        ion_energy = lambda rng_, n_: np.full(n_, p.electron_temp_eV / 2)
     cond, eps, solve = fields["cond"], fields["eps"], fields["solve"]
     conductor = fields["floating"]
     driven = p.field_mode != "floating"
@@ -537,6 +644,8 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
     if conductor:
         ctx.update(g_cond=g_cond, C_cond=C_cond, c_cell=c_cell)
     probes = probes_for(solid, p, cond)
+    reference = ({k: float(fields["external"][ix, iy]) for k, (ix, iy) in probes.items()}
+                 if fields["rf"] is not None else None)                                   # :1167-1168
     hist = {k: [] for k in ("t", "ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "leak", *probes)}
     charge_hist = {"left": [], "right": []}
     q_regions = charge_monitors(solid, mask, i0, i1, p)
@@ -549,8 +658,12 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
     n_ok, converged_at = 0, None
     for b in range(n_max):
         x, y, vx, vy, qm, spc = inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity)
+        options = {}
+        if fields["rf"] is not None:                                                      # :1231-1234
+            options["rf"] = {**fields["rf"], "initial_time": np.concatenate(
+                [launch_times(phase_rng, p.n_per_batch, fields["rf"]["period"]) for _ in range(2)])}
         cnt_i, cnt_e, esc, n_lost = tracer(x, y, vx, vy, qm, spc, Ex, Ey, vac, p,
-                                           (phi, wface) if p.wall_model == "barrier" else None)
+                                           (phi, wface) if p.wall_model == "barrier" else None, **options)
         rho, phi, Ex, Ey, leak_ratio = batch_update(ctx, rho, cnt_i, cnt_e)
         hist["t"].append((b + 1) * p.dt_batch)
         for k, (ix, iy) in probes.items():
@@ -571,7 +684,7 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
                              Ex=Ex, Ey=Ey))
         W = p.steady_window
         if p.until_steady and (b + 1) % W == 0 and b + 1 >= 2 * W:
-            v_ok = check_steady(hist, probes, p)[0]
+            v_ok = check_steady(hist, probes, p, reference)[0]
             ok = bool(v_ok and check_charge_steady(charge_monitor_hist, p)["ok"])
             n_ok = min(n_ok + 1, p.steady_hold) if ok else 0
             if n_ok >= p.steady_hold:
@@ -580,11 +693,15 @@ def run(p, tracer=None, ion_energy=None, ion_velocity=None, batches=None, on_bat
     hist = {k: np.array(v) for k, v in hist.items()}
     Wl = min(p.steady_window, len(hist["t"]))
     steady = {k: float(hist[k][-Wl:].mean()) for k in probes}
+    shown_phi, shown_ext, shown_rhs, volts = field_snapshot(fields, phi, p)          # :1351-1354
+    if fields["rf"] is not None:
+        Ex, Ey = compute_field(shown_phi, vac, p, volts["top"])
     rho_shown = rho.copy()
     if cond.any():
-        induced = (fields["A"] @ phi.ravel()).reshape(rho.shape) - fields["boundary_rhs"]
+        induced = (fields["A"] @ shown_phi.ravel()).reshape(rho.shape) - shown_rhs
         rho_shown[cond] = (-EPS0 * induced)[cond]
-    return dict(phi=phi, rho=rho_shown, rho_raw=rho, Ex=Ex, Ey=Ey, solid=solid, mask=mask, i0=i0, i1=i1,
+    return dict(phi=shown_phi, mean_phi=phi, external_phi=shown_ext, voltages=volts, rho=rho_shown, rho_raw=rho,
+                Ex=Ex, Ey=Ey, solid=solid, mask=mask, i0=i0, i1=i1,
                 hist=hist, probes=probes, steady=steady, charge_hist=charge_hist,
                 charge_monitor_hist={k: np.array(v) for k, v in charge_monitor_hist.items()},
                 t_conv=converged_at * p.dt_batch if converged_at else np.nan)

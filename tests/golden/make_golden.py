@@ -20,6 +20,13 @@ vector pins a reference function that the reference's own fixtures
   charged sidewalls; record-mode outputs incl. flight time and phases.  Run
   with single-threaded BLAS (the bound's matrix product rounds per position;
   see oracle/trace_ref.c).
+* ``runrf_*.npz``   whole run()s in the circuit field modes, with the circuit
+  solver (plasma_circuit.solve_circuit_cached) replaced by the sinusoidal
+  waveform above as the reference's tests/test_rf_cycle.py:127-133 does:
+  ``circuit_rf`` (full-cycle push, pillar sides, surface leak) and
+  ``circuit`` (voltages frozen at rf_phase 120, fixed sides); few batches,
+  single-threaded BLAS; ``result_circuit_rf.npz``: the reference's result file
+  of the circuit_rf run (save_results), for the circuit waveform on disk.
 * ``run_ar5.npz``   a whole run() (:1045-1369) of the plain AR 5 workload
   (80x200, conductive mask, sigma_s 0, barrier wall, 100 eV ions with 3 degree
   polar spread, Te 4 eV), 50 batches of 1e4 particles per species, seed 0;
@@ -127,6 +134,15 @@ def rf_circuit(period=4e-10, n=256, amplitude=45.0):
     return dict(t=t, uw=uw, um=um, up=up, V1=up - um)
 
 
+RUN_RF_CASES = {
+    "circuit_rf": dict(nx=20, trench_w=6, trench_d=12, floor_t=4, mask_t=4, n_vac=8, field_mode="circuit_rf",
+                       side_bc="pillar", sigma_s=1e-14, n_batches=12, n_per_batch=1000, rf_phase=75.0),
+    "circuit": dict(nx=40, trench_w=16, trench_d=20, floor_t=4, mask_t=12, n_vac=10, field_mode="circuit",
+                    trench_taper_deg=float(np.degrees(np.arctan(.25))), side_bc="fixed", left_voltage=-15.0,
+                    right_voltage=4.0, sigma_s=1e-14, n_batches=8, n_per_batch=1000, rf_phase=120.0),
+}
+
+
 def params_json(p):
     """The run's Params as JSON (stored with every vector)."""
     return np.array(json.dumps(asdict(p)))
@@ -197,6 +213,39 @@ def write_rf_case(T, name):
                         path_lengths=np.array([len(v) for v in record["paths"]]),
                         paths=np.concatenate(record["paths"]) if record["paths"] else np.zeros((0, 2)))
     print(f"rftrace_{name}: n={x.size} absorbed={int((record['status'] == 1).sum())} escaped={esc} lost={lost}")
+
+
+def write_run_rf(T, name):
+    from threadpoolctl import threadpool_limits
+    p = replace(T.Params(), bias="rf", rf_freq=2.5e9, mask_type="conductor", wall_model="barrier",
+                max_steps=1000, probe_trajectories=8, seed=3, **RUN_RF_CASES[name])
+    circuit = dict(rf_circuit(), us=None, i=None, E=np.zeros(0), cached=False, ied_computed=False,
+                   sheath=None, ied_model="sheath", n_periods=1, elements={})
+    circuit["us"] = circuit["uw"].copy()
+    circuit["i"] = circuit["t"] * 0
+    original = T.PC.solve_circuit_cached
+    T.PC.solve_circuit_cached = lambda *a, **k: circuit
+    try:
+        with threadpool_limits(1):
+            res = T.run(p, log=lambda s: None)
+    finally:
+        T.PC.solve_circuit_cached = original
+    arrays = dict(params=params_json(p), rho_raw=res["rho_raw"], rho=res["rho"], phi=res["phi"],
+                  mean_phi=res["mean_phi"], external_phi=res["external_phi"], Ex=res["Ex"], Ey=res["Ey"],
+                  solid=res["solid"], voltages=np.array(json.dumps(res["voltages"])),
+                  **{f"circuit_{k}": circuit[k] for k in ("t", "uw", "um", "up", "V1")})
+    arrays.update({f"hist_{k}": v for k, v in res["hist"].items()})
+    arrays.update({f"charge_{k}": v for k, v in res["charge_monitor_hist"].items()})
+    np.savez_compressed(HERE / f"runrf_{name}.npz", **arrays)
+    print(f"runrf_{name}: bottom centre {res['hist']['bottom center'][-1]:.3f} V, "
+          f"lost {res['hist']['lost'][-1]:.4f}")
+    if name == "circuit_rf":
+        # the reference's result file of this run (NPZ only): circuit waveform and phase-stream checkpoint
+        import shutil
+        import tempfile
+        with tempfile.TemporaryDirectory() as tmp:
+            T.save_results(res, p, str(Path(tmp) / "out"))
+            shutil.copy(Path(tmp) / "out.npz", HERE / "result_circuit_rf.npz")
 
 
 def write_setup_case(T, name):
@@ -276,13 +325,15 @@ def main():
     ap.add_argument("--only", default=None)
     a = ap.parse_args()
     T = load_reference(a.reference)
-    jobs = [*(f"trace_{k}" for k in TRACE_CASES), *(f"rftrace_{k}" for k in RF_CASES), "setup_small",
-            "setup_taper", "run_ar5"]
+    jobs = [*(f"trace_{k}" for k in TRACE_CASES), *(f"rftrace_{k}" for k in RF_CASES),
+            *(f"runrf_{k}" for k in RUN_RF_CASES), "setup_small", "setup_taper", "run_ar5"]
     for job in jobs:
         if a.only and a.only != job:
             continue
         if job.startswith("rftrace_"):
             write_rf_case(T, job[8:])
+        elif job.startswith("runrf_"):
+            write_run_rf(T, job[6:])
         elif job.startswith("trace_"):
             write_trace_case(T, job[6:])
         elif job.startswith("setup_"):

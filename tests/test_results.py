@@ -36,3 +36,30 @@ def test_save_reproduces_the_reference_keys(tmp_path):
     from dataclasses import replace
     assert q.backend == "auto"          # the reference's Params has no "hip" backend
     assert replace(q, backend=p.backend) == p
+
+
+def test_reference_circuit_result_round_trip(tmp_path):
+    """The reference's result file of a circuit_rf run: the circuit waveform
+    (rf_* arrays + circuit_meta_json) and the launch-phase stream's state in
+    the checkpoint are read, and written back with the same keys and values."""
+    ref_path = GOLDEN / "result_circuit_rf.npz"
+    res, p = results.load_results(ref_path)
+    assert p.field_mode == "circuit_rf" and p.side_bc == "pillar"
+    c = res["circuit"]
+    assert c["ied_computed"] is False and c["sheath"] is None
+    with np.load(GOLDEN / "runrf_circuit_rf.npz", allow_pickle=False) as g:
+        for k in ("t", "uw", "um", "up", "V1"):
+            np.testing.assert_array_equal(c[k], g[f"circuit_{k}"])
+        np.testing.assert_array_equal(res["mean_phi"], g["mean_phi"])
+        np.testing.assert_array_equal(res["phi"], g["phi"])
+    assert res["checkpoint"]["phase_rng_state"]["bit_generator"] == "PCG64"
+    path = results.save_results(res, str(tmp_path / "again"))
+    with np.load(path, allow_pickle=False) as a, np.load(ref_path, allow_pickle=False) as b:
+        assert set(a.files) - {"params_extra_json"} == set(b.files)
+        for k in b.files:
+            if k in ("params_json", "backend_used", "checkpoint_json", "circuit_meta_json"):
+                continue
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    again, _ = results.load_results(path)
+    assert again["circuit"]["n_periods"] == c["n_periods"]
+    assert again["checkpoint"]["phase_rng_state"] == res["checkpoint"]["phase_rng_state"]

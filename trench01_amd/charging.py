@@ -31,12 +31,12 @@ class ChargingSetup:
     """Host-side, once per geometry: geometry, LU, Green's matrix on the rows
     the push reads, leak operator, floating-conductor response, monitors."""
 
-    def __init__(self, p):
+    def __init__(self, p, circuit=None):
         if p.wall_model not in model.WALL_MODELS:
             raise ValueError(f"wall_model must be one of {model.WALL_MODELS}: {p.wall_model!r}")
         self.p = p
         self.geo = geo = model.Geometry(p)
-        self.fields = fields = model.Fields(p, geo)
+        self.fields = fields = model.Fields(p, geo, circuit)
         N = geo.solid.size
         surf = geo.surface
         m = surf.size
@@ -121,8 +121,13 @@ def ion_spec(p):
     normal of ion_temp_eV), or a polar-angle Gaussian with a monoenergetic or
     arcsine-shaped IED (``ion_distribution`` "mono" / "arcsine")."""
     kind = getattr(p, "ion_distribution", "thermal")
+    e_ion = p.ion_energy_eV
+    if p.field_mode in model.CIRCUIT_FIELDS:
+        # the circuit fields accelerate the ions inside the domain: they enter at
+        # the Bohm energy Te/2 (trench_charging_2d.py:1131-1136)
+        kind, e_ion = "thermal", p.electron_temp_eV / 2
     m_i = p.ion_mass_amu * AMU
-    spec = _lib.InjectParams(qm_i=E_CHARGE / m_i, qm_e=-E_CHARGE / M_E, e_ion=float(p.ion_energy_eV),
+    spec = _lib.InjectParams(qm_i=E_CHARGE / m_i, qm_e=-E_CHARGE / M_E, e_ion=float(e_ion),
                         e_lo=float(getattr(p, "ion_energy_lo_eV", 0.0)),
                         e_hi=float(getattr(p, "ion_energy_hi_eV", 0.0)),
                         sigma_theta=float(np.radians(getattr(p, "ion_angle_deg", 0.0))),
@@ -168,7 +173,9 @@ class DeviceCharging:
         self._h = h
         self.m = s.geo.surface.size
         self.N = s.geo.solid.size
-        self.prm = self.tracer.params_for(p, p.wall_model == "barrier")
+        self.prm = self.tracer.params_for(p, p.wall_model == "barrier", rf=s.fields.rf is not None)
+        if s.fields.rf is not None:
+            self.tracer.set_rf(s.fields.rf)
         self.capacity = 0
 
     def set_schedule(self, **kw):
@@ -277,23 +284,29 @@ def resume_params(res, p):
     from dataclasses import fields as dc_fields, replace
     names = {f.name for f in dc_fields(model.Params)}
     source = model.Params(**{k: v for k, v in res["params"].items() if k in names})
+    # the display phase of an RF-cycle run changes only what is shown (:963)
+    editable = RESUME_EDITABLE | ({"rf_phase"} if source.field_mode == "circuit_rf" else set())
     changed = []
     for k, v in asdict(source).items():
         entered = getattr(p, k, v)
         equal = np.isclose(entered, v, rtol=1e-9, atol=0) if isinstance(v, float) else entered == v
-        if k not in RESUME_EDITABLE and not equal and not (k == "backend"):
+        if k not in editable and not equal and not (k == "backend"):
             changed.append(k)
     if changed:
         raise ValueError("a resumed run must keep the geometry and physics of the stored one: " + ", ".join(changed))
-    return replace(source, **{k: getattr(p, k) for k in RESUME_EDITABLE if hasattr(p, k)})
+    return replace(source, **{k: getattr(p, k) for k in editable if hasattr(p, k)})
 
 
 def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=None, ion_velocity=None,
-        device=0, resume=None, device_monitors=True):
+        device=0, resume=None, device_monitors=True, circuit=None):
     """Charging run (trench_charging_2d.py:1045-1369) with the batch on the GPU.
     ``resume``: a result of ``run`` or ``results.load_results``; its charge,
     histories and random state are continued (fixed length: n_batches more;
-    until_steady: max_batches in total).  Returns the reference's result dict
+    until_steady: max_batches in total).  ``circuit``: the RF circuit waveform
+    for field_mode "circuit" / "circuit_rf" (the reference's
+    PC.solve_circuit_cached output: t, uw, um, up, V1 over one period); with
+    "circuit_rf" every batch draws stratified launch phases and the push
+    follows the voltage wave in flight.  Returns the reference's result dict
     (plus ``timing``), or None if stopped before the first batch."""
     log = log or (lambda s: None)
     start_batch = 0
@@ -310,8 +323,18 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
                            or not np.isfinite(p.steady_charge_rtol) or p.steady_charge_rtol <= 0
                            or not np.isfinite(p.steady_charge_atol) or p.steady_charge_atol < 0):
         raise ValueError("steady_window >= 5, steady_hold >= 1, positive tolerances are required")
+    if p.field_mode in model.CIRCUIT_FIELDS:
+        if p.bias != "rf":
+            raise ValueError("circuit field modes need bias=rf")
+        if circuit is None and resume is not None:
+            circuit = resume.get("circuit")
+        if circuit is None:
+            raise ValueError("circuit field modes need the circuit waveform (circuit=...)")
+        # ions enter at the Bohm energy and are accelerated inside the domain (:1131-1136)
+        ion_velocity = None
+        ion_energy = lambda rng_, n_: np.full(n_, p.electron_temp_eV / 2)
     t_setup = time.perf_counter()
-    s = ChargingSetup(p)
+    s = ChargingSetup(p, circuit)
     geo, fields = s.geo, s.fields
     nx, ny = geo.solid.shape
     dev = DeviceCharging(s, device=device)
@@ -319,6 +342,8 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
         f"{geo.surface.size} surface cells, {s.green.shape[1]} Green's columns on {s.rows.size} rows")
     m = geo.surface.size
     rng = np.random.default_rng(p.seed)
+    # stratified RF launch phases (trench_charging_2d.py:1079, :1231-1233)
+    phase_rng = np.random.default_rng(np.random.SeedSequence([p.seed, 0x5246])) if fields.rf is not None else None
     rho_full = np.zeros(geo.solid.size)
     n_ok = 0
     if resume is not None:
@@ -333,6 +358,10 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
             if not checkpoint.get("rng_state"):
                 raise ValueError("host injection needs the stored random state to resume")
             rng.bit_generator.state = checkpoint["rng_state"]
+        if phase_rng is not None:
+            if not checkpoint.get("phase_rng_state"):
+                raise ValueError("an RF-cycle run needs the stored phase random state to resume")
+            phase_rng.bit_generator.state = checkpoint["phase_rng_state"]
         n_ok = int(checkpoint.get("steady_n_ok", 0) or 0)
     dev.set_rho(rho_full[geo.surface])
     n = 2 * p.n_per_batch
@@ -340,6 +369,10 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     t_setup = time.perf_counter() - t_setup
 
     probes = s.probes
+    # with the RF cycle the criterion scales by the potential relative to the
+    # period-mean external potential (:1167-1168)
+    steady_reference = ({k: float(fields.external[ix, iy]) for k, (ix, iy) in probes.items()}
+                        if fields.rf is not None else None)
     hist = {k: [] for k in ("t", "ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "leak", *probes)}
     charge_hist = {"left": [], "right": []}
     charge_monitor_hist = {k: [] for k in (*s.regions, "surface absolute")}
@@ -366,6 +399,9 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
             dev.tracer.upload(0, *model.inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity))
         else:
             dev.inject_device(0, p.n_per_batch, p.seed, b)
+        if phase_rng is not None:
+            dev.tracer.upload_times(0, np.concatenate(
+                [model.launch_times(phase_rng, p.n_per_batch, fields.rf["period"]) for _ in range(2)]))
         d = dev.step(0, n)
         if d.tally.off_surface:
             raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
@@ -403,7 +439,7 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
         W = p.steady_window
         check_info = None
         if p.until_steady and (b + 1) % W == 0 and b + 1 >= 2 * W:
-            v_ok, name, dvolt, tol = monitors.check_steady(hist, probes, p)
+            v_ok, name, dvolt, tol = monitors.check_steady(hist, probes, p, steady_reference)
             charge = monitors.check_charge_steady(charge_monitor_hist, p)
             ok = bool(v_ok and charge["ok"])
             n_ok = min(n_ok + 1, p.steady_hold) if ok else 0
@@ -437,25 +473,28 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     steady = {k: float(hist[k][-Wl:].mean()) for k in probes}
     rho = rho_full.reshape(nx, ny).copy()
     # Final potential and field on the whole grid with the LU (the device keeps
-    # only the rows the push reads).
+    # only the rows the push reads); with the RF cycle the fields are shown at
+    # p.rf_phase around the period-mean potential (:1353-1358).
     phi_full = fields.solve((-rho / EPS0).ravel()).reshape(nx, ny) + fields.external
-    Ex, Ey = model.compute_field(phi_full, geo.vac, p, fields.volts["top"])
+    shown_phi, shown_ext, shown_rhs, shown_volts = fields.snapshot(phi_full, p)
+    Ex, Ey = model.compute_field(shown_phi, geo.vac, p, shown_volts["top"])
     shown = rho.copy()
     if fields.cond.any():
-        induced = (fields.A @ phi_full.ravel()).reshape(rho.shape) - fields.boundary_rhs
+        induced = (fields.A @ shown_phi.ravel()).reshape(rho.shape) - shown_rhs
         shown[fields.cond] = (-EPS0 * induced)[fields.cond]
     t_conv = converged_at * p.dt_batch if converged_at else np.nan
     nb = len(hist["t"]) - start_batch
-    return dict(phi=phi_full, mean_phi=phi_full, rho=shown, Ex=Ex, Ey=Ey, rho_raw=rho,
-                external_phi=fields.external, voltages=fields.volts,
+    return dict(phi=shown_phi, mean_phi=phi_full, rho=shown, Ex=Ex, Ey=Ey, rho_raw=rho,
+                external_phi=shown_ext, voltages=shown_volts,
                 charge_hist={k: np.array(v) for k, v in charge_hist.items()},
                 charge_monitor_hist={k: np.array(v) for k, v in charge_monitor_hist.items()},
                 steady_criterion_version=STEADY_CRITERION_VERSION,
                 checkpoint=dict(version=1, batches_completed=len(hist["t"]),
                                 rng_state=rng.bit_generator.state if injection == "host" else None,
-                                phase_rng_state=None, steady_n_ok=n_ok),
+                                phase_rng_state=phase_rng.bit_generator.state if phase_rng is not None else None,
+                                steady_n_ok=n_ok),
                 start_batch=start_batch, params=asdict(p), converged=converged_at is not None, backend_used="hip",
                 solid=geo.solid, mask=geo.mask, i0=geo.i0, i1=geo.i1, hist=hist, probes=probes, steady=steady,
-                t_conv=t_conv, stopped=stopped, circuit=None,
+                t_conv=t_conv, stopped=stopped, circuit=circuit,
                 timing=dict(setup_s=t_setup, loop_s=wall, batches=nb, push_ms=push_ms, update_ms=update_ms,
                             device_phi=phi.reshape(nx, ny)))

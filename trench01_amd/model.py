@@ -21,7 +21,9 @@ AMU = 1.66053906660e-27
 
 MASK_TYPES = ("conductor", "dielectric")
 WALL_MODELS = ("barrier", "absorb")
-FIELD_MODES = ("floating", "fixed")
+FIELD_MODES = ("floating", "fixed", "circuit", "circuit_rf")      # trench_charging_2d.py:97-98
+CIRCUIT_FIELDS = ("circuit", "circuit_rf")
+SIDE_BCS = ("periodic", "pillar", "fixed")                         # :99
 
 
 @dataclass
@@ -43,6 +45,8 @@ class Params:
     mask_type: str = "conductor"
     mask_eps_r: float = 3.0
     field_mode: str = "floating"
+    rf_phase: float = 0.0
+    rf_time_steps: int = 256
     substrate_voltage: float = 0.0
     mask_voltage: float = 0.0
     top_voltage: float = 0.0
@@ -67,6 +71,19 @@ class Params:
     ion_energy_lo_eV: float = 0.0
     ion_energy_hi_eV: float = 0.0
     bias: str = "dc"
+    # RF bias source (the equivalent-circuit inputs of the reference; the
+    # circuit itself is solved outside this package and passed as ``circuit``)
+    rf_freq: float = 13.56e6
+    rf_volt: float = 100.0
+    rf_wave: str = "sine"
+    rf_duty: float = 0.1
+    rf_rise: float = 1e-9
+    c_block: float = 4900e-12
+    wafer_d: float = 0.3
+    wall_ratio: float = 5.0
+    ied_model: str = "sheath"
+    gas_pressure: float = 1.0
+    gas_temp: float = 300.0
     sigma_s: float = 1e-15
     until_steady: bool = False
     max_batches: int = 3000
@@ -272,54 +289,146 @@ class PoissonSolver:
         return self.lu.solve(b)
 
 
-def boundary_rhs(p, eps, volts):
+def circuit_voltages(c, phase):
+    """trench_charging_2d.py:312-320: electrode potentials of the circuit
+    waveform at a phase [deg], periodically interpolated, with the plasma
+    potential as 0 V."""
+    t = np.asarray(c["t"])
+    period = (t[1] - t[0]) * t.size
+    at = (phase % 360.0) / 360.0 * period
+    values = {k: float(np.interp(at, np.append(t, period), np.append(c[k], c[k][0]))) for k in ("uw", "um", "up", "V1")}
+    return dict(substrate=values["uw"] - values["up"], mask=values["um"] - values["up"], top=0.0,
+                sheath=values["V1"])
+
+
+def rf_coefficients(rf, at):
+    """trench_rf.py:5-11: the voltage wave (deviation from the period mean,
+    [substrate, mask]) at times ``at``, periodic linear interpolation."""
+    wave = np.asarray(rf["wave"])
+    n_phase = len(wave)
+    pos = np.mod(np.asarray(at, dtype=float), rf["period"]) / rf["period"] * n_phase
+    node = np.floor(pos)
+    k = node.astype(np.int64) % n_phase
+    frac = (pos - node)[..., None]
+    return wave[k] * (1 - frac) + wave[(k + 1) % n_phase] * frac
+
+
+def launch_times(rng, n, period):
+    """trench_rf.py:14-18: one launch time per stratum of the period, shuffled."""
+    times = (np.arange(n) + rng.random(n)) * period / n
+    rng.shuffle(times)
+    return times
+
+
+def boundary_rhs(p, eps, volts, fixed_sides=True):
     """trench_charging_2d.py:323-337: right-hand side of the Dirichlet faces
-    (substrate, top, and the dielectric's side faces when side_bc is fixed)."""
+    (substrate, top, and the dielectric's side faces: fixed voltages, or with
+    side_bc pillar a linear ramp from the substrate to the mask potential)."""
     rhs = np.zeros_like(eps)
     rhs[:, 0] -= 2.0 * eps[:, 0] / p.dx ** 2 * volts["substrate"]
     if p.top_bc == "dirichlet":
         rhs[:, -1] -= 2.0 * eps[:, -1] / p.dx ** 2 * volts["top"]
     top_ox = p.floor_t + p.trench_d
-    if p.side_bc == "fixed":
-        rhs[0, :top_ox] -= 2.0 * eps[0, :top_ox] / p.dx ** 2 * p.left_voltage
-        rhs[-1, :top_ox] -= 2.0 * eps[-1, :top_ox] / p.dx ** 2 * p.right_voltage
-    elif p.side_bc != "periodic":
-        raise NotImplementedError(f"side_bc {p.side_bc!r} is outside the charging path built here")
+    if p.side_bc == "pillar":
+        yfrac = (np.arange(top_ox) + 0.5) / top_ox
+        vl = vr = volts["substrate"] + (volts["mask"] - volts["substrate"]) * yfrac
+    else:
+        vl, vr = (p.left_voltage, p.right_voltage) if fixed_sides else (0.0, 0.0)
+    if p.side_bc != "periodic":
+        rhs[0, :top_ox] -= 2.0 * eps[0, :top_ox] / p.dx ** 2 * vl
+        rhs[-1, :top_ox] -= 2.0 * eps[-1, :top_ox] / p.dx ** 2 * vr
     return rhs
 
 
 class Fields:
-    """Poisson set-up of one geometry (trench_charging_2d.py:340-395): the LU,
+    """Poisson set-up of one geometry (trench_charging_2d.py:340-413): the LU,
     the potential of the applied boundary voltages (``external``), the wall
-    barrier face weights, and the conductor bookkeeping."""
+    barrier face weights, the conductor bookkeeping, and for field_mode
+    "circuit_rf" the RF basis (``rf``: unit substrate / mask responses, the
+    voltage wave around its period mean, the step cap dt_max).  The circuit
+    modes take the circuit waveform (dict of t, uw, um, up, V1 over one
+    period) as ``circuit``."""
 
-    def __init__(self, p, geo):
-        if p.field_mode not in FIELD_MODES:
-            raise NotImplementedError(f"field_mode {p.field_mode!r} is outside the charging path built here")
+    def __init__(self, p, geo, circuit=None):
+        if p.field_mode not in FIELD_MODES or p.side_bc not in SIDE_BCS:
+            raise ValueError(f"field_mode must be one of {FIELD_MODES} and side_bc one of {SIDE_BCS}")
         if p.top_bc not in ("dirichlet", "neumann"):
             raise ValueError("top_bc must be dirichlet or neumann")
-        self.volts = dict(substrate=p.substrate_voltage, mask=p.mask_voltage, top=p.top_voltage)
+        numeric = (p.rf_phase, p.substrate_voltage, p.mask_voltage, p.top_voltage, p.left_voltage, p.right_voltage)
+        if not np.isfinite(numeric).all():
+            raise ValueError("potentials and the RF phase must be finite")
         driven = p.field_mode != "floating"
         if driven and geo.mask.any() and p.mask_type != "conductor":
             raise ValueError("a mask voltage needs mask_type=conductor")
+        if p.side_bc == "pillar" and not driven:
+            raise ValueError("side_bc pillar needs field_mode fixed / circuit / circuit_rf")
+        if p.field_mode in CIRCUIT_FIELDS:
+            if p.bias != "rf" or circuit is None or p.top_bc != "dirichlet":
+                raise ValueError("circuit field modes need bias=rf, the circuit waveform and top_bc=dirichlet")
+            volts = circuit_voltages(circuit, p.rf_phase)
+        else:
+            volts = dict(substrate=p.substrate_voltage, mask=p.mask_voltage, top=p.top_voltage)
+        if p.field_mode == "circuit_rf":
+            if not isinstance(p.rf_time_steps, (int, np.integer)) or p.rf_time_steps < 16:
+                raise ValueError("rf_time_steps must be an integer >= 16")
+            t = np.asarray(circuit["t"])
+            period = (t[1] - t[0]) * t.size
+            if (not np.isfinite(period) or period <= 0 or not np.isclose(t[0], 0, atol=period * 1e-12)
+                    or not np.allclose(np.diff(t), period / t.size, rtol=1e-10, atol=period * 1e-12)):
+                raise ValueError("the circuit waveform must be one equally spaced period starting at 0")
+            if not np.isfinite(p.dt_batch) or p.dt_batch < period * (1 - 1e-12):
+                raise ValueError("circuit_rf needs dt_batch of at least one RF period")
+            volt_wave = np.column_stack((circuit["uw"] - circuit["up"], circuit["um"] - circuit["up"]))
+            mean = volt_wave.mean(axis=0)
+            volts = dict(substrate=float(mean[0]), mask=float(mean[1]), top=0.0, sheath=float(np.mean(circuit["V1"])))
+        self.volts = volts
         self.cond = geo.mask if p.mask_type == "conductor" else np.zeros_like(geo.mask)
         self.eps = geo.permittivity()
         self.A = poisson_matrix(self.eps, self.cond, p)
         self.solver = PoissonSolver(self.A, self.cond, driven)
         if driven and self.cond.any():
-            phi_cond = self.cond.ravel().astype(float) * self.volts["mask"]
+            phi_cond = self.cond.ravel().astype(float) * volts["mask"]
             mask_rhs = -(self.A @ phi_cond)
         else:
             phi_cond = np.zeros(self.cond.size)
             mask_rhs = np.zeros(self.cond.size)
-        self.boundary_rhs = boundary_rhs(p, self.eps, self.volts)
+        self.boundary_rhs = boundary_rhs(p, self.eps, volts)
         self.external = (self.solver.solve(self.boundary_rhs.ravel() + mask_rhs) + phi_cond).reshape(self.eps.shape)
         self.floating = bool(self.cond.any() and not driven)
         self.driven = driven
         self.wface = np.where(self.cond, 1.0, self.eps / (1.0 + self.eps))   # trench_charging_2d.py:1085
+        self.rf = None
+        if p.field_mode == "circuit_rf":
+            # the instantaneous field is linear in the electrode voltages (:397-412)
+            basis, rhs_basis = [], []
+            for vs, vm in ((1.0, 0.0), (0.0, 1.0)):
+                brhs = boundary_rhs(p, self.eps, dict(substrate=vs, mask=vm, top=0.0), fixed_sides=False)
+                bc = self.cond.ravel().astype(float) * vm
+                basis.append((self.solver.solve(brhs.ravel() - self.A @ bc) + bc).reshape(self.eps.shape))
+                rhs_basis.append(brhs)
+            basis = np.array(basis)
+            electric = [compute_field(v, geo.vac, p, 0.0) for v in basis]
+            dt_max = period / p.rf_time_steps
+            if p.rf_wave == "pulse":
+                dt_max = min(dt_max, p.rf_rise / 8)
+            self.rf = dict(period=period, wave=volt_wave - mean, phi=basis, Ex=np.array([v[0] for v in electric]),
+                           Ey=np.array([v[1] for v in electric]), dt_max=dt_max, boundary_rhs=np.array(rhs_basis))
 
     def solve(self, b):
         return self.solver.solve(b)
+
+    def snapshot(self, mean_phi, p):
+        """trench_charging_2d.py:416-426: the potential, external potential,
+        boundary right-hand side and electrode voltages shown at p.rf_phase
+        (circuit_rf); the period means otherwise."""
+        if self.rf is None:
+            return mean_phi, self.external, self.boundary_rhs, self.volts
+        delta = rf_coefficients(self.rf, (p.rf_phase % 360) / 360 * self.rf["period"])
+        shift = np.einsum("i,ijk->jk", delta, self.rf["phi"])
+        rhs = self.boundary_rhs + np.einsum("i,ijk->jk", delta, self.rf["boundary_rhs"])
+        volts = {**self.volts, "substrate": self.volts["substrate"] + float(delta[0]),
+                 "mask": self.volts["mask"] + float(delta[1])}
+        return mean_phi + shift, self.external + shift, rhs, volts
 
 
 def compute_field(phi, vac, p, top_voltage=None):

@@ -29,8 +29,35 @@ def _reference_params(params):
     return out
 
 
+def _circuit_arrays(c):
+    """The circuit waveform as ``rf_*`` arrays plus ``circuit_meta_json`` (its
+    scalar entries; a sheath record's scalars nested), :1692-1703."""
+    if c is None:
+        return {}
+    out = {f"rf_{k}": v for k, v in c.items() if isinstance(v, np.ndarray)}
+    meta = {k: v for k, v in c.items() if not isinstance(v, np.ndarray) and k != "sheath"}
+    sheath = c.get("sheath")
+    meta["sheath"] = None if sheath is None else {k: v for k, v in sheath.items() if not isinstance(v, np.ndarray)}
+    if sheath is not None:
+        out.update({f"rf_sheath_{k}": v for k, v in sheath.items() if isinstance(v, np.ndarray)})
+    out["circuit_meta_json"] = np.array(json.dumps(meta))
+    return out
+
+
+def _circuit_from(d):
+    """Inverse of _circuit_arrays (:1769-1774)."""
+    if "circuit_meta_json" not in d:
+        return None
+    c = json.loads(str(d["circuit_meta_json"]))
+    c.update({k[3:]: d[k].copy() for k in d.files if k.startswith("rf_") and not k.startswith("rf_sheath_")})
+    if c.get("sheath") is not None:
+        c["sheath"].update({k[10:]: d[k].copy() for k in d.files if k.startswith("rf_sheath_")})
+    return c
+
+
 def save_results(res, out):
-    """Write ``out + '.npz'`` (trench_charging_2d.py:1687-1734, the arrays only)."""
+    """Write ``out + '.npz'`` (trench_charging_2d.py:1687-1734, the arrays only;
+    with an RF circuit its waveform as well)."""
     params = res["params"]
     extra = {k: params[k] for k in EXTRA_FIELDS if k in params}
     path = f"{out}.npz"
@@ -47,7 +74,7 @@ def save_results(res, out):
              **{f"qmonitor_{k}": v for k, v in res.get("charge_monitor_hist", {}).items()},
              **{f"charge_{k}": v for k, v in res["charge_hist"].items()},
              **{f"hist_{k}": v for k, v in res["hist"].items()},
-             **{f"steady_{k}": v for k, v in res["steady"].items()})
+             **{f"steady_{k}": v for k, v in res["steady"].items()}, **_circuit_arrays(res.get("circuit")))
     return path
 
 
@@ -86,6 +113,7 @@ def load_results(filename):
                 res["checkpoint"] = cp
         if conv.get("note"):
             res["convergence_note"] = conv["note"]
+        res["circuit"] = _circuit_from(d)
     # A stored "converged" flag is re-checked against the charge histories
     # (trench_charging_2d.py:1780-1800).
     if res["converged"]:
