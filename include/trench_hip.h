@@ -47,6 +47,7 @@ typedef struct th_trace_params {
     int32_t pass_steps;   /* multi-pass schedule: first pass stops particles at this many steps and
                              later passes continue the survivors with doubling caps; 0 = one pass */
     int32_t rf;           /* 1: RF full-cycle push (th_set_rf, launch times per particle), FP64 only */
+    int32_t blocks_per_cu; /* schedule: resident workgroups per CU at most, 0 = as many as fit */
 } th_trace_params;
 
 /* Scalar tallies of one trace call (trench_cuda.py:119-122). */
@@ -275,6 +276,58 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
  * kernel time [ms].  No reference counterpart: it calibrates the HBM traffic
  * counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, double *ms);
+
+/* ---------------------------------------------------------------------------
+ * Ensemble push: many trench instances (own grid, fields, taper, particle
+ * count, ion distributions) traced together by one persistent launch per
+ * pass.  No reference counterpart -- the reference traces one geometry per
+ * call (CudaTracer.__call__, trench_cuda.py:41-138, one launch per 256-step
+ * chunk); every instance's counts, escapes, losses and steps equal a
+ * single-instance th_trace of the same particles bit for bit.  FP64, no RF,
+ * one dx / cfl / max_steps / wall model for the whole ensemble.
+ */
+typedef struct th_ensemble th_ensemble;
+
+int th_ensemble_create(int32_t device, th_ensemble **out);
+void th_ensemble_destroy(th_ensemble *e);
+
+/* Appends an instance: vacuum map and fields as th_tracer_create /
+ * th_set_fields (phi, wface nullable together), `tapered` as
+ * th_trace_params.tapered, and its batch size in particles (ions first, then
+ * electrons).  Instances are added before the first upload, injection or run;
+ * *index receives the instance's index. */
+int th_ensemble_add(th_ensemble *e, const uint8_t *vac, int32_t nx, int32_t ny, const double *Ex, const double *Ey,
+                    const double *phi, const double *wface, int32_t tapered, int64_t n_particles, int32_t *index);
+int th_ensemble_set_fields(th_ensemble *e, int32_t index, const double *Ex, const double *Ey, const double *phi,
+                           const double *wface);
+/* Surface cells of an instance (as th_tracer_surface). */
+int th_ensemble_surface(const th_ensemble *e, int32_t index, int32_t *m, int32_t *flat);
+/* Device injection of an instance's batch (th_inject's distributions; ip->n
+ * is half the instance's particle count). */
+int th_ensemble_set_inject(th_ensemble *e, int32_t index, const th_inject_params *ip);
+/* Host particle states of an instance (n = its particle count). */
+int th_ensemble_upload(th_ensemble *e, int32_t index, const double *x, const double *y, const double *vx,
+                       const double *vy, const double *qm, const int8_t *spc, int64_t n);
+
+/* Runs `batches` batches and accumulates the tallies of every instance
+ * (cleared first).  flags bit 0: inject every batch on the device (batch
+ * indices first_batch, first_batch + 1, ...), else trace the uploaded states
+ * each batch; bit 1: keep per-particle status and hit cell (modes 0 and 2).
+ * mode 0: one launch per pass over all instances; 1: the same batch captured
+ * once as a hipGraph and replayed; 2: one launch per instance and pass (the
+ * reference's one-geometry-per-call schedule).  Results do not depend on the
+ * mode; the elapsed time is th_ensemble_last_ms. */
+int th_ensemble_run(th_ensemble *e, const th_trace_params *p, uint64_t seed, uint64_t first_batch, int32_t batches,
+                    int32_t mode, int32_t flags);
+/* Tallies of one instance after th_ensemble_run: counts on its grid (nx*ny,
+ * nullable together), scalars, and per-particle status / instance-local hit
+ * cell (nullable; need flags bit 1). */
+int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
+                     int32_t *hit);
+/* The particle states of an instance in the pool (after injection: the batch's initial states). */
+int th_ensemble_read_particles(th_ensemble *e, int32_t index, double *x, double *y, double *vx, double *vy,
+                               double *qm, int8_t *spc);
+int th_ensemble_last_ms(const th_ensemble *e, double *ms);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ class TraceParams(ctypes.Structure):
                 ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32), ("order", ctypes.c_int32),
                 ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("pass_steps", ctypes.c_int32),
-                ("rf", ctypes.c_int32)]
+                ("rf", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32)]
 
 
 class Tally(ctypes.Structure):
@@ -101,6 +101,19 @@ SIGNATURES = {
     "th_charging_set_monitors": (ctypes.c_int, [vp, ctypes.c_int32, i32p, ctypes.c_int32, i32p, i32p,
                                                 ctypes.c_int32, ctypes.c_int32, i32p]),
     "th_charging_monitors": (ctypes.c_int, [vp, dp]),
+    "th_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(vp)]),
+    "th_ensemble_destroy": (None, [vp]),
+    "th_ensemble_add": (ctypes.c_int, [vp, u8p, ctypes.c_int32, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_int32,
+                                       ctypes.c_int64, i32p]),
+    "th_ensemble_set_fields": (ctypes.c_int, [vp, ctypes.c_int32, dp, dp, dp, dp]),
+    "th_ensemble_surface": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i32p]),
+    "th_ensemble_set_inject": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(InjectParams)]),
+    "th_ensemble_upload": (ctypes.c_int, [vp, ctypes.c_int32, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
+    "th_ensemble_run": (ctypes.c_int, [vp, ctypes.POINTER(TraceParams), ctypes.c_uint64, ctypes.c_uint64,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "th_ensemble_read": (ctypes.c_int, [vp, ctypes.c_int32, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_ensemble_read_particles": (ctypes.c_int, [vp, ctypes.c_int32, dp, dp, dp, dp, dp, i8p]),
+    "th_ensemble_last_ms": (ctypes.c_int, [vp, dp]),
 }
 
 _lib = None

@@ -75,6 +75,7 @@ struct th_tracer {
     unsigned long long *d_scal = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the push launches
     double push_ms = 0.0;
+    int blocks_per_cu = 0;                     // launch cap of the current trace (0: occupancy limit)
     char name[256] = "";
 };
 
@@ -137,7 +138,7 @@ int reserve_particles(th_tracer *t, int64_t n) {
 int pack_corners(th_tracer *t) {
     int keys = (t->nx + 1) * (t->ny + 1);
     hipLaunchKernelGGL(th::pack_corners_kernel, dim3((keys + 255) / 256), dim3(256), 0, t->stream, t->d_cells,
-                       t->d_vac, t->nx, t->ny, t->d_corners, t->d_corner_vac);
+                       t->d_vac, t->nx, t->ny, 0, 0, t->d_corners, t->d_corner_vac);
     TH_HIP(hipGetLastError());
     return TH_OK;
 }
@@ -177,6 +178,7 @@ template <typename K>
 int launch_kernel(th_tracer *t, K kernel, const th::Grid &g, const th::Work &w, int64_t want, size_t lds) {
     int per_cu = 0;
     TH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, th::kBlock, lds));
+    if (t->blocks_per_cu > 0) per_cu = std::min(per_cu, t->blocks_per_cu);
     if (per_cu < 1) per_cu = 1;
     int blocks = (int)std::min<int64_t>(want, (int64_t)t->cus * per_cu);
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(th::kBlock), lds, t->stream, g, w);
@@ -335,6 +337,7 @@ int check_params(const th_tracer *t, const th_trace_params *p) {
         return fail(TH_EINVAL, "tracing needs 0 < cfl <= 1 and max_steps >= 1");
     if (!(p->dx > 0.0)) return fail(TH_EINVAL, "dx must be positive");
     if (p->precision != 0 && p->precision != 1) return fail(TH_EINVAL, "precision %d is not available", p->precision);
+    if (p->blocks_per_cu < 0) return fail(TH_EINVAL, "blocks_per_cu must be >= 0");
     if (!t->fields) return fail(TH_ESTATE, "th_set_fields must precede tracing");
     if (p->barrier && !t->barrier_fields) return fail(TH_ESTATE, "barrier wall model needs phi and wface");
     if (p->rf) {
@@ -365,6 +368,7 @@ int upload(th_tracer *t, int64_t off, const double *x, const double *y, const do
 int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
                  bool want_status, bool want_hit, int64_t *traced, th_tally *tally,
                  const th::Record *rec = nullptr) {
+    t->blocks_per_cu = p->blocks_per_cu;
     TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
     TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
     TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
@@ -1245,3 +1249,6 @@ extern "C" int th_charging_monitors(const th_charging *c, double *out) {
     memcpy(out, c->mon_host.data(), sizeof(double) * c->mon_host.size());
     return TH_OK;
 }
+
+// Ensemble push (many trench instances per launch): same translation unit.
+#include "th_ensemble_capi.hpp"

@@ -1,0 +1,244 @@
+// Ensemble push: many trench instances (each its own grid, fields, taper and
+// particle count) traced by one persistent launch.
+//
+// The records of all instances sit in shared arenas (cells, interpolation
+// corners, vacuum map, wall weights, surface map); an instance is a table row
+// with its grid size and its offsets into the arenas and the particle pool.
+// The work queue runs over the whole pool, so a lane that finishes a particle
+// of one instance takes the next particle of whatever instance comes next:
+// small instances do not leave the chip idle and long-lived particles of one
+// instance overlap with the bulk of the others.  Per step a lane reads its
+// instance through a LaneView (uniform arena bases in scalar registers, the
+// instance's grid size and offsets in vector registers) with the same FP64
+// step as the single-instance push (step64 in th_push.hpp), so every instance
+// gets the counts the single-instance push would give it, bit for bit.
+//
+// Tallies: the surface slots of all instances are numbered globally (an
+// instance's slots are contiguous); absorptions are global integer atomics
+// into [2][slots] counters, exact and independent of order.  LDS
+// privatisation as in the single-instance push does not apply: the slots of
+// hundreds of instances do not fit in LDS, and a workgroup's lanes hold
+// particles of many instances at once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "th_inject.hpp"
+#include "th_push.hpp"
+
+namespace th {
+
+enum : int { kInstEsc0 = 0, kInstEsc1, kInstLost, kInstSteps, kInstOff, kInstCount };
+
+// One instance: grid size, offsets into the arenas and the pool.
+struct Instance {
+    int32_t nx, ny;
+    int32_t cell0;     // first cell of its records in the cell arenas
+    int32_t key0;      // first lattice key of its corner records
+    int32_t tapered;   // corner crossings count (tapered walls)
+    int32_t pad;
+    int64_t p0, p1;    // pool range [p0, p1): ions, then electrons
+};
+
+struct EnsembleGrid {
+    const double4 *__restrict__ cells;       // {Ex*vac, Ey*vac, vac, phi} per arena cell
+    const double2 *__restrict__ corners;     // [keys][4]
+    const uint8_t *__restrict__ corner_vac;  // [keys]
+    const double *__restrict__ wface;        // per arena cell
+    const uint8_t *__restrict__ vac;         // per arena cell
+    const int32_t *__restrict__ slot;        // arena cell -> global surface slot, -1 elsewhere
+    const Instance *__restrict__ inst;       // [n_inst]
+    const int64_t *__restrict__ starts;      // [n_inst + 1]: p0 of every instance, then the pool size
+    int n_inst, slots;
+    double dx, inv_dx, cfl_dx, two_dx;
+};
+
+struct EnsembleWork {
+    const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
+    const int8_t *__restrict__ spc;
+    int64_t begin, end;              // pool range of this launch (first pass)
+    int8_t *status;                  // nullable
+    int32_t *hit;                    // nullable: instance-local cell index or -1
+    uint32_t *counts;                // [2][slots]
+    unsigned long long *inst_scal;   // [n_inst][kInstCount]
+    unsigned long long *queue;       // this launch's work counter (zero at launch)
+    int max_steps, refill, check_steps, step_cap;
+    const Survivor *in;              // later passes: survivors of the previous one
+    const unsigned *in_count;
+    Survivor *out;
+    unsigned *out_count;
+};
+
+// The FP64 step's view of one lane's instance (see step64).
+struct LaneView {
+    const double4 *__restrict__ cells;
+    const double2 *__restrict__ corners;
+    const uint8_t *__restrict__ corner_vac;
+    const double *__restrict__ wface;
+    const uint8_t *__restrict__ vac;
+    double dx, inv_dx, cfl_dx, two_dx;
+    int nx, ny, cell0, key0;
+    double lx, ly;
+    bool taper;
+    __device__ __forceinline__ double4 cell(int c) const { return cells[cell0 + c]; }
+    __device__ __forceinline__ const double2 *corner(int key) const { return corners + 4 * (int64_t)(key0 + key); }
+    __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key0 + key]; }
+    __device__ __forceinline__ bool is_vac(int c) const { return vac[cell0 + c] != 0; }
+    __device__ __forceinline__ double face(int c) const { return wface[cell0 + c]; }
+};
+
+// Index of the instance holding pool index q: the last i with starts[i] <= q.
+__device__ __forceinline__ int find_instance(const int64_t *__restrict__ starts, int n_inst, int64_t q) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool BARRIER>
+__device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const EnsembleWork &w) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int64_t q_begin = w.in ? 0 : w.begin;
+    const int64_t q_end = w.in ? (int64_t)*w.in_count : w.end;
+    LaneView v;
+    v.cells = g.cells; v.corners = g.corners; v.corner_vac = g.corner_vac; v.wface = g.wface; v.vac = g.vac;
+    v.dx = g.dx; v.inv_dx = g.inv_dx; v.cfl_dx = g.cfl_dx; v.two_dx = g.two_dx;
+    v.nx = 1; v.ny = 1; v.cell0 = 0; v.key0 = 0; v.lx = g.dx; v.ly = g.dx; v.taper = false;
+    int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
+    int inst = 0;
+    State64 s;
+    int sp = 0, k = 0, k0 = 0;
+    double xp = 0.0, yp = 0.0;
+    RfClock rc{0.0, 0.0, 0.0, 0.0};
+    for (;;) {
+        unsigned long long idle = __ballot(pid == -1);
+        unsigned long long busy = __ballot(pid >= 0);
+        if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(w.queue, (unsigned long long)__popcll(idle));
+            base = __shfl(base, leader);
+            if (pid == -1) {
+                int64_t q = q_begin + (int64_t)(base + (unsigned long long)__popcll(idle & below));
+                if (q < q_end) {
+                    if (w.in) {
+                        const Survivor &c = w.in[q];
+                        pid = c.pid;
+                        s.x = c.x; s.y = c.y; s.vx = c.vx; s.vy = c.vy;
+                        k = c.k;
+                    } else {
+                        pid = q;
+                        s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q];
+                        k = 0;
+                    }
+                    k0 = k;
+                    inst = find_instance(g.starts, g.n_inst, pid);
+                    const Instance &d = g.inst[inst];
+                    v.nx = d.nx; v.ny = d.ny; v.cell0 = d.cell0; v.key0 = d.key0; v.taper = d.tapered != 0;
+                    v.lx = (double)d.nx * g.dx;   // the products the single-instance push forms on the host
+                    v.ly = (double)d.ny * g.dx;
+                    s.qm = w.qm[pid];
+                    s.q = div_dx(s.x, g.dx, g.inv_dx);
+                    s.r = div_dx(s.y, g.dx, g.inv_dx);
+                    sp = w.spc[pid] != 0;   // a species byte other than 0 / 1 cannot index past the tallies
+                } else {
+                    pid = -2;
+                }
+            }
+            busy = __ballot(pid >= 0);
+        }
+        if (busy == 0ull) break;
+        if (pid >= 0) {
+            for (int t = 0; t < w.check_steps; ++t) {
+                int cell = -1;
+                int st = step64<BARRIER, true, false, false>(v, s, cell, xp, yp, rc);
+                ++k;
+                if (st != 0 || k >= w.max_steps) {
+                    st = st != 0 ? st : 3;
+                    unsigned long long *is = w.inst_scal + (int64_t)inst * kInstCount;
+                    if (w.status) w.status[pid] = (int8_t)st;
+                    if (w.hit) w.hit[pid] = st == 1 ? cell : -1;
+                    if (st == 1) {
+                        int slot = g.slot[v.cell0 + cell];
+                        if (slot >= 0) atomicAdd(&w.counts[(int64_t)sp * g.slots + slot], 1u);
+                        else atomicAdd(&is[kInstOff], 1ull);
+                    } else {
+                        atomicAdd(&is[st == 2 ? kInstEsc0 + sp : kInstLost], 1ull);
+                    }
+                    atomicAdd(&is[kInstSteps], (unsigned long long)(k - k0));
+                    pid = -1;
+                    break;
+                }
+                if (k >= w.step_cap) {   // carry to the next pass
+                    unsigned o = atomicAdd(w.out_count, 1u);
+                    Survivor c;
+                    c.x = s.x; c.y = s.y; c.vx = s.vx; c.vy = s.vy;
+                    c.clock = 0.0;
+                    c.elapsed = 0.0;
+                    c.pid = (int32_t)pid;
+                    c.k = k;
+                    w.out[o] = c;
+                    atomicAdd(&w.inst_scal[(int64_t)inst * kInstCount + kInstSteps], (unsigned long long)(k - k0));
+                    pid = -1;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// Two register budgets: 5 waves per SIMD (at most 102 VGPRs; a few values
+// spill around the rare wall-entry path) or 4 (up to 128 VGPRs, no spills).
+// Which is faster depends on the mix of instances; the launch picks by the
+// schedule's workgroups-per-CU cap.
+template <bool BARRIER>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+ensemble_push_kernel(EnsembleGrid g, EnsembleWork w) {
+    ensemble_body<BARRIER>(g, w);
+}
+template <bool BARRIER>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+ensemble_push4_kernel(EnsembleGrid g, EnsembleWork w) {
+    ensemble_body<BARRIER>(g, w);
+}
+
+// Device injection of every instance's batch: the particle at pool index k
+// belongs to instance i = find_instance(k); it is particle k - p0 of that
+// instance's batch (ions first), drawn with the instance's distributions from
+// a Philox stream keyed by the seed and the instance index.  The batch index
+// is read from *batch (so a captured graph advances it by itself).
+__global__ void ensemble_inject_kernel(const InjectDev *__restrict__ params, const int64_t *__restrict__ starts,
+                                       int n_inst, uint64_t seed, const unsigned long long *__restrict__ batch,
+                                       double *x, double *y, double *vx, double *vy, double *qm, int8_t *spc) {
+    const int64_t total = starts[n_inst];
+    const uint64_t b = *batch;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        int i = find_instance(starts, n_inst, k);
+        InjectDev d = params[i];
+        d.batch = b;
+        double px, pvx, pvy, pqm;
+        int8_t sp;
+        draw_particle(d, k - starts[i], stream_key(seed ^ (0x632BE59BD9B4E019ull * (uint64_t)(i + 1)), b), px, pvx,
+                      pvy, pqm, sp);
+        x[k] = px; y[k] = d.y0; vx[k] = pvx; vy[k] = pvy; qm[k] = pqm; spc[k] = sp;
+    }
+}
+
+// Zero fills used inside captured batches (no memset nodes in a graph).
+__global__ void zero_u64_kernel(unsigned long long *p, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0ull;
+}
+__global__ void zero_u32_kernel(uint32_t *p, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0u;
+}
+__global__ void bump_kernel(unsigned long long *p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p += 1ull;
+}
+
+}  // namespace th

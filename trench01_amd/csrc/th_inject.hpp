@@ -55,19 +55,19 @@ struct InjectDev {
     int8_t *spc;
 };
 
-__global__ void inject_kernel(InjectDev d) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 2 * d.n) return;
+// Particle k of a batch (k < n: ions, then n electrons), drawn from the
+// Philox block (k, batch, key).
+__device__ __forceinline__ void draw_particle(const InjectDev &d, int64_t k, uint64_t key, double &x, double &vx,
+                                              double &vy, double &qm, int8_t &sp) {
     uint32_t c0[4] = {(uint32_t)k, (uint32_t)(k >> 32), (uint32_t)d.batch, 0u};
     uint32_t c1[4] = {(uint32_t)k, (uint32_t)(k >> 32), (uint32_t)d.batch, 1u};
-    Philox::block(c0, d.seed ^ (d.batch >> 32) * 0x9E3779B97F4A7C15ull);
-    Philox::block(c1, d.seed ^ (d.batch >> 32) * 0x9E3779B97F4A7C15ull);
+    Philox::block(c0, key);
+    Philox::block(c1, key);
     double u0 = u01(c0[0], c0[1]), u1 = u01(c0[2], c0[3]);
     double u2 = u01(c1[0], c1[1]), u3 = u01(c1[2], c1[3]);
     // standard normal from (u2, u3), Box-Muller
     double gauss = sqrt(-2.0 * log(u2)) * cos(6.283185307179586476925286766559 * u3);
-    double x = u0 * d.lx, vx, vy, qm;
-    int8_t sp;
+    x = u0 * d.lx;
     if (k < d.n) {
         sp = 0;
         qm = d.qm_i;
@@ -87,6 +87,18 @@ __global__ void inject_kernel(InjectDev d) {
         vx = d.s_e * gauss;
         vy = -d.s_e * sqrt(-2.0 * log(u1));
     }
+}
+
+__device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint64_t batch) {
+    return seed ^ (batch >> 32) * 0x9E3779B97F4A7C15ull;
+}
+
+__global__ void inject_kernel(InjectDev d) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 2 * d.n) return;
+    double x, vx, vy, qm;
+    int8_t sp;
+    draw_particle(d, k, stream_key(d.seed, d.batch), x, vx, vy, qm, sp);
     d.x[k] = x; d.y[k] = d.y0; d.vx[k] = vx; d.vy[k] = vy; d.qm[k] = qm; d.spc[k] = sp;
 }
 

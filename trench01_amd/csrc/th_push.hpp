@@ -67,6 +67,14 @@ struct Grid {
     const double2 *__restrict__ wave;     // [n_wave] {substrate, mask} deviation from the period mean
     int n_wave;
     double period, dt_max, bound0, bound1;   // bound = max |wave| per column
+
+    // Record access of the FP64 step (shared with the ensemble's lane view).
+    static constexpr bool taper = true;   // the TAPERED template flag decides
+    __device__ __forceinline__ double4 cell(int c) const { return cells[c]; }
+    __device__ __forceinline__ const double2 *corner(int key) const { return corners + 4 * (int64_t)key; }
+    __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key]; }
+    __device__ __forceinline__ bool is_vac(int c) const { return vac[c] != 0; }
+    __device__ __forceinline__ double face(int c) const { return wface[c]; }
 };
 
 // Record-mode outputs (trench_trace.cu:149-183, :189-192): the end point of
@@ -193,9 +201,11 @@ using State32 = StateT<float>;
 
 // One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
 // (escaped).  With RECORD, (xp, yp) is the step's recorded point.  With RF the
-// voltage wave acts during the step and rc advances.
-template <bool BARRIER, bool TAPERED, bool RECORD, bool RF>
-__device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc) {
+// voltage wave acts during the step and rc advances.  V reads one instance's
+// records: Grid itself, or an ensemble lane's view (th_ensemble.hpp); cell
+// indices are the instance's own.
+template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V>
+__device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc) {
     const int nx = g.nx, ny = g.ny;
     // Vacuum-weighted bilinear field at the cell-centred lattice.
     double fx = s.q - 0.5, fy = s.r - 0.5;
@@ -209,9 +219,9 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
     if (__builtin_expect((unsigned)(i + 1) <= (unsigned)nx && (unsigned)(j + 1) <= (unsigned)ny, 1)) {
         // one 64-byte corner record and one flag byte per step
         const int key = (i + 1) * (ny + 1) + (j + 1);
-        const double2 *c = g.corners + 4 * (int64_t)key;
+        const double2 *c = g.corner(key);
         e00 = c[0]; e10 = c[1]; e01 = c[2]; e11 = c[3];
-        const unsigned f = g.corner_vac[key];
+        const unsigned f = g.corner_flags(key);
         v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
         v01 = (f & 4u) ? 1.0 : 0.0; v11 = (f & 8u) ? 1.0 : 0.0;
         i0 = i < 0 ? nx - 1 : i;
@@ -222,8 +232,8 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
         wrap_pair(i, nx, i0, i1);
         j0 = clip_cell(j, ny);
         j1 = clip_cell(j + 1, ny);
-        double4 c00 = g.cells[i0 * ny + j0], c10 = g.cells[i1 * ny + j0];
-        double4 c01 = g.cells[i0 * ny + j1], c11 = g.cells[i1 * ny + j1];
+        double4 c00 = g.cell(i0 * ny + j0), c10 = g.cell(i1 * ny + j0);
+        double4 c01 = g.cell(i0 * ny + j1), c11 = g.cell(i1 * ny + j1);
         e00 = make_double2(c00.x, c00.y); e10 = make_double2(c10.x, c10.y);
         e01 = make_double2(c01.x, c01.y); e11 = make_double2(c11.x, c11.y);
         v00 = c00.z; v10 = c10.z; v01 = c01.z; v11 = c11.z;
@@ -279,9 +289,9 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
     // Classification of the end point.
     bool escaped = s.y >= g.ly;
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
-    bool hit = !escaped && !g.vac[ix * ny + iy];
+    bool hit = !escaped && !g.is_vac(ix * ny + iy);
     bool corner = false;
-    if (TAPERED && !hit && !escaped)
+    if (TAPERED && g.taper && !hit && !escaped)
         corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
     int cx = ix, cy = iy;
     bool face_x = false;
@@ -295,16 +305,16 @@ __device__ __forceinline__ int step64(const Grid &g, State64 &s, int &cell, doub
         bool x_first = tfx < tfy;
         int ax_ = x_first ? wrap_cell(ix0 + dix, nx) : ix0;
         int ay_ = x_first ? iy0 : iy0 + diy;
-        bool last = dix != 0 && diy != 0 && g.vac[ax_ * ny + ay_];
+        bool last = dix != 0 && diy != 0 && g.is_vac(ax_ * ny + ay_);
         cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
         cy = last ? iy0 + diy : ay_;
         int vcx = last ? ax_ : ix0, vcy = last ? ay_ : iy0;
         face_x = last ? !x_first : x_first;
-        hit = !g.vac[cx * ny + cy];
+        hit = !g.is_vac(cx * ny + cy);
         if (BARRIER && hit) {
             double vn = face_x ? s.vx : s.vy;
             int c = cx * ny + cy, v = vcx * ny + vcy;
-            double dphi = g.wface[c] * (g.cells[c].w - g.cells[v].w);
+            double dphi = g.face(c) * (g.cell(c).w - g.cell(v).w);
             if constexpr (RF) {   // the wall potential at the step's end time
                 double cs, cm;
                 rf_coefficients(g, rc.clock, cs, cm);
@@ -518,7 +528,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         s.q = div_dx(s.x, g.dx, g.inv_dx);
                         s.r = div_dx(s.y, g.dx, g.inv_dx);
                     }
-                    sp = w.spc[pid];
+                    sp = w.spc[pid] != 0;   // a species byte other than 0 / 1 cannot index past the tallies
                     if constexpr (RECORD) {
                         xp = s.x;
                         yp = s.y;
@@ -631,9 +641,12 @@ __global__ void __launch_bounds__(kBlock) push32_kernel(Grid g, Work w) {
     push_body<kModeF32, BARRIER, TAPERED>(g, w);
 }
 
-// Corner records of every lattice key (see Grid::corners) from the cell records.
+// Corner records of every lattice key (see Grid::corners) from the cell
+// records of one nx x ny grid whose cells start at cell0 and whose keys start
+// at key0 in their arrays (both 0 for a single instance).
 __global__ void pack_corners_kernel(const double4 *__restrict__ cells, const uint8_t *__restrict__ vac, int nx,
-                                    int ny, double2 *__restrict__ corners, uint8_t *__restrict__ corner_vac) {
+                                    int ny, int cell0, int key0, double2 *__restrict__ corners,
+                                    uint8_t *__restrict__ corner_vac) {
     int key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= (nx + 1) * (ny + 1)) return;
     int i = key / (ny + 1) - 1, j = key % (ny + 1) - 1;
@@ -642,11 +655,11 @@ __global__ void pack_corners_kernel(const double4 *__restrict__ cells, const uin
     int c[4] = {i0 * ny + j0, i1 * ny + j0, i0 * ny + j1, i1 * ny + j1};
     unsigned f = 0;
     for (int k = 0; k < 4; ++k) {
-        double4 r = cells[c[k]];
-        corners[4 * (int64_t)key + k] = make_double2(r.x, r.y);
-        f |= (vac[c[k]] ? 1u : 0u) << k;
+        double4 r = cells[cell0 + c[k]];
+        corners[4 * (int64_t)(key0 + key) + k] = make_double2(r.x, r.y);
+        f |= (vac[cell0 + c[k]] ? 1u : 0u) << k;
     }
-    corner_vac[key] = (uint8_t)f;
+    corner_vac[key0 + key] = (uint8_t)f;
 }
 
 // Packs the RF basis (two unit-voltage responses) into per-cell records.
