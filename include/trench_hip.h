@@ -312,7 +312,7 @@ int th_ensemble_upload(th_ensemble *e, int32_t index, const double *x, const dou
 /* Runs `batches` batches and accumulates the tallies of every instance
  * (cleared first).  flags bit 0: inject every batch on the device (batch
  * indices first_batch, first_batch + 1, ...), else trace the uploaded states
- * each batch; bit 1: keep per-particle status and hit cell (modes 0 and 2).
+ * each batch; bit 1: keep the absorbing cell per particle (modes 0 and 2).
  * mode 0: one launch per pass over all instances; 1: the same batch captured
  * once as a hipGraph and replayed; 2: one launch per instance and pass (the
  * reference's one-geometry-per-call schedule).  Results do not depend on the
@@ -320,8 +320,9 @@ int th_ensemble_upload(th_ensemble *e, int32_t index, const double *x, const dou
 int th_ensemble_run(th_ensemble *e, const th_trace_params *p, uint64_t seed, uint64_t first_batch, int32_t batches,
                     int32_t mode, int32_t flags);
 /* Tallies of one instance after th_ensemble_run: counts on its grid (nx*ny,
- * nullable together), scalars, and per-particle status / instance-local hit
- * cell (nullable; need flags bit 1). */
+ * nullable together), scalars, per-particle status of the last batch (1
+ * absorbed, 2 escaped, 3 lost; nullable) and instance-local absorbing cell or
+ * -1 (nullable; needs flags bit 1). */
 int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
                      int32_t *hit);
 /* The particle states of an instance in the pool (after injection: the batch's initial states). */

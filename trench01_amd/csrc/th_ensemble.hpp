@@ -15,10 +15,12 @@
 //
 // Tallies: the surface slots of all instances are numbered globally (an
 // instance's slots are contiguous); absorptions are global integer atomics
-// into [2][slots] counters, exact and independent of order.  LDS
+// into [2][slots] counters, exact and independent of order, replicated so
+// that workgroups spread over several copies (summed on readback).  LDS
 // privatisation as in the single-instance push does not apply: the slots of
 // hundreds of instances do not fit in LDS, and a workgroup's lanes hold
-// particles of many instances at once.
+// particles of many instances at once.  Escapes, losses and step counts are
+// summed per instance after the push from per-particle end records.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -57,9 +59,11 @@ struct EnsembleWork {
     const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
     const int8_t *__restrict__ spc;
     int64_t begin, end;              // pool range of this launch (first pass)
-    int8_t *status;                  // nullable
-    int32_t *hit;                    // nullable: instance-local cell index or -1
-    uint32_t *counts;                // [2][slots]
+    int8_t *status;                  // per particle: end status (1 absorbed, 2 escaped, 3 lost, 4 absorbed off-surface)
+    int32_t *steps;                  // per particle: steps taken in total
+    int32_t *hit;                    // nullable: instance-local absorbing cell
+    uint32_t *counts;                // [replicas][2][slots]
+    int replicas;                    // workgroup b adds into replica b % replicas
     unsigned long long *inst_scal;   // [n_inst][kInstCount]
     unsigned long long *queue;       // this launch's work counter (zero at launch)
     int max_steps, refill, check_steps, step_cap;
@@ -78,13 +82,16 @@ struct LaneView {
     const uint8_t *__restrict__ vac;
     double dx, inv_dx, cfl_dx, two_dx;
     int nx, ny, cell0, key0;
-    double lx, ly;
     bool taper;
     __device__ __forceinline__ double4 cell(int c) const { return cells[cell0 + c]; }
     __device__ __forceinline__ const double2 *corner(int key) const { return corners + 4 * (int64_t)(key0 + key); }
     __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key0 + key]; }
     __device__ __forceinline__ bool is_vac(int c) const { return vac[cell0 + c] != 0; }
     __device__ __forceinline__ double face(int c) const { return wface[cell0 + c]; }
+    // nx*dx and ny*dx: the products the single-instance push forms on the host
+    // (recomputed per use rather than held in four more vector registers)
+    __device__ __forceinline__ double len_x() const { return (double)nx * dx; }
+    __device__ __forceinline__ double len_y() const { return (double)ny * dx; }
 };
 
 // Index of the instance holding pool index q: the last i with starts[i] <= q.
@@ -97,26 +104,46 @@ __device__ __forceinline__ int find_instance(const int64_t *__restrict__ starts,
     return lo;
 }
 
+// A particle's end is recorded at the lane's next refill, not in the step
+// loop: its status byte, its total step count and -- if absorbed on a surface
+// cell -- one global atomic into the deposit counters.  Global stores and
+// atomics inside the step loop, even on rarely taken branches, halve the push
+// rate (they share the wait counter of the loads the step waits on); at the
+// refill they overlap the refill's own queue atomic.  A lane ends at most one
+// particle between refills, so one pending record per lane suffices.  Escapes,
+// losses and steps per instance are summed afterwards by
+// ensemble_reduce_kernel from the per-particle records.
+enum : int8_t { kEndAbsorbed = 1, kEndEscaped = 2, kEndLost = 3, kEndOffSurface = 4 };
+
 template <bool BARRIER>
 __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const EnsembleWork &w) {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
+    // deposit counters of this workgroup's replica: fewer atomics meet on one address
+    uint32_t *counts = w.counts + (int64_t)(blockIdx.x % w.replicas) * 2 * g.slots;
     const int64_t q_begin = w.in ? 0 : w.begin;
     const int64_t q_end = w.in ? (int64_t)*w.in_count : w.end;
     LaneView v;
     v.cells = g.cells; v.corners = g.corners; v.corner_vac = g.corner_vac; v.wface = g.wface; v.vac = g.vac;
     v.dx = g.dx; v.inv_dx = g.inv_dx; v.cfl_dx = g.cfl_dx; v.two_dx = g.two_dx;
-    v.nx = 1; v.ny = 1; v.cell0 = 0; v.key0 = 0; v.lx = g.dx; v.ly = g.dx; v.taper = false;
+    v.nx = 1; v.ny = 1; v.cell0 = 0; v.key0 = 0; v.taper = false;
     int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
-    int inst = 0;
+    int32_t end_pid = -1, end_k = 0, end_dep = -1;   // pending end record (end_pid < 0: none)
+    int end_st = 0;
     State64 s;
-    int sp = 0, k = 0, k0 = 0;
+    int sp = 0, k = 0;
     double xp = 0.0, yp = 0.0;
     RfClock rc{0.0, 0.0, 0.0, 0.0};
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
         unsigned long long busy = __ballot(pid >= 0);
         if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
+            if (end_pid >= 0) {
+                if (end_dep >= 0) atomicAdd(&counts[end_dep], 1u);
+                w.status[end_pid] = (int8_t)end_st;
+                w.steps[end_pid] = end_k;
+                end_pid = -1;
+            }
             int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(w.queue, (unsigned long long)__popcll(idle));
@@ -134,12 +161,8 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                         s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q];
                         k = 0;
                     }
-                    k0 = k;
-                    inst = find_instance(g.starts, g.n_inst, pid);
-                    const Instance &d = g.inst[inst];
+                    const Instance &d = g.inst[find_instance(g.starts, g.n_inst, pid)];
                     v.nx = d.nx; v.ny = d.ny; v.cell0 = d.cell0; v.key0 = d.key0; v.taper = d.tapered != 0;
-                    v.lx = (double)d.nx * g.dx;   // the products the single-instance push forms on the host
-                    v.ly = (double)d.ny * g.dx;
                     s.qm = w.qm[pid];
                     s.q = div_dx(s.x, g.dx, g.inv_dx);
                     s.r = div_dx(s.y, g.dx, g.inv_dx);
@@ -157,18 +180,16 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                 int st = step64<BARRIER, true, false, false>(v, s, cell, xp, yp, rc);
                 ++k;
                 if (st != 0 || k >= w.max_steps) {
-                    st = st != 0 ? st : 3;
-                    unsigned long long *is = w.inst_scal + (int64_t)inst * kInstCount;
-                    if (w.status) w.status[pid] = (int8_t)st;
-                    if (w.hit) w.hit[pid] = st == 1 ? cell : -1;
+                    end_pid = (int32_t)pid;
+                    end_k = k;
+                    end_st = st != 0 ? st : kEndLost;
+                    end_dep = -1;
                     if (st == 1) {
                         int slot = g.slot[v.cell0 + cell];
-                        if (slot >= 0) atomicAdd(&w.counts[(int64_t)sp * g.slots + slot], 1u);
-                        else atomicAdd(&is[kInstOff], 1ull);
-                    } else {
-                        atomicAdd(&is[st == 2 ? kInstEsc0 + sp : kInstLost], 1ull);
+                        if (slot >= 0) end_dep = sp * g.slots + slot;
+                        else end_st = kEndOffSurface;
+                        if (w.hit) w.hit[pid] = cell;
                     }
-                    atomicAdd(&is[kInstSteps], (unsigned long long)(k - k0));
                     pid = -1;
                     break;
                 }
@@ -181,12 +202,53 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                     c.pid = (int32_t)pid;
                     c.k = k;
                     w.out[o] = c;
-                    atomicAdd(&w.inst_scal[(int64_t)inst * kInstCount + kInstSteps], (unsigned long long)(k - k0));
                     pid = -1;
                     break;
                 }
             }
         }
+    }
+}
+
+// A contiguous range of one instance's particles, the unit of the
+// reduction below.
+struct Chunk {
+    int64_t begin, end;
+    int32_t inst, pad;
+};
+
+// Per-instance sums of the end records of one batch, added to the
+// instance's scalar tallies: escapes per species, losses, absorptions off
+// the surface cells, steps.  One workgroup per chunk (grid-stride), so large
+// instances are summed by many workgroups.
+__global__ void __launch_bounds__(256) ensemble_reduce_kernel(const Chunk *__restrict__ chunks, int n_chunks,
+                                                               const int8_t *__restrict__ status,
+                                                               const int32_t *__restrict__ steps,
+                                                               const int8_t *__restrict__ spc,
+                                                               unsigned long long *__restrict__ inst_scal) {
+    __shared__ unsigned long long part[4][kInstCount];
+    for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        unsigned long long acc[kInstCount] = {0ull, 0ull, 0ull, 0ull, 0ull};
+        for (int64_t q = ch.begin + threadIdx.x; q < ch.end; q += blockDim.x) {
+            int st = status[q];
+            acc[kInstEsc0] += (st == kEndEscaped && spc[q] == 0) ? 1ull : 0ull;
+            acc[kInstEsc1] += (st == kEndEscaped && spc[q] != 0) ? 1ull : 0ull;
+            acc[kInstLost] += st == kEndLost ? 1ull : 0ull;
+            acc[kInstOff] += st == kEndOffSurface ? 1ull : 0ull;
+            acc[kInstSteps] += (unsigned long long)steps[q];
+        }
+        for (int f = 0; f < kInstCount; ++f)
+            for (int off = 32; off > 0; off >>= 1) acc[f] += __shfl_xor(acc[f], off, 64);
+        if ((threadIdx.x & 63) == 0)
+            for (int f = 0; f < kInstCount; ++f) part[threadIdx.x >> 6][f] = acc[f];
+        __syncthreads();
+        if (threadIdx.x < kInstCount) {
+            unsigned long long v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                   part[3][threadIdx.x];
+            if (v) atomicAdd(&inst_scal[(int64_t)ch.inst * kInstCount + threadIdx.x], v);
+        }
+        __syncthreads();
     }
 }
 

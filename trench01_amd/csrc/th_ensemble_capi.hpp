@@ -3,6 +3,11 @@
 // surface-cell rule and pass limit.
 #include "th_ensemble.hpp"
 
+namespace {
+constexpr int kCountReplicas = 16;        // copies of the deposit counters (see th_ensemble.hpp)
+constexpr int64_t kReduceChunk = 1 << 15;   // particles per workgroup of the end-record reduction
+}
+
 struct th_ensemble {
     int device = 0, cus = 0;
     int per_cu[2][2] = {{1, 1}, {1, 1}};   // resident workgroups per CU: [register budget 5 / 4][barrier]
@@ -31,15 +36,17 @@ struct th_ensemble {
     th::Instance *d_inst = nullptr;
     int64_t *d_starts = nullptr;
     th::InjectDev *d_inject = nullptr;
+    th::Chunk *d_chunks = nullptr;   // reduction ranges (kReduceChunk particles of one instance)
+    int n_chunks = 0;
     // pool and tallies
     double *d_x = nullptr, *d_y = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_qm = nullptr;
-    int8_t *d_spc = nullptr, *d_status = nullptr;
-    int32_t *d_hit = nullptr;
+    int8_t *d_spc = nullptr, *d_status = nullptr;   // status: end record per particle
+    int32_t *d_hit = nullptr, *d_steps = nullptr;
     th::Survivor *d_surv[2] = {nullptr, nullptr};
     unsigned *d_surv_count = nullptr;        // [kMaxPasses + 1]
     unsigned long long *d_queue = nullptr;   // [kMaxPasses] work counters
     unsigned long long *d_batch = nullptr;   // batch index read by the injection
-    uint32_t *d_counts = nullptr;            // [2][slots]
+    uint32_t *d_counts = nullptr;            // [kCountReplicas][2][slots]
     unsigned long long *d_scal = nullptr;    // [n_inst][kInstCount]
     // captured batch
     hipGraph_t graph = nullptr;
@@ -94,15 +101,24 @@ int ensemble_build(th_ensemble *e) {
         (rc = dev_alloc(&e->d_inject, n)) || (rc = dev_alloc(&e->d_x, pool)) || (rc = dev_alloc(&e->d_y, pool)) ||
         (rc = dev_alloc(&e->d_vx, pool)) || (rc = dev_alloc(&e->d_vy, pool)) || (rc = dev_alloc(&e->d_qm, pool)) ||
         (rc = dev_alloc(&e->d_spc, pool)) || (rc = dev_alloc(&e->d_status, pool)) ||
-        (rc = dev_alloc(&e->d_hit, pool)) || (rc = dev_alloc(&e->d_surv[0], pool)) ||
+        (rc = dev_alloc(&e->d_hit, pool)) || (rc = dev_alloc(&e->d_steps, pool)) ||
+        (rc = dev_alloc(&e->d_surv[0], pool)) ||
         (rc = dev_alloc(&e->d_surv[1], pool)) || (rc = dev_alloc(&e->d_surv_count, kMaxPasses + 1)) ||
         (rc = dev_alloc(&e->d_queue, kMaxPasses)) || (rc = dev_alloc(&e->d_batch, 1)) ||
-        (rc = dev_alloc(&e->d_counts, 2 * (size_t)std::max(e->slots, 1))) ||
+        (rc = dev_alloc(&e->d_counts, (size_t)kCountReplicas * 2 * std::max(e->slots, 1))) ||
         (rc = dev_alloc(&e->d_scal, (size_t)n * th::kInstCount)))
         return rc;
     std::vector<int64_t> starts(n + 1);
     for (int i = 0; i < n; ++i) starts[i] = e->inst[i].p0;
     starts[n] = e->n_pool;
+    std::vector<th::Chunk> chunks;
+    for (int i = 0; i < n; ++i)
+        for (int64_t b = e->inst[i].p0; b < e->inst[i].p1; b += kReduceChunk)
+            chunks.push_back(th::Chunk{b, std::min(e->inst[i].p1, b + kReduceChunk), i, 0});
+    e->n_chunks = (int)chunks.size();
+    if ((rc = dev_alloc(&e->d_chunks, std::max<size_t>(chunks.size(), 1)))) return rc;
+    if (!chunks.empty())
+        TH_HIP(hipMemcpy(e->d_chunks, chunks.data(), sizeof(th::Chunk) * chunks.size(), hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(e->d_cells, e->h_cells.data(), sizeof(double4) * e->n_cells, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(e->d_vac, e->h_vac.data(), e->n_cells, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(e->d_wface, e->h_wface.data(), sizeof(double) * e->n_cells, hipMemcpyHostToDevice));
@@ -115,6 +131,7 @@ int ensemble_build(th_ensemble *e) {
         TH_HIP(hipMemset(a, 0, sizeof(double) * (size_t)pool));
     TH_HIP(hipMemset(e->d_spc, 0, (size_t)pool));
     TH_HIP(hipMemset(e->d_status, 3, (size_t)pool));
+    TH_HIP(hipMemset(e->d_steps, 0, sizeof(int32_t) * (size_t)pool));
     TH_HIP(hipMemset(e->d_hit, 0xff, sizeof(int32_t) * (size_t)pool));
     for (int i = 0; i < n; ++i)
         if ((rc = ensemble_corners(e, i))) return rc;
@@ -217,9 +234,11 @@ int ensemble_batch(th_ensemble *e, const th_trace_params *p, uint64_t seed, int 
     th::EnsembleGrid g = ensemble_grid(e, p);
     th::EnsembleWork w;
     w.x = e->d_x; w.y = e->d_y; w.vx = e->d_vx; w.vy = e->d_vy; w.qm = e->d_qm; w.spc = e->d_spc;
-    w.status = (flags & kEnsStatus) ? e->d_status : nullptr;
+    w.status = e->d_status;
+    w.steps = e->d_steps;
     w.hit = (flags & kEnsStatus) ? e->d_hit : nullptr;
     w.counts = e->d_counts;
+    w.replicas = kCountReplicas;
     w.inst_scal = e->d_scal;
     w.queue = nullptr;
     w.max_steps = p->max_steps;
@@ -240,6 +259,11 @@ int ensemble_batch(th_ensemble *e, const th_trace_params *p, uint64_t seed, int 
         if ((rc = zero_u64(e, e->d_queue, kMaxPasses)) || (rc = zero_u32(e, e->d_surv_count, kMaxPasses + 1)) ||
             (rc = ensemble_passes(e, p, g, w, 0, e->n_pool)))
             return rc;
+    }
+    if (e->n_chunks > 0) {
+        hipLaunchKernelGGL(th::ensemble_reduce_kernel, dim3((unsigned)std::min(e->n_chunks, e->cus * 8)), dim3(256), 0,
+                           e->stream, e->d_chunks, e->n_chunks, e->d_status, e->d_steps, e->d_spc, e->d_scal);
+        TH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(th::bump_kernel, dim3(1), dim3(64), 0, e->stream, e->d_batch);
     TH_HIP(hipGetLastError());
@@ -326,7 +350,7 @@ void th_ensemble_destroy(th_ensemble *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     ensemble_free_graph(e);
-    for (void *q : {(void *)e->d_cells, (void *)e->d_corners, (void *)e->d_corner_vac, (void *)e->d_vac,
+    for (void *q : {(void *)e->d_cells, (void *)e->d_corners, (void *)e->d_corner_vac, (void *)e->d_vac, (void *)e->d_steps, (void *)e->d_chunks,
                     (void *)e->d_wface, (void *)e->d_slot, (void *)e->d_inst, (void *)e->d_starts,
                     (void *)e->d_inject, (void *)e->d_x, (void *)e->d_y, (void *)e->d_vx, (void *)e->d_vy,
                     (void *)e->d_qm, (void *)e->d_spc, (void *)e->d_status, (void *)e->d_hit, (void *)e->d_surv[0],
@@ -350,6 +374,7 @@ int th_ensemble_add(th_ensemble *e, const uint8_t *vac, int32_t nx, int32_t ny, 
     if (e->n_cells + N > INT32_MAX || e->n_keys + keys > INT32_MAX / 4)
         return fail(TH_EINVAL, "the ensemble's grids exceed 2^31 cells");
     if (e->n_pool + n_particles > INT32_MAX) return fail(TH_EINVAL, "at most 2^31-1 particles per ensemble batch");
+    if (e->slots + N > (1 << 30)) return fail(TH_EINVAL, "the ensemble's surface cells exceed 2^30");
     th::Instance d;
     d.nx = nx;
     d.ny = ny;
@@ -488,13 +513,12 @@ int th_ensemble_run(th_ensemble *e, const th_trace_params *p, uint64_t seed, uin
     if (rc) return rc;
     if (mode == kEnsGraph && (rc = ensemble_capture(e, p, seed, flags))) return rc;
     const int n = (int)e->inst.size();
-    TH_HIP(hipMemsetAsync(e->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(e->slots, 1), e->stream));
+    TH_HIP(hipMemsetAsync(e->d_counts, 0, sizeof(uint32_t) * kCountReplicas * 2 * (size_t)std::max(e->slots, 1),
+                          e->stream));
     TH_HIP(hipMemsetAsync(e->d_scal, 0, sizeof(unsigned long long) * n * th::kInstCount, e->stream));
     TH_HIP(hipMemcpyAsync(e->d_batch, &first_batch, sizeof first_batch, hipMemcpyHostToDevice, e->stream));
-    if (flags & kEnsStatus) {
-        TH_HIP(hipMemsetAsync(e->d_status, 3, (size_t)std::max<int64_t>(e->n_pool, 1), e->stream));
+    if (flags & kEnsStatus)
         TH_HIP(hipMemsetAsync(e->d_hit, 0xff, sizeof(int32_t) * (size_t)std::max<int64_t>(e->n_pool, 1), e->stream));
-    }
     TH_HIP(hipStreamSynchronize(e->stream));   // first_batch lives on this stack frame
     TH_HIP(hipEventRecord(e->ev0, e->stream));
     for (int b = 0; b < batches; ++b) {
@@ -524,18 +548,16 @@ int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt
     const int m = (int)surf.size();
     const int64_t N = (int64_t)d.nx * d.ny, np = d.p1 - d.p0;
     if (cnt_i) {
-        std::vector<uint32_t> c(2 * (size_t)std::max(m, 1));
-        if (m) {
-            TH_HIP(hipMemcpy(c.data(), e->d_counts + e->slot0[index], sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
-            TH_HIP(hipMemcpy(c.data() + m, e->d_counts + e->slots + e->slot0[index], sizeof(uint32_t) * m,
-                             hipMemcpyDeviceToHost));
-        }
         memset(cnt_i, 0, sizeof(int64_t) * (size_t)N);
         memset(cnt_e, 0, sizeof(int64_t) * (size_t)N);
-        for (int k = 0; k < m; ++k) {
-            cnt_i[surf[k]] = c[k];
-            cnt_e[surf[k]] = c[m + k];
-        }
+        std::vector<uint32_t> c((size_t)std::max(m, 1));
+        for (int r = 0; r < kCountReplicas && m; ++r)
+            for (int sp = 0; sp < 2; ++sp) {
+                const uint32_t *src = e->d_counts + ((size_t)r * 2 + sp) * e->slots + e->slot0[index];
+                TH_HIP(hipMemcpy(c.data(), src, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+                int64_t *dst = sp ? cnt_e : cnt_i;
+                for (int k = 0; k < m; ++k) dst[surf[k]] += c[k];
+            }
     }
     if (tally) {
         unsigned long long s[th::kInstCount];
@@ -547,7 +569,11 @@ int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt
         tally->steps = (int64_t)s[th::kInstSteps];
         tally->off_surface = (int64_t)s[th::kInstOff];
     }
-    if (status && np) TH_HIP(hipMemcpy(status, e->d_status + d.p0, (size_t)np, hipMemcpyDeviceToHost));
+    if (status && np) {
+        TH_HIP(hipMemcpy(status, e->d_status + d.p0, (size_t)np, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < np; ++q)
+            if (status[q] == th::kEndOffSurface) status[q] = th::kEndAbsorbed;
+    }
     if (hit && np) TH_HIP(hipMemcpy(hit, e->d_hit + d.p0, sizeof(int32_t) * (size_t)np, hipMemcpyDeviceToHost));
     return TH_OK;
 }

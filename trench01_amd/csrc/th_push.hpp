@@ -75,6 +75,8 @@ struct Grid {
     __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key]; }
     __device__ __forceinline__ bool is_vac(int c) const { return vac[c] != 0; }
     __device__ __forceinline__ double face(int c) const { return wface[c]; }
+    __device__ __forceinline__ double len_x() const { return lx; }
+    __device__ __forceinline__ double len_y() const { return ly; }
 };
 
 // Record-mode outputs (trench_trace.cu:149-183, :189-192): the end point of
@@ -280,14 +282,14 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
     const double x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
     double sx = 0.5 * (s.vx + vxn) * dt, sy = 0.5 * (s.vy + vyn) * dt;
     double xu = x0 + sx;
-    s.x = wrap_pos(xu, g.lx);
+    s.x = wrap_pos(xu, g.len_x());
     s.y = y0 + sy;
     s.vx = vxn;
     s.vy = vyn;
     s.q = div_dx(s.x, g.dx, g.inv_dx);
     s.r = div_dx(s.y, g.dx, g.inv_dx);
     // Classification of the end point.
-    bool escaped = s.y >= g.ly;
+    bool escaped = s.y >= g.len_y();
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
     bool hit = !escaped && !g.is_vac(ix * ny + iy);
     bool corner = false;
@@ -338,21 +340,21 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
             double frac;
             if (face_x) {
                 double xb = (double)(cx + (sx < 0.0)) * g.dx;
-                xb += nearbyint((x0 - xb) / g.lx) * g.lx;
+                xb += nearbyint((x0 - xb) / g.len_x()) * g.len_x();
                 frac = (xb - x0) / sx;
             } else {
                 frac = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
             }
             frac = fmin(fmax(frac, 0.0), 1.0);
             tf = frac;
-            xp = wrap_pos(x0 + frac * sx, g.lx);
+            xp = wrap_pos(x0 + frac * sx, g.len_x());
             yp = y0 + frac * sy;
         }
         if (escaped) {
-            double frac = fmin(fmax((g.ly - y0) / sy, 0.0), 1.0);
+            double frac = fmin(fmax((g.len_y() - y0) / sy, 0.0), 1.0);
             tf = frac;
-            xp = wrap_pos(x0 + frac * sx, g.lx);
-            yp = g.ly;
+            xp = wrap_pos(x0 + frac * sx, g.len_x());
+            yp = g.len_y();
         }
         if constexpr (RF) {
             rc.ft = rc.elapsed - (1.0 - tf) * dt;
