@@ -65,6 +65,9 @@ def parse():
                     help="potential update: 0 FP64 Green's matrix, 1 BF16 hi/lo split, 2 FP16 hi/lo split")
     ap.add_argument("--pass-steps", type=int, default=None,
                     help="multi-pass schedule's first cap (0: one pass); default: the library's tuned value")
+    ap.add_argument("--batching", type=int, default=256, metavar="INSTANCES",
+                    help="instances of the heterogeneous batching workload (0: skip)")
+    ap.add_argument("--batching-batches", type=int, default=5)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -175,7 +178,52 @@ def main():
         out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
     if not a.no_steady:
         out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green)
+    if a.batching > 0:
+        dev.close()
+        dev.tracer.close()
+        out["batching"] = batching(a.batching, a.batching_batches, log)
     print(json.dumps(out), flush=True)
+
+
+def batching(count, batches, log):
+    """The heterogeneous batching workload (trench01_amd.ensemble.batching_workload,
+    seed 0): ``count`` instances in one persistent launch per pass, schedule
+    picked by the autotuner, device injection per instance; timed as eager
+    launches, as a replayed hipGraph of the batch, and as one launch per
+    instance and pass (the one-geometry-per-call schedule)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from trench01_amd import autotune, ensemble as E
+    from trench01_amd.tracer import HipTracer
+    t0 = time.perf_counter()
+    specs = E.batching_workload(count, seed=0)
+    with ThreadPoolExecutor(max(1, min(16, len(os.sched_getaffinity(0))))) as ex:
+        fields = list(ex.map(E.instance_field, specs))
+    ens = E.build_ensemble(specs, fields)
+    setup_s = time.perf_counter() - t0
+    n = sum(2 * s.p.n_per_batch for s in specs)
+    cells = sum(f[0].size for f in fields)
+    prm = HipTracer.params_for(HipTracer.__new__(HipTracer), specs[0].p, True)
+    log(f"batching: {count} instances, {n} particles per batch, {cells} cells; set-up {setup_s:.1f}s; tuning")
+    best, table = autotune.tune_ensemble(ens, prm, log=log)
+    autotune.apply(prm, best)
+    ens.run(prm, seed=1, first_batch=0, batches=1, inject=True)          # warm-up
+    ms = {}
+    for mode, nb in (("one", batches), ("graph", batches), ("per_instance", max(1, batches // 2))):
+        ms[mode] = ens.run(prm, seed=1, first_batch=100, batches=nb, inject=True, mode=mode) / nb
+        log(f"  {mode}: {ms[mode]:.1f} ms per batch")
+    _, scal = ens.totals()
+    steps = int(scal[:, 3].sum())
+    nb_last = max(1, batches // 2)
+    ens.close()
+    return {"workload": f"{count} heterogeneous trench instances (width 10-40 cells, AR 2-20, mask 0-40 "
+                        "conductive or dielectric eps 3, tapers -3..+5 deg, DC or bimodal IED 50-300 eV; "
+                        "particles per batch log-uniform 1e4-1e6 per instance), charged floating fields, FP64",
+            "instances": count, "particles_per_batch": n, "cells": cells,
+            "mean_steps_per_particle": steps / (nb_last * n),
+            "schedule": best, "tuned_candidates": len(table),
+            "ms_per_batch": ms, "particles_per_s": {k: n / (v * 1e-3) for k, v in ms.items()},
+            "speedup_vs_per_instance": ms["per_instance"] / ms["one"],
+            "graph_vs_eager": ms["one"] / ms["graph"], "setup_s": setup_s}
 
 
 GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums"}

@@ -325,6 +325,11 @@ int th_ensemble_run(th_ensemble *e, const th_trace_params *p, uint64_t seed, uin
  * -1 (nullable; needs flags bit 1). */
 int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt_e, th_tally *tally, int8_t *status,
                      int32_t *hit);
+/* Tallies of all instances at once: counts [2][total surface slots] (ions,
+ * then electrons; instance i's slots follow those of instances < i, in
+ * th_ensemble_surface order) and per instance {escaped ions, escaped
+ * electrons, lost, steps, absorbed off the surface cells} (nullable each). */
+int th_ensemble_totals(th_ensemble *e, int64_t *counts, int64_t *scal);
 /* The particle states of an instance in the pool (after injection: the batch's initial states). */
 int th_ensemble_read_particles(th_ensemble *e, int32_t index, double *x, double *y, double *vx, double *vy,
                                double *qm, int8_t *spc);

@@ -112,6 +112,7 @@ SIGNATURES = {
     "th_ensemble_run": (ctypes.c_int, [vp, ctypes.POINTER(TraceParams), ctypes.c_uint64, ctypes.c_uint64,
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "th_ensemble_read": (ctypes.c_int, [vp, ctypes.c_int32, i64p, i64p, ctypes.POINTER(Tally), i8p, i32p]),
+    "th_ensemble_totals": (ctypes.c_int, [vp, i64p, i64p]),
     "th_ensemble_read_particles": (ctypes.c_int, [vp, ctypes.c_int32, dp, dp, dp, dp, dp, i8p]),
     "th_ensemble_last_ms": (ctypes.c_int, [vp, dp]),
 }

@@ -127,7 +127,10 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
     v.cells = g.cells; v.corners = g.corners; v.corner_vac = g.corner_vac; v.wface = g.wface; v.vac = g.vac;
     v.dx = g.dx; v.inv_dx = g.inv_dx; v.cfl_dx = g.cfl_dx; v.two_dx = g.two_dx;
     v.nx = 1; v.ny = 1; v.cell0 = 0; v.key0 = 0; v.taper = false;
-    int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
+    // pid >= 0: particle in flight; -1 idle; -2 no work left; <= -3: particle
+    // -3 - pid reached the pass's step cap and is written to the survivor list
+    // at the next refill (state intact meanwhile).
+    int64_t pid = -1;
     int32_t end_pid = -1, end_k = 0, end_dep = -1;   // pending end record (end_pid < 0: none)
     int end_st = 0;
     State64 s;
@@ -135,9 +138,26 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
     double xp = 0.0, yp = 0.0;
     RfClock rc{0.0, 0.0, 0.0, 0.0};
     for (;;) {
-        unsigned long long idle = __ballot(pid == -1);
+        unsigned long long idle = __ballot(pid == -1 || pid <= -3);
         unsigned long long busy = __ballot(pid >= 0);
         if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
+            unsigned long long carry = __ballot(pid <= -3);
+            if (carry != 0ull) {   // one slot reservation for the wave's survivors
+                int cl = __ffsll((long long)carry) - 1;
+                unsigned o = 0;
+                if (lane == cl) o = atomicAdd(w.out_count, (unsigned)__popcll(carry));
+                o = __shfl(o, cl);
+                if (pid <= -3) {
+                    Survivor c;
+                    c.x = s.x; c.y = s.y; c.vx = s.vx; c.vy = s.vy;
+                    c.clock = 0.0;
+                    c.elapsed = 0.0;
+                    c.pid = (int32_t)(-3 - pid);
+                    c.k = k;
+                    w.out[o + (unsigned)__popcll(carry & below)] = c;
+                    pid = -1;
+                }
+            }
             if (end_pid >= 0) {
                 if (end_dep >= 0) atomicAdd(&counts[end_dep], 1u);
                 w.status[end_pid] = (int8_t)end_st;
@@ -193,16 +213,8 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                     pid = -1;
                     break;
                 }
-                if (k >= w.step_cap) {   // carry to the next pass
-                    unsigned o = atomicAdd(w.out_count, 1u);
-                    Survivor c;
-                    c.x = s.x; c.y = s.y; c.vx = s.vx; c.vy = s.vy;
-                    c.clock = 0.0;
-                    c.elapsed = 0.0;
-                    c.pid = (int32_t)pid;
-                    c.k = k;
-                    w.out[o] = c;
-                    pid = -1;
+                if (k >= w.step_cap) {   // carry to the next pass (written at the next refill)
+                    pid = -3 - pid;
                     break;
                 }
             }

@@ -578,6 +578,28 @@ int th_ensemble_read(th_ensemble *e, int32_t index, int64_t *cnt_i, int64_t *cnt
     return TH_OK;
 }
 
+int th_ensemble_totals(th_ensemble *e, int64_t *counts, int64_t *scal) {
+    if (!e) return fail(TH_EINVAL, "ensemble is NULL");
+    if (!e->built) return fail(TH_ESTATE, "nothing traced yet");
+    TH_HIP(hipSetDevice(e->device));
+    const size_t per = 2 * (size_t)e->slots;
+    if (counts && per) {
+        std::vector<uint32_t> c(kCountReplicas * per);
+        TH_HIP(hipMemcpy(c.data(), e->d_counts, sizeof(uint32_t) * c.size(), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < per; ++j) {
+            int64_t v = 0;
+            for (int r = 0; r < kCountReplicas; ++r) v += c[r * per + j];
+            counts[j] = v;
+        }
+    }
+    if (scal) {
+        std::vector<unsigned long long> t(e->inst.size() * th::kInstCount);
+        TH_HIP(hipMemcpy(t.data(), e->d_scal, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < t.size(); ++j) scal[j] = (int64_t)t[j];
+    }
+    return TH_OK;
+}
+
 int th_ensemble_read_particles(th_ensemble *e, int32_t index, double *x, double *y, double *vx, double *vy,
                                double *qm, int8_t *spc) {
     int rc = check_index(e, index);

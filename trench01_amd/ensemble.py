@@ -126,6 +126,16 @@ class Ensemble:
             out.update(status=st, hit=hit)
         return out
 
+    def totals(self):
+        """All instances' tallies: counts [2, slots] (instance-major surface
+        slots) and scalars [n_instances, 5] (escaped ions, escaped electrons,
+        lost, steps, off-surface absorptions)."""
+        slots = sum(self.surface(i).size for i in range(len(self.shapes)))
+        counts = np.empty((2, slots), dtype=np.int64)
+        scal = np.empty((len(self.shapes), 5), dtype=np.int64)
+        _lib.check(_lib.load().th_ensemble_totals(self._h, _lib.ptr(counts, _lib.i64p), _lib.ptr(scal, _lib.i64p)))
+        return counts, scal
+
     def read_particles(self, index):
         n = self.sizes[index]
         arrs = [np.empty(n) for _ in range(5)]
