@@ -485,11 +485,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     const unsigned long long below = (1ull << lane) - 1ull;
     const int64_t q_begin = w.in ? 0 : w.begin;
     const int64_t q_end = w.in ? (int64_t)*w.in_count : w.end;
-    // pid >= 0: particle in flight; -1 idle; -2 no work left; <= -3: particle
-    // -3 - pid reached the pass's step cap and waits (state intact) to be
-    // written to the survivor list at the next refill -- global stores and
-    // atomics inside the step loop would make its load waits conservative.
-    int64_t pid = -1;
+    int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
     StateT<Real> s;
     int sp = 0, k = 0, k0 = 0;    // step count, and its value when this pass took the particle
     int slot = -1, plen = 0;      // record mode: probe slot and path length
@@ -497,26 +493,9 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     RfClock rc{0.0, 0.0, 0.0, 0.0};
     unsigned long long my_steps = 0;
     for (;;) {
-        unsigned long long idle = __ballot(pid == -1 || pid <= -3);
+        unsigned long long idle = __ballot(pid == -1);
         unsigned long long busy = __ballot(pid >= 0);
         if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
-            unsigned long long carry = __ballot(pid <= -3);
-            if (carry != 0ull) {   // one slot reservation for the wave's survivors
-                int cl = __ffsll((long long)carry) - 1;
-                unsigned o = 0;
-                if (lane == cl) o = atomicAdd(w.out_count, (unsigned)__popcll(carry));
-                o = __shfl(o, cl);
-                if (pid <= -3) {
-                    Survivor v;
-                    v.x = (double)s.x; v.y = (double)s.y; v.vx = (double)s.vx; v.vy = (double)s.vy;
-                    v.clock = rc.clock;
-                    v.elapsed = rc.elapsed;
-                    v.pid = (int32_t)(-3 - pid);
-                    v.k = k;
-                    w.out[o + (unsigned)__popcll(carry & below)] = v;
-                    pid = -1;
-                }
-            }
             int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(&w.scal[kScalQueue], (unsigned long long)__popcll(idle));
@@ -612,9 +591,17 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                     pid = -1;
                     break;
                 }
-                if (k >= w.step_cap) {   // carry to the next pass (written at the next refill)
+                if (k >= w.step_cap) {   // carry to the next pass
+                    unsigned slot_out = atomicAdd(w.out_count, 1u);
+                    Survivor v;
+                    v.x = (double)s.x; v.y = (double)s.y; v.vx = (double)s.vx; v.vy = (double)s.vy;
+                    v.clock = rc.clock;
+                    v.elapsed = rc.elapsed;
+                    v.pid = (int32_t)pid;
+                    v.k = k;
+                    w.out[slot_out] = v;
                     my_steps += (unsigned long long)(k - k0);
-                    pid = -3 - pid;
+                    pid = -1;
                     break;
                 }
             }
