@@ -47,6 +47,7 @@ CPU_SAMPLE_S = 10.0         # minimum CPU work behind the cpu_baseline figure
 # scaled by the copy-kernel calibration); the command that made it is stored
 # with it.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "push64_ar8_1e6_traffic.json")
+ENSEMBLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ensemble_push_256_traffic.json")
 
 
 def parse():
@@ -68,6 +69,8 @@ def parse():
     ap.add_argument("--batching", type=int, default=256, metavar="INSTANCES",
                     help="instances of the heterogeneous batching workload (0: skip)")
     ap.add_argument("--batching-batches", type=int, default=5)
+    ap.add_argument("--batching-schedule", default=None, metavar="JSON",
+                    help="ensemble schedule instead of the autotuner's pick, e.g. '{\"pass_steps\": 0}'")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -181,16 +184,18 @@ def main():
     if a.batching > 0:
         dev.close()
         dev.tracer.close()
-        out["batching"] = batching(a.batching, a.batching_batches, log)
+        out["batching"] = batching(a.batching, a.batching_batches, log,
+                                   json.loads(a.batching_schedule) if a.batching_schedule else None)
     print(json.dumps(out), flush=True)
 
 
-def batching(count, batches, log):
+def batching(count, batches, log, schedule=None):
     """The heterogeneous batching workload (trench01_amd.ensemble.batching_workload,
     seed 0): ``count`` instances in one persistent launch per pass, schedule
     picked by the autotuner, device injection per instance; timed as eager
     launches, as a replayed hipGraph of the batch, and as one launch per
-    instance and pass (the one-geometry-per-call schedule)."""
+    instance and pass (the one-geometry-per-call schedule).  ``schedule``
+    (a dict of schedule fields) replaces the autotuner's pick."""
     from concurrent.futures import ThreadPoolExecutor
     from trench01_amd import autotune, ensemble as E
     from trench01_amd.tracer import HipTracer
@@ -204,7 +209,10 @@ def batching(count, batches, log):
     cells = sum(f[0].size for f in fields)
     prm = HipTracer.params_for(HipTracer.__new__(HipTracer), specs[0].p, True)
     log(f"batching: {count} instances, {n} particles per batch, {cells} cells; set-up {setup_s:.1f}s; tuning")
-    best, table = autotune.tune_ensemble(ens, prm, log=log)
+    if schedule is None:
+        best, table = autotune.tune_ensemble(ens, prm, log=log)
+    else:
+        best, table = dict(schedule), []
     autotune.apply(prm, best)
     ens.run(prm, seed=1, first_batch=0, batches=1, inject=True)          # warm-up
     ms = {}
@@ -215,6 +223,14 @@ def batching(count, batches, log):
     steps = int(scal[:, 3].sum())
     nb_last = max(1, batches // 2)
     ens.close()
+    steps_per_batch = steps / nb_last
+    achieved = FLOP_PER_STEP * steps_per_batch / (ms["one"] * 1e-3) / 1e12
+    try:
+        with open(ENSEMBLE_TRAFFIC_FILE) as f:
+            t = json.load(f)
+        traffic = t["fetch_bytes_per_batch"] + t["write_bytes_per_batch"] if count == 256 else None
+    except (OSError, ValueError, KeyError):
+        traffic = None
     return {"workload": f"{count} heterogeneous trench instances (width 10-40 cells, AR 2-20, mask 0-40 "
                         "conductive or dielectric eps 3, tapers -3..+5 deg, DC or bimodal IED 50-300 eV; "
                         "particles per batch log-uniform 1e4-1e6 per instance), charged floating fields, FP64",
@@ -223,7 +239,13 @@ def batching(count, batches, log):
             "schedule": best, "tuned_candidates": len(table),
             "ms_per_batch": ms, "particles_per_s": {k: n / (v * 1e-3) for k, v in ms.items()},
             "speedup_vs_per_instance": ms["per_instance"] / ms["one"],
-            "graph_vs_eager": ms["one"] / ms["graph"], "setup_s": setup_s}
+            "graph_vs_eager": ms["one"] / ms["graph"], "setup_s": setup_s,
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                         "traffic_source": os.path.relpath(ENSEMBLE_TRAFFIC_FILE, ROOT) if traffic else None,
+                         "kernel": "ensemble_push_kernel",
+                         "note": "FP64 FLOP at 69 per particle-step over the eager batch time (one push launch plus "
+                                 "injection, counter clears and the end-record reduction, about 1% of it)"}}
 
 
 GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums"}
