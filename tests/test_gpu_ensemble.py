@@ -165,3 +165,31 @@ def test_argument_errors(workload):
     with pytest.raises(ValueError):
         ens.run(prm, mode="graph", status=True)
     ens.close()
+
+
+def test_autotuner_and_bulk_totals(workload):
+    """The ensemble autotuner times candidate schedules on one injected batch
+    and checks that they agree; the bulk readback equals the per-instance one."""
+    _case("ensemble autotune")
+    from trench01_amd import autotune
+    specs, fields, parts = workload
+    ens = build(specs, fields, parts)
+    for i, s in enumerate(specs):
+        ens.set_inject(i, s.p)
+    prm = params(specs[0].p)
+    best, table = autotune.tune_ensemble(ens, prm, space=dict(pass_steps=(0, 120), refill=(16,), check_steps=(4, 8),
+                                                              blocks_per_cu=(0, 4)))
+    assert len(table) == 8 and best in [t[0] for t in table]
+    counts, scal = ens.totals()
+    off = 0
+    for i in range(len(specs)):
+        r = ens.read(i)
+        flat = ens.surface(i)
+        m = flat.size
+        np.testing.assert_array_equal(counts[0, off:off + m], r["cnt_i"].ravel()[flat])
+        np.testing.assert_array_equal(counts[1, off:off + m], r["cnt_e"].ravel()[flat])
+        np.testing.assert_array_equal(scal[i], [r["escaped"][0], r["escaped"][1], r["lost"], r["steps"],
+                                                r["off_surface"]])
+        off += m
+    assert off == counts.shape[1]
+    ens.close()

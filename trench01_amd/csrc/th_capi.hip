@@ -250,7 +250,8 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
     int rc = TH_OK;
     auto bad = [&](int code) { th_tracer_destroy(t); return code; };
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bad(fail(TH_EHIP, "hipGetDeviceProperties failed"));
-    snprintf(t->name, sizeof t->name, "%s", prop.name);
+    // the marketing name can be empty on some drivers; the ISA name is always set
+    snprintf(t->name, sizeof t->name, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     t->cus = prop.multiProcessorCount;
     if (2 * (size_t)t->m * sizeof(uint32_t) > 96 * 1024)
         return bad(fail(TH_EINVAL, "%d surface cells exceed the LDS tally capacity", t->m));
