@@ -108,3 +108,26 @@ def test_patience_stops_a_noisy_map():
 
     steady.anderson_iterate(g, np.zeros(n), [], tol=1e-12, memory=5, max_evals=200, patience=6)
     assert len(calls) < 60
+
+
+def test_gmres_solves_a_small_system_in_n_steps():
+    rng = np.random.default_rng(8)
+    n = 9
+    A = rng.normal(size=(n, n)) + 5 * np.eye(n)
+    b = rng.normal(size=n)
+    x, k = steady.gmres(lambda v: A @ v, b, krylov=n)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    assert k == n
+
+
+def test_newton_krylov_solves_an_affine_map_in_one_newton_step():
+    """With exact differences (affine map) and a Krylov space of full
+    dimension, one Newton step lands on the fixed point."""
+    n = 15
+    A, b, xs = affine(n, 0.99, 6)
+    res = []
+    _, gx, evals = steady.newton_krylov_iterate(lambda x: (None, A @ x + b), np.zeros(n), tol=1e-8, h=1.0,
+                                                krylov=n, max_newton=3, log=lambda k, r, e: res.append(r))
+    np.testing.assert_allclose(gx, xs, atol=1e-8)
+    assert len(res) == 2 and res[1] < 1e-8          # converged after one Newton step
+    assert evals == 1 + n + 1                        # no backtracking on an affine map

@@ -129,6 +129,85 @@ def potential_map(setup):
     return to_u, to_rho
 
 
+class _Session:
+    """One device-resident charging state for a steady solve: the batch map
+    with common random numbers, the potential coordinates, and the plain
+    phase with the reference criterion."""
+
+    def __init__(self, p, device, setup, push_precision, green_precision):
+        self.t_start = time.perf_counter()
+        self.p = p
+        self.s = s = setup or ChargingSetup(p)
+        self.dev = dev = DeviceCharging(s, device=device)
+        dev.set_schedule(precision=push_precision)
+        if green_precision:
+            dev.set_green_precision(green_precision)
+        dev.set_monitors(s.probes, s.regions)
+        self.n = 2 * p.n_per_batch
+        dev.reserve(self.n)
+        self.to_u, self.to_rho = potential_map(s)
+        self.setup_s = time.perf_counter() - self.t_start
+        self.traced = 0
+
+    def evaluate(self, rho, first, K, R):
+        """The batch map from charge ``rho``: R replicas of K batches (device
+        injection, batch indices first .. first + R*K - 1), surface potentials
+        averaged over the replicas.  Returns (charge of the last replica,
+        mean surface potentials)."""
+        p, dev = self.p, self.dev
+        us = []
+        for r in range(R):
+            dev.set_rho(rho)
+            for b in range(K):
+                dev.inject_device(0, p.n_per_batch, p.seed, first + r * K + b)
+                dev.step(0, self.n)
+            self.traced += K
+            rho_g = dev.read(phi=False, counts=False)[0]
+            us.append(self.to_u(rho_g))
+        return rho_g, np.mean(us, axis=0)
+
+    def plain_phase(self, rho, log):
+        """Plain batches with fresh particles from ``rho`` until the reference
+        criterion passes (trench_charging_2d.py:1299-1316); the steady values
+        are the last-window means (:1342-1343)."""
+        p, dev, s = self.p, self.dev, self.s
+        probes = s.probes
+        dev.set_rho(rho)
+        hist = {k: [] for k in probes}
+        qhist = {k: [] for k in (*s.regions, "surface absolute")}
+        W = p.steady_window
+        n_ok, done, b2 = 0, False, 0
+        first = 1 << 20                              # batch indices away from the solver's
+        while self.traced < p.max_batches:
+            dev.inject_device(0, p.n_per_batch, p.seed, first + b2)
+            d = dev.step(0, self.n)
+            if d.tally.off_surface:
+                raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
+            b2 += 1
+            self.traced += 1
+            values, charges, _, _ = dev.monitors()
+            for k in probes:
+                hist[k].append(values[k])
+            for k, q in charges.items():
+                qhist[k].append(q)
+            if b2 % W == 0 and b2 >= 2 * W:
+                v_ok, name, dv, tol = monitors.check_steady(hist, probes, p)
+                q_ok = monitors.check_charge_steady(qhist, p)["ok"]
+                n_ok = min(n_ok + 1, p.steady_hold) if (v_ok and q_ok) else 0
+                log(f"  [plain] batch {b2}: voltage {'OK' if v_ok else 'NG'} ({name} {dv:+.3g} V, tol {tol:.3g}); "
+                    f"charge {'OK' if q_ok else 'NG'} -> {n_ok}/{p.steady_hold}")
+                if n_ok >= p.steady_hold:
+                    done = True
+                    break
+        rho_s, _, _ = dev.read(phi=False, counts=False)
+        Wl = min(W, len(hist[next(iter(probes))]))
+        steady = {k: float(np.mean(hist[k][-Wl:])) for k in probes}
+        return done, b2, steady, {k: np.array(v) for k, v in hist.items()}, rho_s
+
+    def close(self):
+        self.dev.close()
+
+
 def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None,
              push_precision=0, green_precision=0):
     """Steady state by multilevel Anderson acceleration of the batch map with
@@ -142,92 +221,147 @@ def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log
     last-window mean), hist (probe histories of the plain phase), rho_s,
     wall_s and setup_s."""
     log = log or (lambda s: None)
-    t_start = time.perf_counter()
-    s = setup or ChargingSetup(p)
-    dev = DeviceCharging(s, device=device)
-    dev.set_schedule(precision=push_precision)
-    if green_precision:
-        dev.set_green_precision(green_precision)
-    probes = s.probes
-    dev.set_monitors(probes, s.regions)
-    n = 2 * p.n_per_batch
-    dev.reserve(n)
-    to_u, to_rho = potential_map(s)
-    setup_s = time.perf_counter() - t_start
-    traced = 0
-
-    def evaluate(rho, first, K, R):
-        nonlocal traced
-        us = []
-        for r in range(R):
-            dev.set_rho(rho)
-            for b in range(K):
-                dev.inject_device(0, p.n_per_batch, p.seed, first + r * K + b)
-                dev.step(0, n)
-            traced += K
-            rho_g = dev.read(phi=False, counts=False)[0]
-            us.append(to_u(rho_g))
-        return rho_g, np.mean(us, axis=0)
-
+    ss = _Session(p, device, setup, push_precision, green_precision)
     pairs = []
-    rho = np.zeros(dev.m)
-    x = to_u(rho)
+    rho = np.zeros(ss.dev.m)
+    x = ss.to_u(rho)
     evals_per_stage = []
     first = 0
     for K, R, tol in stages:
         box = {"like": rho}
 
         def g(x, first=first, K=K, R=R):
-            rho_in = to_rho(x, box["like"])
-            rho_g, gx = evaluate(rho_in, first, K, R)
+            rho_in = ss.to_rho(x, box["like"])
+            rho_g, gx = ss.evaluate(rho_in, first, K, R)
             box["like"] = rho_g
             return rho_g, gx
 
         rho_g, x, e = anderson_iterate(
             g, x, pairs, tol, memory, max_evals, patience,
             log=lambda k, res, K=K, R=R: log(f"  [anderson K={K} R={R}] eval {k}: max |g(x) - x| = {res:.4g} V"))
-        rho = to_rho(x, rho_g)
+        rho = ss.to_rho(x, rho_g)
         evals_per_stage.append(e)
         first += K * R
-    secant_batches = traced
-    # plain batches with fresh particles until the reference criterion passes
-    dev.set_rho(rho)
-    hist = {k: [] for k in probes}
-    qhist = {k: [] for k in (*s.regions, "surface absolute")}
-    W = p.steady_window
-    n_ok, done, b2 = 0, False, 0
-    first = 1 << 20                              # batch indices away from the secant phase's
-    while traced < p.max_batches:
-        dev.inject_device(0, p.n_per_batch, p.seed, first + b2)
-        d = dev.step(0, n)
-        if d.tally.off_surface:
-            raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
-        b2 += 1
-        traced += 1
-        values, charges, _, _ = dev.monitors()
-        for k in probes:
-            hist[k].append(values[k])
-        for k, q in charges.items():
-            qhist[k].append(q)
-        if b2 % W == 0 and b2 >= 2 * W:
-            v_ok, name, dv, tol = monitors.check_steady(hist, probes, p)
-            q_ok = monitors.check_charge_steady(qhist, p)["ok"]
-            n_ok = min(n_ok + 1, p.steady_hold) if (v_ok and q_ok) else 0
-            log(f"  [plain] batch {b2}: voltage {'OK' if v_ok else 'NG'} ({name} {dv:+.3g} V, tol {tol:.3g}); "
-                f"charge {'OK' if q_ok else 'NG'} -> {n_ok}/{p.steady_hold}")
-            if n_ok >= p.steady_hold:
-                done = True
-                break
-    rho_s, _, _ = dev.read(phi=False, counts=False)
-    wall = time.perf_counter() - t_start
-    Wl = min(W, len(hist[next(iter(probes))]))
-    steady = {k: float(np.mean(hist[k][-Wl:])) for k in probes}
-    dev.close()
+    secant_batches = ss.traced
+    done, b2, steady, hist, rho_s = ss.plain_phase(rho, log)
+    wall = time.perf_counter() - ss.t_start
+    ss.close()
     return dict(method="multilevel Anderson acceleration of the batch map (common random numbers), "
                        "then plain batches to the reference criterion",
-                converged=done, batches=traced,
+                converged=done, batches=ss.traced,
                 phases=[dict(kind="secant", stages=[list(st) for st in stages], memory=memory,
                              evaluations_per_stage=evals_per_stage, batches=secant_batches),
                         dict(kind="plain", batches=b2, converged=done)],
-                steady=steady, hist={k: np.array(v) for k, v in hist.items()}, rho_s=rho_s, wall_s=wall,
-                setup_s=setup_s)
+                steady=steady, hist=hist, rho_s=rho_s, wall_s=wall, setup_s=ss.setup_s)
+
+
+def gmres(matvec, b, krylov):
+    """``krylov`` steps of GMRES from x0 = 0 (Arnoldi with modified
+    Gram-Schmidt, least squares on the Hessenberg matrix; Saad & Schultz,
+    SIAM J. Sci. Stat. Comput. 7(3), 1986).  Stops early when the Krylov space
+    is exhausted.  Returns (x, matvecs used)."""
+    n = b.size
+    beta = float(np.linalg.norm(b))
+    if beta == 0.0:
+        return np.zeros(n), 0
+    V = np.zeros((krylov + 1, n))
+    H = np.zeros((krylov + 1, krylov))
+    V[0] = b / beta
+    k = 0
+    for j in range(krylov):
+        w = matvec(V[j])
+        k += 1
+        for i in range(j + 1):
+            H[i, j] = V[i] @ w
+            w = w - H[i, j] * V[i]
+        H[j + 1, j] = float(np.linalg.norm(w))
+        if H[j + 1, j] <= 1e-14 * beta:
+            j += 1
+            break
+        V[j + 1] = w / H[j + 1, j]
+    else:
+        j = krylov
+    e1 = np.zeros(j + 1)
+    e1[0] = beta
+    y = np.linalg.lstsq(H[:j + 1, :j], e1, rcond=None)[0]
+    return V[:j].T @ y, k
+
+
+def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, log=None):
+    """Jacobian-free Newton-Krylov on F(x) = g(x) - x: every Newton step
+    solves (J_g - I) dx = -F(x) by ``krylov`` GMRES steps whose Jacobian-vector
+    products are forward differences of g with step ``h`` along the unit
+    direction (J_g v ~ |v| (g(x + h v/|v|) - g(x)) / h; g sees the same
+    particles at every evaluation: common random numbers).  A step that does
+    not lower max |F| is halved up to ``backtrack`` times (then the plain step
+    x <- g(x) is taken).  Stops when max |F| < tol or after ``max_newton``
+    steps.  ``g(x)`` returns (state, image).  Returns (state at the last
+    evaluation, its image, evaluations)."""
+    state, gx = g(x)
+    evals = 1
+    res = float(np.max(np.abs(gx - x)))
+    for it in range(max_newton):
+        if log:
+            log(it, res, evals)
+        if res < tol:
+            break
+        f = gx - x
+
+        def matvec(v, x=x, gx=gx):
+            nrm = float(np.linalg.norm(v))
+            if nrm == 0.0:
+                return np.zeros_like(v)
+            _, gp = g(x + (h / nrm) * v)
+            return (gp - gx) * (nrm / h) - v
+
+        dx, k = gmres(matvec, -f, krylov)
+        evals += k
+        lam = 1.0
+        for _ in range(backtrack + 1):
+            xt = x + lam * dx
+            st, gt = g(xt)
+            evals += 1
+            rt = float(np.max(np.abs(gt - xt)))
+            if rt < res:
+                break
+            lam *= 0.5
+        else:                                   # no descent along dx: plain step
+            xt, st, gt = gx, *g(gx)
+            evals += 1
+            rt = float(np.max(np.abs(gt - xt)))
+        x, state, gx, res = xt, st, gt, rt
+    return state, gx, evals
+
+
+def newton_krylov(p, K=1, R=4, h_volts=0.5, krylov=30, max_newton=3, tol=0.003, log=None, device=0, setup=None):
+    """Steady state by Jacobian-free Newton-Krylov on the batch map (GMRES,
+    ``krylov`` steps per Newton step, JVPs by forward differences of step
+    ``h_volts`` on the surface potentials, R replicas of K batches per map
+    evaluation with common random numbers), then plain batches to the
+    reference criterion.  Same return dict as ``anderson``."""
+    log = log or (lambda s: None)
+    ss = _Session(p, device, setup, 0, 0)
+    rho = np.zeros(ss.dev.m)
+    box = {"like": rho}
+
+    def g(x):
+        rho_in = ss.to_rho(x, box["like"])
+        rho_g, gx = ss.evaluate(rho_in, 0, K, R)
+        box["like"] = rho_g
+        return rho_g, gx
+
+    rho_g, x, evals = newton_krylov_iterate(
+        g, ss.to_u(rho), tol, h_volts, krylov, max_newton,
+        log=lambda k, res, e: log(f"  [newton-krylov] step {k}: max |g(x) - x| = {res:.4g} V after {e} evaluations"))
+    rho = ss.to_rho(x, rho_g)
+    solver_batches = ss.traced
+    done, b2, steady, hist, rho_s = ss.plain_phase(rho, log)
+    wall = time.perf_counter() - ss.t_start
+    ss.close()
+    return dict(method="Jacobian-free Newton-Krylov (GMRES) on the batch map (common random numbers, forward-"
+                       "difference JVPs), then plain batches to the reference criterion",
+                converged=done, batches=ss.traced,
+                phases=[dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov,
+                             evaluations=evals, batches=solver_batches),
+                        dict(kind="plain", batches=b2, converged=done)],
+                steady=steady, hist=hist, rho_s=rho_s, wall_s=wall, setup_s=ss.setup_s)
