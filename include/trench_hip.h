@@ -261,6 +261,14 @@ int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *pr
                              int32_t n_bottom, const int32_t *bottom_s);
 int th_charging_monitors(const th_charging *c, double *out);
 
+/* Adjoint of the potential update: out[n_cols] = G^T w, the gradient of
+ * sum over cells of w[cell] * phi[cell] (w on the nx*ny grid; only the kept
+ * rows enter) with respect to the charge of each Green's column [V per C/m].
+ * No reference counterpart (the reference has no derivatives); the transpose
+ * of the G Q product behind phi = solve(-rho/EPS0) + external
+ * (trench_charging_2d.py:1268, :795-797). */
+int th_charging_green_adjoint(th_charging *c, const double *w, double *out);
+
 /* Reads the state: rho (m), potential on the grid (nx*ny; cells outside the
  * kept rows read 0), and the last batch's per-surface-cell counts (2m). */
 int th_charging_read(th_charging *c, double *rho_s, double *phi /* nullable */, int32_t *counts /* nullable */);

@@ -214,6 +214,16 @@ class DeviceCharging:
             self.tracer.reserve(n)
             self.capacity = n
 
+    def potential_adjoint(self, w):
+        """G^T w: the gradient of sum(w * phi) over the grid with respect to the
+        charge of each Green's column [V per C/m] (th_charging_green_adjoint)."""
+        w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+        if w.size != self.setup.geo.solid.size:
+            raise ValueError("weights must cover the grid")
+        out = np.empty(self.setup.green.shape[1])
+        _lib.check(_lib.load().th_charging_green_adjoint(self._h, _lib.ptr(w, _lib.dp), _lib.ptr(out, _lib.dp)))
+        return out
+
     def inject_device(self, offset, n_per_species, seed, batch):
         p = self.setup.p
         spec = ion_spec(p)

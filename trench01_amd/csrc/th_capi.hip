@@ -952,6 +952,33 @@ int th_charging_read(th_charging *c, double *rho_s, double *phi, int32_t *counts
     return TH_OK;
 }
 
+int th_charging_green_adjoint(th_charging *c, const double *w, double *out) {
+    if (!c || !w || !out) return fail(TH_EINVAL, "invalid argument");
+    th_tracer *t = c->t;
+    TH_HIP(hipSetDevice(t->device));
+    const int rows_per_block = 64;
+    const int blocks = std::max(1, (c->d.n_rows + rows_per_block - 1) / rows_per_block);
+    double *buf = nullptr;   // [N] weights, [blocks][n_cols] partial sums, [n_cols] result
+    const size_t n_buf = (size_t)t->N + (size_t)blocks * c->d.n_cols + c->d.n_cols;
+    int rc = dev_alloc(&buf, n_buf);
+    if (rc) return rc;
+    double *dw = buf, *partial = buf + t->N, *dout = partial + (size_t)blocks * c->d.n_cols;
+    hipError_t e = hipMemcpyAsync(dw, w, sizeof(double) * t->N, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(th::green_adjoint_partial_kernel, dim3(blocks), dim3(256), 0, t->stream, c->d, dw,
+                           rows_per_block, partial);
+        hipLaunchKernelGGL(th::green_adjoint_sum_kernel, dim3((c->d.n_cols + 255) / 256), dim3(256), 0, t->stream,
+                           partial, blocks, c->d.n_cols, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * c->d.n_cols, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) rc = fail(TH_EHIP, "green adjoint: %s", hipGetErrorString(e));
+    (void)hipFree(buf);
+    return rc;
+}
+
 int th_inject(th_tracer *t, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t batch) {
     if (!t || !ip || offset < 0 || ip->n < 0) return fail(TH_EINVAL, "invalid argument");
     if (ip->ion_kind < 0 || ip->ion_kind > 2) return fail(TH_EINVAL, "unknown ion distribution %d", ip->ion_kind);

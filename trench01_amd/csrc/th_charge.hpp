@@ -183,6 +183,30 @@ __global__ void __launch_bounds__(256) green_gemv_kernel(ChargeDev d) {
     }
 }
 
+// Adjoint of the potential update phi = G Q + phi_ext: the gradient of
+// sum_cells w * phi with respect to the column charges, G^T w over the kept
+// rows.  Workgroup b sums rows [b R, (b+1) R) for every column into
+// partial[b][c] (coalesced: consecutive threads read consecutive columns of a
+// row); green_adjoint_sum_kernel adds the slices in order, so the result does
+// not depend on the launch.
+__global__ void __launch_bounds__(256) green_adjoint_partial_kernel(ChargeDev d, const double *__restrict__ w,
+                                                                     int rows_per_block, double *partial) {
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = min(r0 + rows_per_block, d.n_rows);
+    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) {
+        double acc = 0.0;
+        for (int r = r0; r < r1; ++r) acc += d.green[(int64_t)r * d.n_cols + c] * w[d.rows[r]];
+        partial[(int64_t)blockIdx.x * d.n_cols + c] = acc;
+    }
+}
+__global__ void green_adjoint_sum_kernel(const double *__restrict__ partial, int blocks, int n_cols, double *out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cols) return;
+    double acc = 0.0;
+    for (int b = 0; b < blocks; ++b) acc += partial[(int64_t)b * n_cols + c];
+    out[c] = acc;
+}
+
 enum : int { kGreenF64 = 0, kGreenBF16 = 1, kGreenF16 = 2 };
 
 __device__ __forceinline__ float bf16_to_float(uint32_t h) { return __uint_as_float(h << 16); }
