@@ -221,8 +221,12 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
     if (__builtin_expect((unsigned)(i + 1) <= (unsigned)nx && (unsigned)(j + 1) <= (unsigned)ny, 1)) {
         // one 64-byte corner record and one flag byte per step
         const int key = (i + 1) * (ny + 1) + (j + 1);
-        const double2 *c = g.corner(key);
-        e00 = c[0]; e10 = c[1]; e01 = c[2]; e11 = c[3];
+        // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
+        typedef double dvec4 __attribute__((ext_vector_type(4)));
+        const dvec4 *c = (const dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
+        const dvec4 a = c[0], b = c[1];
+        e00 = make_double2(a.x, a.y); e10 = make_double2(a.z, a.w);
+        e01 = make_double2(b.x, b.y); e11 = make_double2(b.z, b.w);
         const unsigned f = g.corner_flags(key);
         v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
         v01 = (f & 4u) ? 1.0 : 0.0; v11 = (f & 8u) ? 1.0 : 0.0;
