@@ -131,3 +131,22 @@ def test_circuit_rf_resume_equals_continuous_run(tmp_path):
     shifted = charging.run(replace(p, n_batches=5, rf_phase=p.rf_phase + 90), log=None, resume=loaded)
     np.testing.assert_array_equal(shifted["rho_raw"], full["rho_raw"])
     assert np.max(np.abs(shifted["phi"] - full["phi"])) > 1
+
+
+@pytest.mark.parametrize("name", RUN_CASES)
+def test_circuit_run_on_device_against_the_reference_run(name):
+    """The device run against the reference's own run() in the circuit field
+    modes (tests/golden/runrf_*.npz, same seed and waveform): every
+    count-derived history identical, probe potentials and the final charge
+    within 1e-12 of their range.  (circuit_rf: the RF bound's rounding that
+    the reference leaves to its BLAS does not change any particle's cell in
+    these runs.)"""
+    _case(f"circuit run vs reference {name}")
+    g = load(f"runrf_{name}")
+    res = charging.run(product_params_from(params_of(g)), log=None, circuit=circuit_of(g))
+    for k in ("ratio_i_bottom", "ratio_e_bottom", "esc_e", "lost", "t"):
+        np.testing.assert_array_equal(res["hist"][k], g[f"hist_{k}"], err_msg=k)
+    for k in res["probes"]:
+        b = g[f"hist_{k}"]
+        np.testing.assert_allclose(res["hist"][k], b, rtol=0, atol=1e-12 * max(1.0, np.abs(b).max()), err_msg=k)
+    np.testing.assert_allclose(res["rho_raw"], g["rho_raw"], rtol=0, atol=1e-12 * np.abs(g["rho_raw"]).max())
