@@ -95,7 +95,7 @@ struct LaneView {
 };
 
 // Index of the instance holding pool index q: the last i with starts[i] <= q.
-__device__ __forceinline__ int find_instance(const int64_t *__restrict__ starts, int n_inst, int64_t q) {
+__device__ __forceinline__ int find_instance(const int64_t *starts, int n_inst, int64_t q) {
     int lo = 0, hi = n_inst - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -115,8 +115,17 @@ __device__ __forceinline__ int find_instance(const int64_t *__restrict__ starts,
 // ensemble_reduce_kernel from the per-particle records.
 enum : int8_t { kEndAbsorbed = 1, kEndEscaped = 2, kEndLost = 3, kEndOffSurface = 4 };
 
+constexpr int kLdsStarts = 1024;   // instances whose pool starts are searched in LDS
+
 template <bool BARRIER>
 __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const EnsembleWork &w) {
+    // pool starts of the instances in LDS for the refill's binary search
+    __shared__ int64_t lds_starts[kLdsStarts + 1];
+    const bool starts_in_lds = g.n_inst <= kLdsStarts;
+    if (starts_in_lds)
+        for (int i = threadIdx.x; i <= g.n_inst; i += blockDim.x) lds_starts[i] = g.starts[i];
+    __syncthreads();
+    const int64_t *starts = starts_in_lds ? lds_starts : g.starts;
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
     // deposit counters of this workgroup's replica: fewer atomics meet on one address
@@ -181,7 +190,7 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                         s.x = w.x[q]; s.y = w.y[q]; s.vx = w.vx[q]; s.vy = w.vy[q];
                         k = 0;
                     }
-                    const Instance &d = g.inst[find_instance(g.starts, g.n_inst, pid)];
+                    const Instance &d = g.inst[find_instance(starts, g.n_inst, pid)];
                     v.nx = d.nx; v.ny = d.ny; v.cell0 = d.cell0; v.key0 = d.key0; v.taper = d.tapered != 0;
                     s.qm = w.qm[pid];
                     s.q = div_dx(s.x, g.dx, g.inv_dx);
