@@ -88,8 +88,30 @@ int ensemble_corners(th_ensemble *e, int index) {
     return TH_OK;
 }
 
+void ensemble_free_device(th_ensemble *e) {
+    for (void **q : {(void **)&e->d_cells, (void **)&e->d_corners, (void **)&e->d_corner_vac, (void **)&e->d_vac,
+                     (void **)&e->d_steps, (void **)&e->d_chunks, (void **)&e->d_wface, (void **)&e->d_slot,
+                     (void **)&e->d_inst, (void **)&e->d_starts, (void **)&e->d_inject, (void **)&e->d_x,
+                     (void **)&e->d_y, (void **)&e->d_vx, (void **)&e->d_vy, (void **)&e->d_qm, (void **)&e->d_spc,
+                     (void **)&e->d_status, (void **)&e->d_hit, (void **)&e->d_surv[0], (void **)&e->d_surv[1],
+                     (void **)&e->d_surv_count, (void **)&e->d_queue, (void **)&e->d_batch, (void **)&e->d_counts,
+                     (void **)&e->d_scal}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+}
+
+int ensemble_build_device(th_ensemble *e);
+
+// Arenas, pool and tallies on the device; on failure nothing stays allocated.
 int ensemble_build(th_ensemble *e) {
     if (e->built) return TH_OK;
+    int rc = ensemble_build_device(e);
+    if (rc) ensemble_free_device(e);
+    return rc;
+}
+
+int ensemble_build_device(th_ensemble *e) {
     if (e->inst.empty()) return fail(TH_ESTATE, "the ensemble has no instances");
     const int n = (int)e->inst.size();
     const int64_t pool = std::max<int64_t>(e->n_pool, 1);
@@ -350,13 +372,7 @@ void th_ensemble_destroy(th_ensemble *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     ensemble_free_graph(e);
-    for (void *q : {(void *)e->d_cells, (void *)e->d_corners, (void *)e->d_corner_vac, (void *)e->d_vac, (void *)e->d_steps, (void *)e->d_chunks,
-                    (void *)e->d_wface, (void *)e->d_slot, (void *)e->d_inst, (void *)e->d_starts,
-                    (void *)e->d_inject, (void *)e->d_x, (void *)e->d_y, (void *)e->d_vx, (void *)e->d_vy,
-                    (void *)e->d_qm, (void *)e->d_spc, (void *)e->d_status, (void *)e->d_hit, (void *)e->d_surv[0],
-                    (void *)e->d_surv[1], (void *)e->d_surv_count, (void *)e->d_queue, (void *)e->d_batch,
-                    (void *)e->d_counts, (void *)e->d_scal})
-        if (q) (void)hipFree(q);
+    ensemble_free_device(e);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
