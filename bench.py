@@ -108,7 +108,7 @@ def precharge(p, batches, log):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; this build has no multi-GPU path (asked for --gpus {a.gpus})",
+        print(f"bench.py measures one GPU; this package has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:

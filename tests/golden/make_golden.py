@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Writes the golden vectors under tests/golden/ by importing the reference
-(Reasonia-TK/trench01, read-only checkout) in the build container.
+(Reasonia-TK/trench01, read-only checkout).
 
-Only run here, never on the GPU box; the reference does not travel.  Each
+Only run where that checkout exists; the reference does not travel with the tests.  Each
 vector pins a reference function that the reference's own fixtures
 (tests/data/base_*.npz: absorb wall, no mask, no taper) leave open:
 
