@@ -10,7 +10,8 @@ Te = 4 eV).
 One step = one batch, entirely on the device: injection (Philox), FP64 push
 with LDS deposit tallies, deposit, leak, floating-conductor step, potential
 (Green's matrix GEMV) and field, plus the batch's diagnostics read back for
-the host loop.  Timed from a pre-charged state (the same loop run first with
+the host loop (tallies, probe potentials and region charges: what the
+reference's loop records per batch, trench_charging_2d.py:1271-1285).  Timed from a pre-charged state (the same loop run first with
 2e4 particles per species per batch, which deposits the same charge per
 batch), so path lengths are those of a charged trench.
 
@@ -128,10 +129,11 @@ def main():
     precision = precision_errors(dev, setup, a.green) if (a.green or a.precision) else None
     n = 2 * p.n_per_batch
     dev.reserve(n)
+    dev.set_monitors(setup.probes, setup.regions)   # what the batch loop records per batch
     for b in range(a.warmup):
         dev.inject_device(0, p.n_per_batch, p.seed, b)
         dev.step(0, n)
-        dev.read()
+        dev.monitors()
     # keep the state of the first timed batch for the CPU baseline
     dev.inject_device(0, p.n_per_batch, p.seed, a.warmup)
     cpu_inputs = (dev.tracer.read_particles(0, n), dev.tracer.read_field(), dev.read(counts=False)[1])
@@ -141,7 +143,7 @@ def main():
     for b in range(a.warmup, a.warmup + a.steps):
         dev.inject_device(0, p.n_per_batch, p.seed, b)
         d = dev.step(0, n)
-        dev.read()
+        dev.monitors()
         steps_total += d.tally.steps
         push_ms += d.push_ms
         update_ms += d.update_ms
