@@ -318,7 +318,9 @@ def time_to_steady(p, max_batches, log, push_precision=0, green=0):
     1e6 particles per batch): plain time-marching until the reference's
     criterion passes (W = 30, hold = 2), and the chosen method (Anderson
     solve of the batch map's fixed point, then plain batches to the same
-    criterion).  Wall times include each method's host set-up."""
+    criterion), and the chosen method with the precision variants (FP32 push,
+    BF16 split Green's update, unless others are given) with its drift.  Wall
+    times include each method's host set-up."""
     from trench01_amd import charging, steady
     ps = replace(p, until_steady=True, max_batches=max_batches)
     t0 = time.perf_counter()
@@ -338,16 +340,18 @@ def time_to_steady(p, max_batches, log, push_precision=0, green=0):
         f"bottom centre {chosen['bottom_center_V']:.3f} V")
     out = {"plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
            "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]}
-    if push_precision or green:
-        rv = steady.anderson(ps, push_precision=push_precision, green_precision=green)
-        out["chosen_precision_variant"] = {
-            "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green],
-            "converged": bool(rv["converged"]), "batches": rv["batches"], "wall_s": rv["wall_s"],
-            "bottom_center_V": rv["steady"]["bottom center"],
-            "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain["bottom_center_V"],
-            "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen["bottom_center_V"]}
-        log(f"time-to-steady, chosen with precision variants: {rv['wall_s']:.1f}s, bottom centre "
-            f"{rv['steady']['bottom center']:.3f} V")
+    # the precision pattern in the steady solve: FP32 push + BF16 split Green's unless asked otherwise
+    if not (push_precision or green):
+        push_precision, green = 1, 1
+    rv = steady.anderson(ps, push_precision=push_precision, green_precision=green)
+    out["chosen_precision_variant"] = {
+        "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green],
+        "converged": bool(rv["converged"]), "batches": rv["batches"], "wall_s": rv["wall_s"],
+        "speedup_vs_plain": wall / rv["wall_s"], "bottom_center_V": rv["steady"]["bottom center"],
+        "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain["bottom_center_V"],
+        "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen["bottom_center_V"]}
+    log(f"time-to-steady, chosen with precision variants: {rv['wall_s']:.1f}s, bottom centre "
+        f"{rv['steady']['bottom center']:.3f} V")
     return out
 
 
