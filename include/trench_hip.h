@@ -139,6 +139,25 @@ int th_trace_rf_record(th_tracer *t, const double *x, const double *y, const dou
                        const int32_t *slots, int32_t n_probe, double *paths, int32_t *path_lengths,
                        double *flight_time, double *phase_clock);
 
+/* Tangent (forward-mode, pathwise) push: the record-mode FP64 trace of n
+ * particles -- status, hit, endpoints and final_velocity equal those of
+ * th_trace_record -- together with the derivative of every path along one
+ * direction theta: initial-state tangents dx0, dy0, dvx0, dvy0 per particle
+ * (each nullable = 0) and a field tangent dEx, dEy per cell (both nullable =
+ * fixed field).  Outputs per particle: steps, the face crossed by the recorded
+ * end point (1 x-face, 0 y-face, -1 none: still live), d_endpoints and
+ * d_final_velocity ([n][2] each).  The discrete events of each path (cells,
+ * wall hits, barrier decisions, step count) are held fixed.  No reference
+ * counterpart: the derivative that the differentiation of the charging loop
+ * asks for (gradient of the deposit with respect to the ion distribution and
+ * the surface charge), built on the push of trench_trace.cu:17-185. */
+int th_trace_tangent(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                     const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                     const double *dx0, const double *dy0, const double *dvx0, const double *dvy0,
+                     const double *dEx, const double *dEy, th_tally *tally, int8_t *status, int32_t *hit,
+                     int32_t *steps, int8_t *face, double *endpoints, double *d_endpoints, double *final_velocity,
+                     double *d_final_velocity);
+
 /* Device-resident particle pool, so a batch can be traced with its states
  * already in HBM.  th_particles_reserve sizes the pool (contents are lost);
  * th_particles_upload copies n host particles to pool[offset, offset+n).

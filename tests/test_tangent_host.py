@@ -1,0 +1,165 @@
+"""Host side of the pathwise derivative (trench01_amd.tangent): the ion
+sampler's draws and derivatives, the batch draw order, the field tangent of a
+charge direction, and the face-smoothed deposit."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from oracle import charging_ref as R
+from trench01_amd import configs, model, tangent
+
+
+@pytest.mark.parametrize("name", ["AR5", "AR8"])
+def test_sampler_draws_are_the_configured_sampler(name):
+    p, sampler = configs.product_params(name)
+    a = tangent.ion_sampler(p)(np.random.default_rng(9), 5000)
+    b = sampler(np.random.default_rng(9), 5000)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("name", ["AR5", "AR8"])
+def test_sampler_tangents_are_the_derivatives_of_the_draws(name):
+    """Central differences with the same random stream (sigma scaled by
+    1 +- 1e-4, energy shifted by +- 1e-4 of its mean)."""
+    p, _ = configs.product_params(name)
+    s0 = tangent.ion_sampler(p)
+    vx, vy = s0(np.random.default_rng(5), 4000)
+    sig = p.ion_angle_deg
+    e_mean = p.ion_energy_eV if p.ion_distribution == "mono" else 0.5 * (p.ion_energy_lo_eV + p.ion_energy_hi_eV)
+    h = 1e-4
+    for key, plus, minus, step in (
+            ("sigma", dict(sigma_deg=sig * (1 + h)), dict(sigma_deg=sig * (1 - h)), 2 * h * sig),
+            ("energy", dict(energy_shift=h * e_mean), dict(energy_shift=-h * e_mean), 2 * h * e_mean)):
+        ap = tangent.ion_sampler(p, **plus)(np.random.default_rng(5), 4000)
+        am = tangent.ion_sampler(p, **minus)(np.random.default_rng(5), 4000)
+        for c in (0, 1):
+            fd = (ap[c] - am[c]) / step
+            t = s0.tangents[key][c]
+            np.testing.assert_allclose(t, fd, rtol=1e-6, atol=1e-6 * np.abs(t).max())
+    # closed forms: |v| is independent of sigma, and the angle of v independent of the energy
+    speed = np.hypot(vx, vy)
+    ds = (vx * s0.tangents["sigma"][0] + vy * s0.tangents["sigma"][1]) / speed
+    np.testing.assert_allclose(ds, 0.0, atol=1e-9 * np.abs(s0.tangents["sigma"][0]).max())
+    cross = vx * s0.tangents["energy"][1] - vy * s0.tangents["energy"][0]
+    np.testing.assert_allclose(cross, 0.0, atol=1e-9 * speed.max() * np.abs(s0.tangents["energy"][0]).max())
+
+
+def test_draw_batch_keeps_the_reference_draw_order():
+    p, sampler = configs.product_params("AR5", n_per_batch=700)
+    s = tangent.ion_sampler(p)
+    parts, tan = tangent.draw_batch(p, 200, 13, s)
+    ref = R.inject(replace(R.Params(), **{**configs.AR5, "n_per_batch": 700}), 700, 200, np.random.default_rng(13),
+                   ion_velocity=sampler)
+    for a, b in zip(parts, ref):
+        np.testing.assert_array_equal(a, b)
+    for key in ("sigma", "energy"):
+        assert np.all(tan[key][0][700:] == 0) and np.all(tan[key][1][700:] == 0)
+        assert np.abs(tan[key][0][:700]).max() > 0
+
+
+def test_linear_field_is_the_linear_part_of_the_field_map():
+    p, _ = configs.product_params("AR5")
+    geo = model.Geometry(p)
+    rng = np.random.default_rng(2)
+    a, b = rng.standard_normal((2, *geo.solid.shape))
+    fa = model.compute_field(a + b, geo.vac, p)
+    fb = model.compute_field(b, geo.vac, p)
+    la = tangent.linear_field(a, geo.vac, p)
+    for c in (0, 1):
+        np.testing.assert_allclose(fa[c] - fb[c], la[c], rtol=1e-9, atol=1e-9 * np.abs(la[c]).max())
+        l2 = tangent.linear_field(2.0 * a, geo.vac, p)[c]
+        np.testing.assert_array_equal(l2, 2.0 * la[c])
+
+
+# A small wall: a solid block in the lower half of a periodic 8 x 6 grid;
+# its surface is the block's top row and its exposed sides.
+def _block():
+    vac = np.ones((8, 6), bool)
+    vac[2:6, 0:3] = False
+    surface = np.flatnonzero(R.surface_cells(~vac))
+    return vac, surface
+
+
+def _record(vac, crossings, dx=1.0):
+    """Per-particle outputs of absorbed particles: (cell (ix, iy), x-face?,
+    end point, its tangent)."""
+    ny = vac.shape[1]
+    n = len(crossings)
+    status = np.ones(n, np.int8)
+    hit = np.array([ix * ny + iy for (ix, iy), *_ in crossings], np.int32)
+    face = np.array([1 if xf else 0 for _, xf, *_ in crossings], np.int8)
+    end = np.array([e for *_, e, _ in crossings], float) * dx
+    dend = np.array([d for *_, d in crossings], float) * dx
+    return status, hit, face, end, dend
+
+
+def test_face_deposit_shares_by_the_crossing_point():
+    vac, surface = _block()
+    ny = vac.shape[1]
+    k = {c: i for i, c in enumerate(surface)}
+    spc = np.zeros(4, np.int8)
+    rec = _record(vac, [
+        ((3, 2), False, (3.5, 3.0), (0.0, 0.0)),    # top face, centre: all to the hit cell
+        ((3, 2), False, (3.8, 3.0), (1.0, 0.0)),    # top face, right of centre: 0.3 to (4, 2)
+        ((2, 1), True, (2.0, 1.2), (0.0, 2.0)),     # left side face from the cell before: 0.3 to (2, 0)
+        ((5, 2), False, (5.9, 3.0), (0.5, 0.0)),    # top face at the corner: no neighbour along the face
+    ])
+    dep, ddep = tangent.face_deposit(vac, surface, spc, *rec, 1.0)
+    expect = np.zeros(surface.size)
+    expect[k[3 * ny + 2]] += 1.0 + 0.7
+    expect[k[4 * ny + 2]] += 0.3
+    expect[k[2 * ny + 1]] += 0.7
+    expect[k[2 * ny + 0]] += 0.3
+    expect[k[5 * ny + 2]] += 1.0
+    np.testing.assert_allclose(dep[0], expect, atol=1e-12)
+    np.testing.assert_array_equal(dep[1], 0.0)
+    dexp = np.zeros(surface.size)
+    dexp[k[4 * ny + 2]] += 1.0
+    dexp[k[3 * ny + 2]] -= 1.0
+    dexp[k[2 * ny + 0]] -= 2.0        # moving up, away from the cell below
+    dexp[k[2 * ny + 1]] += 2.0
+    np.testing.assert_allclose(ddep[0], dexp, atol=1e-12)
+
+
+def test_face_deposit_is_continuous_and_conserving_and_its_derivative_is_its_slope():
+    vac, surface = _block()
+    rng = np.random.default_rng(4)
+    n = 400
+    # crossing points along the block's top (y = 3), x in (2, 6), and its left face (x = 2), y in (0, 3)
+    top = rng.uniform(2.0, 6.0, n)
+    left = rng.uniform(0.0, 3.0, n)
+    d_top, d_left = rng.standard_normal((2, n))
+    spc = rng.integers(0, 2, 2 * n).astype(np.int8)
+
+    def rec(shift):
+        xs, ys = top + shift * d_top, left + shift * d_left
+        crossings = [((int(np.floor(x)), 2), False, (x, 3.0), (dt, 0.0)) for x, dt in zip(xs, d_top)]
+        crossings += [((2, int(np.floor(y))), True, (2.0, y), (0.0, dl)) for y, dl in zip(ys, d_left)]
+        return _record(vac, crossings)
+
+    dep, ddep = tangent.face_deposit(vac, surface, spc, *rec(0.0), 1.0)
+    np.testing.assert_allclose(dep.sum(axis=1), [np.sum(spc == 0), np.sum(spc == 1)], atol=1e-9)
+    np.testing.assert_allclose(ddep.sum(axis=1), 0.0, atol=1e-9)
+    h = 1e-7
+    plus = tangent.face_deposit(vac, surface, spc, *rec(h), 1.0)[0]
+    minus = tangent.face_deposit(vac, surface, spc, *rec(-h), 1.0)[0]
+    # no crossing point sits within h of a cell edge or a face centre here, so the slope is exact
+    np.testing.assert_allclose((plus - minus) / (2 * h), ddep, atol=1e-6)
+    # continuity across a cell edge: a point just left / right of x = 4 gives the same split
+    eps = 1e-9
+    a = tangent.face_deposit(vac, surface, np.zeros(1, np.int8),
+                             *_record(vac, [((3, 2), False, (4.0 - eps, 3.0), (0.0, 0.0))]), 1.0)[0]
+    b = tangent.face_deposit(vac, surface, np.zeros(1, np.int8),
+                             *_record(vac, [((4, 2), False, (4.0 + eps, 3.0), (0.0, 0.0))]), 1.0)[0]
+    np.testing.assert_allclose(a, b, atol=1e-8)
+
+
+def test_face_deposit_skips_live_and_escaped_particles():
+    vac, surface = _block()
+    st, hit, face, end, dend = _record(vac, [((3, 2), False, (3.2, 3.0), (1.0, 0.0))] * 3)
+    st = np.array([1, 2, 3], np.int8)
+    face = np.array([0, 0, -1], np.int8)
+    dep, ddep = tangent.face_deposit(vac, surface, np.zeros(3, np.int8), st, hit, face, end, dend, 1.0)
+    assert dep.sum() == pytest.approx(1.0)

@@ -78,6 +78,9 @@ SIGNATURES = {
     "th_trace_rf_record": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, dp, ctypes.c_int64,
                                           ctypes.POINTER(TraceParams), i32p, i64p, i64p, ctypes.POINTER(Tally),
                                           i8p, i32p, dp, dp, i32p, ctypes.c_int32, dp, i32p, dp, dp]),
+    "th_trace_tangent": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                        dp, dp, dp, dp, dp, dp, ctypes.POINTER(Tally), i8p, i32p, i32p, i8p, dp, dp,
+                                        dp, dp]),
     "th_particles_upload_times": (ctypes.c_int, [vp, ctypes.c_int64, dp, ctypes.c_int64]),
     "th_particles_reserve": (ctypes.c_int, [vp, ctypes.c_int64]),
     "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),

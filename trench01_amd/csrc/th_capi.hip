@@ -189,6 +189,7 @@ int launch_kernel(th_tracer *t, K kernel, const th::Grid &g, const th::Work &w, 
 template <bool B, bool T>
 int launch_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds, int precision,
                    bool record, bool rf) {
+    if (w.tan.dend) return launch_kernel(t, th::push64_tangent_kernel<B, T>, g, w, want, lds);
     if (rf)
         return record ? launch_kernel(t, th::push64_rf_record_kernel<B, T>, g, w, want, lds)
                       : launch_kernel(t, th::push64_rf_kernel<B, T>, g, w, want, lds);
@@ -368,7 +369,7 @@ int upload(th_tracer *t, int64_t off, const double *x, const double *y, const do
 // is read between them; *traced is the end of the traced prefix.
 int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p, const volatile int32_t *stop,
                  bool want_status, bool want_hit, int64_t *traced, th_tally *tally,
-                 const th::Record *rec = nullptr) {
+                 const th::Record *rec = nullptr, const th::TangentIO *tan = nullptr) {
     t->blocks_per_cu = p->blocks_per_cu;
     TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
     TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
@@ -393,13 +394,14 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
         w.prio_steps = p->prio_steps > 0 ? p->prio_steps : 0;
         w.check_steps = p->check_steps > 0 ? std::min(p->check_steps, 256) : th::kStepsPerCheck;
         w.rec = rec ? *rec : th::Record{};
+        w.tan = tan ? *tan : th::TangentIO{};
         w.step_cap = p->max_steps;
         w.in = nullptr;
         w.in_count = nullptr;
         w.out = nullptr;
         w.out_count = nullptr;
         int rc;
-        const bool passes = !stop && !rec && p->pass_steps > 0 && p->pass_steps < p->max_steps;
+        const bool passes = !stop && !rec && !tan && p->pass_steps > 0 && p->pass_steps < p->max_steps;
         if (passes) {
             // Multi-pass: caps pass_steps, 2*pass_steps, ... up to max_steps; pass k
             // continues the survivors of pass k-1.  The passes are stream-ordered;
@@ -1215,6 +1217,98 @@ extern "C" int th_trace_rf_record(th_tracer *t, const double *x, const double *y
     return trace_record_host(t, x, y, vx, vy, qm, spc, initial_time, n, p, stop, cnt_i, cnt_e, tally, status, hit,
                              endpoints, final_velocity, slots, n_probe, paths, path_lengths, flight_time,
                              phase_clock);
+}
+
+extern "C" int th_trace_tangent(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                                const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                                const double *dx0, const double *dy0, const double *dvx0, const double *dvy0,
+                                const double *dEx, const double *dEy, th_tally *tally, int8_t *status, int32_t *hit,
+                                int32_t *steps, int8_t *face, double *endpoints, double *d_endpoints,
+                                double *final_velocity, double *d_final_velocity) {
+    if (!t || !p || !tally || !status || !hit || !steps || !face || !endpoints || !d_endpoints || !final_velocity ||
+        !d_final_velocity)
+        return fail(TH_EINVAL, "invalid argument");
+    if (n < 0) return fail(TH_EINVAL, "invalid particle count");
+    if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    if ((dEx == nullptr) != (dEy == nullptr)) return fail(TH_EINVAL, "dEx and dEy go together");
+    if (p->precision != 0 || p->rf) return fail(TH_EINVAL, "the tangent push follows the FP64 push without RF");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; ++k)
+        if (spc[k] != 0 && spc[k] != 1) return fail(TH_EINVAL, "species must be 0 (ion) or 1 (electron)");
+    memset(tally, 0, sizeof *tally);
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    if ((rc = reserve_particles(t, n))) return rc;
+    if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
+    // per-particle buffers: 4 initial tangents, end point, its tangent, velocity, its tangent
+    double *d_buf = nullptr;
+    double2 *d_field = nullptr;
+    int8_t *d_face = nullptr;
+    int32_t *d_steps = nullptr;
+    auto release = [&]() {
+        for (void *q : {(void *)d_buf, (void *)d_field, (void *)d_face, (void *)d_steps})
+            if (q) (void)hipFree(q);
+    };
+    const size_t nn = (size_t)n;
+    if ((rc = dev_alloc(&d_buf, 12 * nn)) || (dEx && (rc = dev_alloc(&d_field, (size_t)t->N))) ||
+        (rc = dev_alloc(&d_face, nn)) || (rc = dev_alloc(&d_steps, nn))) {
+        release();
+        return rc;
+    }
+    double *d_init[4] = {d_buf, d_buf + nn, d_buf + 2 * nn, d_buf + 3 * nn};
+    const double *h_init[4] = {dx0, dy0, dvx0, dvy0};
+    double *d_end = d_buf + 4 * nn, *d_dend = d_buf + 6 * nn, *d_vel = d_buf + 8 * nn, *d_dvel = d_buf + 10 * nn;
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 4 && e == hipSuccess; ++c)
+        if (h_init[c]) e = hipMemcpyAsync(d_init[c], h_init[c], sizeof(double) * nn, hipMemcpyHostToDevice, t->stream);
+    std::vector<double2> field;
+    if (dEx) {   // vacuum-masked, as the push's own records
+        field.resize((size_t)t->N);
+        for (int c = 0; c < t->N; ++c)
+            field[c] = t->vac[c] ? make_double2(dEx[c], dEy[c]) : make_double2(0.0, 0.0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_field, field.data(), sizeof(double2) * field.size(), hipMemcpyHostToDevice,
+                               t->stream);
+    }
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(t->stream);
+        release();
+        return fail(TH_EHIP, "tangent upload: %s", hipGetErrorString(e));
+    }
+    th::Record rec{d_end, d_vel, nullptr, nullptr, nullptr, p->max_steps + 1, nullptr, nullptr};
+    th::TangentIO tan{h_init[0] ? d_init[0] : nullptr, h_init[1] ? d_init[1] : nullptr,
+                      h_init[2] ? d_init[2] : nullptr, h_init[3] ? d_init[3] : nullptr, d_field, d_dend, d_dvel,
+                      d_face, d_steps};
+    int64_t traced = 0;
+    rc = launch_trace(t, 0, n, p, nullptr, true, true, &traced, tally, &rec, &tan);
+    if (rc) {
+        (void)hipStreamSynchronize(t->stream);
+        release();
+        return rc;
+    }
+    unsigned long long scal[th::kScalCount];
+    e = hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(status, t->d_status, nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hit, t->d_hit, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(steps, d_steps, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(face, d_face, nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(endpoints, d_end, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_endpoints, d_dend, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(final_velocity, d_vel, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_final_velocity, d_dvel, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "tangent readback: %s", hipGetErrorString(e));
+    float ms = 0.0f;
+    TH_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    t->push_ms = ms;
+    fill_tally(scal, n - traced, tally);
+    return TH_OK;
 }
 
 extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *probe_cells,

@@ -102,6 +102,18 @@ struct Survivor {
     int32_t pid, k;
 };
 
+// Tangent-mode inputs and outputs (push64_tangent_kernel): the direction's
+// initial-state tangents per particle and field tangent per cell (each null =
+// zero), and per particle the tangent of the recorded end point and of the
+// final velocity, the face crossed at the end, and the step count.
+struct TangentIO {
+    const double *dx0, *dy0, *dvx0, *dvy0;   // [n] or null
+    const double2 *dfield;                   // [nx*ny] d{Ex*vac, Ey*vac} or null
+    double *dend, *dvel;                     // [n][2]
+    int8_t *face;                            // [n] 1 x-face, 0 y-face, -1 none
+    int32_t *steps;                          // [n]
+};
+
 struct Work {
     const double *__restrict__ x, *__restrict__ y, *__restrict__ vx, *__restrict__ vy, *__restrict__ qm;
     const int8_t *__restrict__ spc;
@@ -123,6 +135,7 @@ struct Work {
     const unsigned *in_count;
     Survivor *out;
     unsigned *out_count;
+    TangentIO tan;                       // tangent mode only (single pass)
 };
 
 __device__ __forceinline__ int wrap_cell(int i, int n) {
@@ -201,13 +214,113 @@ struct StateT {
 using State64 = StateT<double>;
 using State32 = StateT<float>;
 
+// Step hooks of the FP64 push that do nothing: the push kernels' policy.
+struct NoTangent {
+    template <class V>
+    __device__ __forceinline__ void field(const V &, int, int, int, int, double, double, double, double, double,
+                                          double, double2, double2, double2, double2, double, double, double, double,
+                                          bool, double, double, double) {}
+    __device__ __forceinline__ void push(double, double, double, double, double, double, double) {}
+    __device__ __forceinline__ void reflect(bool) {}
+    __device__ __forceinline__ void crossing(bool, double, double, double, double) {}
+    __device__ __forceinline__ void point() {}
+};
+
+// Forward-mode derivative of one particle's path along one direction theta
+// (pathwise: the discrete events of the path -- cells, wall hits, barrier
+// decisions, step count -- are held fixed).  The direction enters through the
+// initial state (dx, dy, dvx, dvy) and through a field tangent: per cell
+// d(Ex, Ey)/d(theta), vacuum-masked, or null for a fixed field.  The hooks
+// follow the FP64 step's arithmetic term by term:
+//   E = sum w_k e_k / sum w_k v_k over the four corners, w bilinear in
+//       (x/dx - 0.5, y/dx - 0.5), the denominator's clamp at 1e-12 held;
+//   dt = cfl dx / (|v| + sqrt(2 dx |q/m E|) + 1);
+//   v' = v + a dt,  s = (v + v') dt / 2,  x' = x + s (the periodic wrap is a
+//   constant shift);
+//   a barrier reflection restores the start point and flips one velocity
+//   component; the recorded crossing point x0 + f s with f = (face - x0)/s.
+struct Tangent64 {
+    double dx = 0.0, dy = 0.0, dvx = 0.0, dvy = 0.0;   // d(state)/d(theta)
+    double dx0 = 0.0, dy0 = 0.0;                       // at the step's start
+    double dex = 0.0, dey = 0.0;                       // d(E) at the step
+    double dsx = 0.0, dsy = 0.0;                       // d(step displacement)
+    double dpx = 0.0, dpy = 0.0;                       // d(recorded point)
+    int face = -1;                                     // crossing: 1 x-face, 0 y-face, -1 none
+    const double2 *__restrict__ dfield = nullptr;
+
+    template <class V>
+    __device__ __forceinline__ void field(const V &g, int i0, int i1, int j0, int j1, double tx, double ty,
+                                          double w00, double w10, double w01, double w11, double2 e00, double2 e10,
+                                          double2 e01, double2 e11, double v00, double v10, double v01, double v11,
+                                          bool clamped, double sw, double ex, double ey) {
+        dx0 = dx;
+        dy0 = dy;
+        const double dtx = dx * g.inv_dx, dty = dy * g.inv_dx;
+        const double d00 = -dtx * (1.0 - ty) - (1.0 - tx) * dty, d10 = dtx * (1.0 - ty) - tx * dty;
+        const double d01 = -dtx * ty + (1.0 - tx) * dty, d11 = dtx * ty + tx * dty;
+        double nx_ = d00 * e00.x + d10 * e10.x + d01 * e01.x + d11 * e11.x;
+        double ny_ = d00 * e00.y + d10 * e10.y + d01 * e01.y + d11 * e11.y;
+        if (dfield) {
+            const int ny = g.ny;
+            const double2 f00 = dfield[i0 * ny + j0], f10 = dfield[i1 * ny + j0];
+            const double2 f01 = dfield[i0 * ny + j1], f11 = dfield[i1 * ny + j1];
+            nx_ += w00 * f00.x + w10 * f10.x + w01 * f01.x + w11 * f11.x;
+            ny_ += w00 * f00.y + w10 * f10.y + w01 * f01.y + w11 * f11.y;
+        }
+        const double dsw = clamped ? 0.0 : d00 * v00 + d10 * v10 + d01 * v01 + d11 * v11;
+        dex = (nx_ - ex * dsw) / sw;
+        dey = (ny_ - ey * dsw) / sw;
+        face = -1;
+    }
+    // vx, vy: the step's start velocity; (ax, ay) = q/m E; dt as computed.
+    __device__ __forceinline__ void push(double vx, double vy, double qm, double ax, double ay, double dt,
+                                         double two_dx) {
+        const double dax = qm * dex, day = qm * dey;
+        const double hv = th_hypot(vx, vy), ha = th_hypot(ax, ay);
+        const double root = sqrt(two_dx * ha);
+        const double dhv = hv > 0.0 ? (vx * dvx + vy * dvy) / hv : 0.0;
+        const double dha = ha > 0.0 ? (ax * dax + ay * day) / ha : 0.0;
+        const double droot = root > 0.0 ? 0.5 * two_dx * dha / root : 0.0;
+        const double ddt = -dt * (dhv + droot) / (hv + root + 1.0);
+        const double dvxn = dvx + dax * dt + ax * ddt, dvyn = dvy + day * dt + ay * ddt;
+        const double vxn = vx + ax * dt, vyn = vy + ay * dt;
+        dsx = 0.5 * ((dvx + dvxn) * dt + (vx + vxn) * ddt);
+        dsy = 0.5 * ((dvy + dvyn) * dt + (vy + vyn) * ddt);
+        dx = dx0 + dsx;
+        dy = dy0 + dsy;
+        dvx = dvxn;
+        dvy = dvyn;
+    }
+    __device__ __forceinline__ void reflect(bool face_x) {
+        dx = dx0;
+        dy = dy0;
+        if (face_x) dvx = -dvx; else dvy = -dvy;
+    }
+    // The step's end was recorded at fraction f of the step, where it crosses
+    // an x-face (x_face) or a y-face; f_raw before the clip to [0, 1].
+    __device__ __forceinline__ void crossing(bool x_face, double f_raw, double f, double sx, double sy) {
+        double df = 0.0;
+        if (f_raw > 0.0 && f_raw < 1.0) df = x_face ? -(dx0 + f_raw * dsx) / sx : -(dy0 + f_raw * dsy) / sy;
+        dpx = dx0 + df * sx + f * dsx;
+        dpy = dy0 + df * sy + f * dsy;
+        face = x_face ? 1 : 0;
+    }
+    __device__ __forceinline__ void point() {
+        dpx = dx;
+        dpy = dy;
+    }
+};
+
 // One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
 // (escaped).  With RECORD, (xp, yp) is the step's recorded point.  With RF the
 // voltage wave acts during the step and rc advances.  V reads one instance's
 // records: Grid itself, or an ensemble lane's view (th_ensemble.hpp); cell
-// indices are the instance's own.
-template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V>
-__device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc) {
+// indices are the instance's own.  T receives the step's intermediate values
+// (NoTangent, or Tangent64 for the derivative of the path).
+template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V, class T>
+__device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc,
+                                       T &tan) {
+    static_assert(std::is_same<T, NoTangent>::value || (RECORD && !RF), "the tangent follows the FP64 record step");
     const int nx = g.nx, ny = g.ny;
     // Vacuum-weighted bilinear field at the cell-centred lattice.
     double fx = s.q - 0.5, fy = s.r - 0.5;
@@ -249,9 +362,12 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
     ex += w10 * e10.x; ey += w10 * e10.y; sw += w10 * v10;
     ex += w01 * e01.x; ey += w01 * e01.y; sw += w01 * v01;
     ex += w11 * e11.x; ey += w11 * e11.y; sw += w11 * v11;
-    sw = sw < 1e-12 ? 1e-12 : sw;
+    const bool clamped = sw < 1e-12;
+    sw = clamped ? 1e-12 : sw;
     ex = ex / sw;
     ey = ey / sw;
+    tan.field(g, i0, i1, j0, j1, tx, ty, w00, w10, w01, w11, e00, e10, e01, e11, v00, v10, v01, v11, clamped, sw,
+              ex, ey);
     // CFL-limited trapezoidal push.
     double ax, ay, dt;
     if constexpr (RF) {
@@ -281,6 +397,7 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
         ax = s.qm * ex;
         ay = s.qm * ey;
         dt = g.cfl_dx / (th_hypot(s.vx, s.vy) + sqrt(g.two_dx * th_hypot(ax, ay)) + 1.0);
+        tan.push(s.vx, s.vy, s.qm, ax, ay, dt, g.two_dx);
     }
     double vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
     const double x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
@@ -331,6 +448,7 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
                 if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
                 s.x = x0; s.y = y0; s.q = q0; s.r = r0;
                 hit = false;
+                tan.reflect(face_x);
             }
         }
         ix = cx;
@@ -340,25 +458,29 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
         xp = s.x;
         yp = s.y;
         double tf = 1.0;   // fraction of the step to the recorded point
+        tan.point();
         if (hit) {
-            double frac;
+            double raw;
             if (face_x) {
                 double xb = (double)(cx + (sx < 0.0)) * g.dx;
                 xb += nearbyint((x0 - xb) / g.len_x()) * g.len_x();
-                frac = (xb - x0) / sx;
+                raw = (xb - x0) / sx;
             } else {
-                frac = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
+                raw = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
             }
-            frac = fmin(fmax(frac, 0.0), 1.0);
+            double frac = fmin(fmax(raw, 0.0), 1.0);
             tf = frac;
             xp = wrap_pos(x0 + frac * sx, g.len_x());
             yp = y0 + frac * sy;
+            tan.crossing(face_x, raw, frac, sx, sy);
         }
         if (escaped) {
-            double frac = fmin(fmax((g.len_y() - y0) / sy, 0.0), 1.0);
+            double raw = (g.len_y() - y0) / sy;
+            double frac = fmin(fmax(raw, 0.0), 1.0);
             tf = frac;
             xp = wrap_pos(x0 + frac * sx, g.len_x());
             yp = g.len_y();
+            tan.crossing(false, raw, frac, sx, sy);
         }
         if constexpr (RF) {
             rc.ft = rc.elapsed - (1.0 - tf) * dt;
@@ -367,6 +489,12 @@ __device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double 
     }
     if (hit) { cell = ix * ny + iy; return 1; }
     return escaped ? 2 : 0;
+}
+
+template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V>
+__device__ __forceinline__ int step64(const V &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc) {
+    NoTangent none;
+    return step64t<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc, none);
 }
 
 __device__ __forceinline__ float wrap_posf(float a, float L) {
@@ -471,14 +599,16 @@ __device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *l
     }
 }
 
-enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4 };
+enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4, kModeF64Tangent = 5 };
 
 // Persistent refill loop shared by every push variant.
 template <int MODE, bool BARRIER, bool TAPERED>
 __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     using Real = typename std::conditional<MODE == kModeF32, float, double>::type;
-    constexpr bool RECORD = MODE == kModeF64Record || MODE == kModeF64RfRecord;
+    constexpr bool TANGENT = MODE == kModeF64Tangent;
+    constexpr bool RECORD = MODE == kModeF64Record || MODE == kModeF64RfRecord || TANGENT;
     constexpr bool RF = MODE == kModeF64Rf || MODE == kModeF64RfRecord;
+    using Tan = typename std::conditional<TANGENT, Tangent64, NoTangent>::type;
     extern __shared__ uint32_t lds_counts[];   // [2][m]
     __shared__ unsigned long long lds_scal[kScalCount];
     for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
@@ -495,6 +625,8 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     int slot = -1, plen = 0;      // record mode: probe slot and path length
     double xp = 0.0, yp = 0.0;    // record mode: last recorded point
     RfClock rc{0.0, 0.0, 0.0, 0.0};
+    Tan tn;
+    if constexpr (TANGENT) tn.dfield = w.tan.dfield;
     unsigned long long my_steps = 0;
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
@@ -535,10 +667,19 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         s.r = div_dx(s.y, g.dx, g.inv_dx);
                     }
                     sp = w.spc[pid] != 0;   // a species byte other than 0 / 1 cannot index past the tallies
+                    if constexpr (TANGENT) {   // single pass: every particle starts here
+                        tn.dx = w.tan.dx0 ? w.tan.dx0[pid] : 0.0;
+                        tn.dy = w.tan.dy0 ? w.tan.dy0[pid] : 0.0;
+                        tn.dvx = w.tan.dvx0 ? w.tan.dvx0[pid] : 0.0;
+                        tn.dvy = w.tan.dvy0 ? w.tan.dvy0[pid] : 0.0;
+                        tn.dpx = tn.dx;
+                        tn.dpy = tn.dy;
+                        tn.face = -1;
+                    }
                     if constexpr (RECORD) {
                         xp = s.x;
                         yp = s.y;
-                        slot = w.rec.slots[pid];
+                        slot = TANGENT ? -1 : w.rec.slots[pid];
                         plen = 0;
                         if (slot >= 0) {
                             double *path = w.rec.paths + 2 * (int64_t)slot * w.rec.stride;
@@ -567,7 +708,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 if constexpr (MODE == kModeF32) {
                     st = step32<BARRIER, TAPERED>(g, s, cell);
                 } else {
-                    st = step64<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc);
+                    st = step64t<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc, tn);
                     if constexpr (RECORD) {
                         if (slot >= 0) {
                             double *path = w.rec.paths + 2 * ((int64_t)slot * w.rec.stride + plen);
@@ -590,6 +731,14 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                             w.rec.flight[pid] = rc.ft;
                             w.rec.clock[pid] = rc.ck;
                         }
+                    }
+                    if constexpr (TANGENT) {
+                        w.tan.dend[2 * pid] = tn.dpx;
+                        w.tan.dend[2 * pid + 1] = tn.dpy;
+                        w.tan.dvel[2 * pid] = tn.dvx;
+                        w.tan.dvel[2 * pid + 1] = tn.dvy;
+                        w.tan.face[pid] = (int8_t)tn.face;
+                        w.tan.steps[pid] = k;
                     }
                     my_steps += (unsigned long long)(k - k0);
                     pid = -1;
@@ -639,6 +788,12 @@ __global__ void __launch_bounds__(kBlock) push64_rf_kernel(Grid g, Work w) {
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(kBlock) push64_rf_record_kernel(Grid g, Work w) {
     push_body<kModeF64RfRecord, BARRIER, TAPERED>(g, w);
+}
+
+// FP64 record push with the forward-mode derivative of every path (single pass).
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_tangent_kernel(Grid g, Work w) {
+    push_body<kModeF64Tangent, BARRIER, TAPERED>(g, w);
 }
 
 // FP32 push.

@@ -1,0 +1,179 @@
+"""Pathwise (forward-mode) derivative of one charging batch on the device.
+
+The deposit of a batch (trench_charging_2d.py:1237-1243) is a sum of integer
+counts per surface cell, whose pathwise derivative is zero almost everywhere.
+The quantity differentiated here is the face-smoothed deposit: each absorbed
+particle's charge is shared between its hit cell and the next surface cell
+along the face it crossed, linearly in the crossing point (cloud-in-cell on
+the wall: weight |u - 1/2| to the neighbour on the side of u, the rest to the
+hit cell, u the crossing point's coordinate along the face in cell units).  It
+conserves the absorbed count and is continuous in the crossing point between
+the discrete events of a path (cell sequence, wall hits, barrier decisions,
+step count) and at wall corners without a neighbour along the face, where the
+hit cell keeps the whole charge.
+
+``HipTracer.tangent`` (th_trace_tangent) traces a batch and carries the
+derivative of every path along one direction theta; this module supplies the
+directions the charging loop is differentiated in:
+
+* the ion distribution: angular sigma [deg] and a shift of the ion energy
+  distribution [eV], through the initial velocities (``IonSampler``, the
+  draws of configs.ion_velocity_sampler with their derivatives);
+* the surface charge, through the field: a charge direction drho gives
+  dphi = LU^-1(-drho / eps0) and dE = -grad(dphi) by the field stencil with
+  zero ghost voltages (``charge_direction``).
+
+With common random numbers (the same seed at every theta), the derivative
+equals the central difference of the traced paths for every particle whose
+discrete events are the same at theta - h and theta + h; the derivative of
+the face deposit is the sum of those particles' contributions.  Particles that
+change events (a barrier decision, the hit cell) contribute jumps that the
+pathwise derivative leaves out by construction.
+"""
+from dataclasses import replace
+
+import numpy as np
+
+from . import model
+from .model import AMU, E_CHARGE, EPS0
+
+
+class IonSampler:
+    """The ion velocity draws of configs.ion_velocity_sampler (energy E,
+    monoenergetic or E = lo + (hi - lo) sin^2(pi U / 2); polar angle theta ~
+    N(0, sigma); vx = v sin(theta), vy = -v cos(theta)), with the energy
+    distribution shifted by ``energy_shift`` eV, and the derivatives of the
+    last draw with respect to sigma [per degree] and to the shift [per eV] in
+    ``tangents`` = {"sigma": (dvx, dvy), "energy": (dvx, dvy)}."""
+
+    def __init__(self, ion_mass_amu, kind, sigma_deg, energy_eV=None, e_lo=None, e_hi=None, energy_shift=0.0):
+        if kind not in ("mono", "arcsine"):
+            raise ValueError(f"unknown ion energy distribution {kind!r}")
+        self.m_i = ion_mass_amu * AMU
+        self.kind, self.sigma_deg = kind, float(sigma_deg)
+        self.energy_eV, self.e_lo, self.e_hi = energy_eV, e_lo, e_hi
+        self.energy_shift = float(energy_shift)
+        self.tangents = None
+
+    def __call__(self, rng, n):
+        sig = np.radians(self.sigma_deg)
+        if self.kind == "mono":
+            E = np.full(n, float(self.energy_eV))
+        else:
+            E = self.e_lo + (self.e_hi - self.e_lo) * np.sin(0.5 * np.pi * rng.uniform(size=n)) ** 2
+        if self.energy_shift:
+            E = E + self.energy_shift
+        theta = rng.normal(0.0, sig, n)
+        v = np.sqrt(2.0 * E * E_CHARGE / self.m_i)
+        s, c = np.sin(theta), np.cos(theta)
+        dtheta = theta / self.sigma_deg if self.sigma_deg else np.zeros(n)   # theta = sigma * z
+        dv = E_CHARGE / (self.m_i * v)                                      # dv/dE
+        self.tangents = {"sigma": (v * c * dtheta, v * s * dtheta), "energy": (s * dv, -c * dv)}
+        return v * s, -v * c
+
+
+def ion_sampler(p, energy_shift=0.0, sigma_deg=None):
+    """IonSampler of Params ``p``'s ion distribution fields."""
+    sig = p.ion_angle_deg if sigma_deg is None else sigma_deg
+    if p.ion_distribution == "mono":
+        return IonSampler(p.ion_mass_amu, "mono", sig, energy_eV=p.ion_energy_eV, energy_shift=energy_shift)
+    return IonSampler(p.ion_mass_amu, "arcsine", sig, e_lo=p.ion_energy_lo_eV, e_hi=p.ion_energy_hi_eV,
+                      energy_shift=energy_shift)
+
+
+def draw_batch(p, ny, seed, sampler, n=None):
+    """One host-injected batch (model.inject, the reference's draw order) with
+    ``sampler`` as the ion velocity hook; returns the particles and the ion
+    velocity tangents (zero for electrons) as {"sigma": ..., "energy": ...}
+    of (dvx, dvy) over the whole batch."""
+    n = p.n_per_batch if n is None else n
+    parts = model.inject(p, n, ny, np.random.default_rng(seed), ion_velocity=sampler)
+    total = parts[0].size
+    tan = {}
+    for key, (dvx, dvy) in sampler.tangents.items():
+        full_x, full_y = np.zeros(total), np.zeros(total)
+        full_x[:n], full_y[:n] = dvx, dvy
+        tan[key] = (full_x, full_y)
+    return parts, tan
+
+
+def linear_field(dphi, vac, p):
+    """dE = -grad(dphi) by the field stencil (model.compute_field) with zero
+    ghost voltages: the linear part of the field map."""
+    return model.compute_field(dphi, vac, replace(p, substrate_voltage=0.0), 0.0)
+
+
+def charge_direction(fields, vac, p, drho):
+    """Potential and field tangents of a charge-density direction drho
+    [C/m^3] on the grid: dphi = LU^-1(-drho / eps0), dE = linear_field(dphi)."""
+    dphi = fields.solve((-np.asarray(drho, dtype=float) / EPS0).ravel()).reshape(vac.shape)
+    dex, dey = linear_field(dphi, vac, p)
+    return dphi, dex, dey
+
+
+def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx):
+    """Face-smoothed deposit per species on the surface cells and its
+    derivative: (dep, ddep), each [2, m] in particles (``surface`` = flat
+    indices of the surface cells, ascending)."""
+    vac = np.asarray(vac, dtype=bool)
+    nx, ny = vac.shape
+    m = surface.size
+    smap = np.full(vac.size, -1, dtype=np.int64)
+    smap[surface] = np.arange(m)
+    dep, ddep = np.zeros((2, m)), np.zeros((2, m))
+    a = np.flatnonzero(status == 1)
+    a = a[smap[hit[a]] >= 0]   # hit cells off the surface carry no surface charge
+    if a.size == 0:
+        return dep, ddep
+    c = hit[a].astype(np.int64)
+    cx, cy = c // ny, c % ny
+    xf = face[a] == 1
+    e, de = endpoints[a], d_endpoints[a]
+    sp = np.asarray(spc)[a].astype(np.int64)
+    # coordinate along the face and the side of the face the particle came from
+    along = np.where(xf, e[:, 1] / dx - cy, np.mod(e[:, 0] / dx - cx, nx))
+    across = np.where(xf, np.mod(e[:, 0] / dx - cx, nx), e[:, 1] / dx - cy)
+    d_along = np.where(xf, de[:, 1], de[:, 0]) / dx
+    d_along = np.where((along > 0.0) & (along < 1.0), d_along, 0.0)   # a clipped crossing point stays put
+    along = np.clip(along, 0.0, 1.0)
+    side = np.where(np.rint(across) % nx == 1, 1, -1)   # +1: from the cell after, -1: from the cell before
+    step = np.where(along >= 0.5, 1, -1)
+    # neighbour along the face and the vacuum cell in front of it
+    nbx = np.where(xf, cx, np.mod(cx + step, nx))
+    nby = np.where(xf, cy + step, cy)
+    fx = np.where(xf, np.mod(nbx + side, nx), nbx)
+    fy = np.where(xf, nby, nby + side)
+    inside = (nby >= 0) & (nby < ny) & (fy >= 0) & (fy < ny)
+    nbx_c, nby_c = np.clip(nbx, 0, nx - 1), np.clip(nby, 0, ny - 1)
+    fx_c, fy_c = np.clip(fx, 0, nx - 1), np.clip(fy, 0, ny - 1)
+    nb = nbx_c * ny + nby_c
+    valid = inside & ~vac[nbx_c, nby_c] & vac[fx_c, fy_c] & (smap[nb] >= 0)
+    w_nb = np.where(valid, np.abs(along - 0.5), 0.0)
+    dw_nb = np.where(valid, np.sign(along - 0.5) * d_along, 0.0)
+    k_hit, k_nb = smap[c], smap[np.where(valid, nb, c)]
+    for s in (0, 1):
+        sel = sp == s
+        dep[s] = np.bincount(k_hit[sel], 1.0 - w_nb[sel], m) + np.bincount(k_nb[sel], w_nb[sel], m)
+        ddep[s] = np.bincount(k_hit[sel], -dw_nb[sel], m) + np.bincount(k_nb[sel], dw_nb[sel], m)
+    return dep, ddep
+
+
+class BatchDerivative:
+    """Traces batches on one geometry and frozen field with the derivative
+    along one direction and returns the face deposit and its derivative.
+
+    ``tracer`` is a HipTracer of the geometry; ``barrier`` = (phi, wface) for
+    the barrier wall model, else None."""
+
+    def __init__(self, tracer, p, Ex, Ey, barrier=None):
+        self.tracer, self.p = tracer, p
+        self.Ex, self.Ey, self.barrier = Ex, Ey, barrier
+        self.surface = tracer.surface_cells()
+
+    def __call__(self, particles, dstate=None, dfield=None):
+        x, y, vx, vy, qm, spc = particles
+        r = self.tracer.tangent(x, y, vx, vy, qm, spc, self.Ex, self.Ey, self.p, self.barrier, dstate, dfield)
+        dep, ddep = face_deposit(self.tracer.vac, self.surface, spc, r["status"], r["hit"], r["face"],
+                                 r["endpoints"], r["d_endpoints"], self.p.dx)
+        r.update(deposit=dep, d_deposit=ddep)
+        return r

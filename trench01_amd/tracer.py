@@ -286,6 +286,46 @@ class HipTracer:
         _lib.check(_lib.load().th_read_field(self._h, _lib.ptr(ex, _lib.dp), _lib.ptr(ey, _lib.dp)))
         return ex, ey
 
+    def tangent(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier=None, dstate=None, dfield=None):
+        """Record-mode FP64 trace with the pathwise derivative of every path
+        along one direction: ``dstate`` = (dx, dy, dvx, dvy) initial tangents
+        (entries may be None = 0), ``dfield`` = (dEx, dEy) field tangent or
+        None.  Returns dict(status, hit, steps, face, endpoints, d_endpoints,
+        final_velocity, d_final_velocity, escaped, lost, total_steps)."""
+        L = _lib.load()
+        n = len(x)
+        nx, ny = self.vac.shape
+        if not (0 < p.cfl <= 1) or p.max_steps < 1:
+            raise ValueError("HIP tracing needs 0 < cfl <= 1 and max_steps >= 1")
+        if any(np.shape(v) != (n,) for v in (x, y, vx, vy, qm, spc)):
+            raise ValueError("particle arrays differ in length")
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        self.set_fields(Ex, Ey, barrier)
+        xs = [f64(v) for v in (x, y, vx, vy, qm)]
+        spc = np.ascontiguousarray(spc, dtype=np.int8)
+        ds = [None] * 4 if dstate is None else [f64(v) for v in dstate]
+        if any(v is not None and v.shape != (n,) for v in ds):
+            raise ValueError("initial tangents must have one entry per particle")
+        dex, dey = (None, None) if dfield is None else (f64(v) for v in dfield)
+        if dex is not None and (dex.shape != (nx, ny) or dey.shape != (nx, ny)):
+            raise ValueError("field tangent shape does not match the geometry")
+        prm = self.params_for(p, barrier is not None)
+        prm.precision = 0
+        out = dict(status=np.empty(n, np.int8), hit=np.empty(n, np.int32), steps=np.empty(n, np.int32),
+                   face=np.empty(n, np.int8), endpoints=np.empty((n, 2)), d_endpoints=np.empty((n, 2)),
+                   final_velocity=np.empty((n, 2)), d_final_velocity=np.empty((n, 2)))
+        tally = _lib.Tally()
+        _lib.check(L.th_trace_tangent(self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n,
+                                      ctypes.byref(prm), *(_lib.ptr(v, _lib.dp) for v in ds),
+                                      _lib.ptr(dex, _lib.dp), _lib.ptr(dey, _lib.dp), ctypes.byref(tally),
+                                      _lib.ptr(out["status"], _lib.i8p), _lib.ptr(out["hit"], _lib.i32p),
+                                      _lib.ptr(out["steps"], _lib.i32p), _lib.ptr(out["face"], _lib.i8p),
+                                      *(_lib.ptr(out[k], _lib.dp) for k in ("endpoints", "d_endpoints",
+                                                                           "final_velocity", "d_final_velocity"))))
+        out.update(escaped=np.array([tally.escaped[0], tally.escaped[1]], dtype=np.int64), lost=int(tally.lost),
+                   total_steps=int(tally.steps))
+        return out
+
     def selftest_math(self, a, b):
         """Device hypot / sqrt / divide / fmod on arrays (for arithmetic pinning)."""
         a = np.ascontiguousarray(a, dtype=np.float64)
