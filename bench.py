@@ -72,6 +72,8 @@ def parse():
     ap.add_argument("--batching-batches", type=int, default=5)
     ap.add_argument("--batching-schedule", default=None, metavar="JSON",
                     help="ensemble schedule instead of the autotuner's pick, e.g. '{\"pass_steps\": 0}'")
+    ap.add_argument("--differentiation", action="store_true",
+                    help="also time the tangent push and compare pathwise derivatives with differences")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (traffic-counter calibration)")
     return ap.parse_args()
@@ -183,12 +185,42 @@ def main():
         out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
     if not a.no_steady:
         out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green)
+    if a.differentiation:
+        out["differentiation"] = differentiation(p, dev, rho0 if rho0 is not None else np.zeros(m), push_ms / a.steps)
     if a.batching > 0:
         dev.close()
         dev.tracer.close()
         out["batching"] = batching(a.batching, a.batching_batches, log,
                                    json.loads(a.batching_schedule) if a.batching_schedule else None)
     print(json.dumps(out), flush=True)
+
+
+def differentiation(p, dev, rho, push_ms):
+    """The pathwise derivative at the timed workload's pre-charged state: the
+    tangent push over one batch of it (1e6 particles, device-injected, single
+    pass, along the surface charge's own direction) next to the plain push,
+    and the agreement of pathwise derivatives with central differences of the
+    batch increment (tangent.derivative_agreement, 5e5 particles per batch)."""
+    import ctypes
+    from trench01_amd import _lib, tangent
+    n = 2 * p.n_per_batch
+    dev.inject_device(0, p.n_per_batch, p.seed, 0)
+    parts = dev.tracer.read_particles(0, n)
+    b = tangent.PathwiseBatch(p, seed=7)
+    phi, ex, ey = b.field(rho)
+    drho = np.zeros(b.geo.solid.size)
+    drho[b.surface] = rho
+    _, dex, dey = tangent.charge_direction(b.fields, b.geo.vac, p, drho.reshape(b.geo.solid.shape))
+    times = []
+    for _ in range(3):
+        b.tracer.tangent(*parts, ex, ey, p, (phi, b.fields.wface), None, (dex, dey))
+        ms = ctypes.c_double(0.0)
+        _lib.check(_lib.load().th_last_push_ms(b.tracer._h, ctypes.byref(ms)))
+        times.append(ms.value)
+    b.tracer.close()
+    agreement = tangent.derivative_agreement(replace(p, n_per_batch=250_000), rho)
+    return {"tangent_push_ms": min(times), "plain_push_ms": push_ms, "particles": n,
+            "kernel": "push64_tangent_kernel", "agreement": agreement}
 
 
 def batching(count, batches, log, schedule=None):

@@ -1,0 +1,71 @@
+"""torch.autograd over the device's pathwise batch derivative
+(trench01_amd.autodiff over tangent.PathwiseBatch): the forward value is the
+batch's face-smoothed charge increment, backward and forward-mode products
+use the tangent push, and the surface-charge backward (one tangent push per
+surface cell) is the transpose of the surface-charge JVP."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+import torch
+import torch.autograd.forward_ad as fwAD
+
+from trench01_amd import autodiff, model, tangent
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def small_params():
+    return replace(model.Params(), nx=30, trench_w=10, trench_d=20, floor_t=5, mask_t=6, n_vac=12, bias="dc",
+                   sigma_s=0.0, n_per_batch=3000, max_steps=1000, wall_model="barrier", mask_type="conductor",
+                   field_mode="floating", ion_distribution="mono", ion_energy_eV=100.0, ion_angle_deg=3.0)
+
+
+@pytest.fixture(scope="module")
+def batch():
+    b = tangent.PathwiseBatch(small_params(), seed=3)
+    rng = np.random.default_rng(8)
+    rho = np.where(b.cond_s, 0.0, rng.uniform(-2e7, 4e7, b.surface.size))
+    return b, rho
+
+
+def test_ion_increment_through_autograd(batch):
+    _case("autodiff ion increment")
+    b, rho = batch
+    inc, jac = b.ion_jacobian(rho, 3.0, 0.0)
+    assert np.abs(jac).max() > 0 and np.abs(inc).max() > 0
+    np.testing.assert_array_equal(b.increment(rho, 3.0, 0.0), inc)   # same random stream, same batch
+    theta = torch.tensor([3.0, 0.0], dtype=torch.float64, requires_grad=True)
+    out = autodiff.IonIncrement.apply(theta, b, rho)
+    np.testing.assert_array_equal(out.detach().numpy(), inc)
+    w = torch.tensor(np.random.default_rng(1).standard_normal(inc.size))
+    (out @ w).backward()
+    np.testing.assert_allclose(theta.grad.numpy(), jac.T @ w.numpy(), rtol=1e-12)
+    v = torch.tensor([0.5, -1.0], dtype=torch.float64)
+    with fwAD.dual_level():
+        t = fwAD.make_dual(torch.tensor([3.0, 0.0], dtype=torch.float64), v)
+        tan = fwAD.unpack_dual(autodiff.IonIncrement.apply(t, b, rho)).tangent
+    np.testing.assert_allclose(tan.numpy(), jac @ v.numpy(), rtol=1e-12)
+
+
+def test_charge_backward_is_the_transpose_of_the_charge_jvp(batch):
+    _case("autodiff charge adjoint identity")
+    b, rho = batch
+    rng = np.random.default_rng(5)
+    g = torch.tensor(rng.standard_normal(rho.size))
+    v = torch.tensor(np.where(b.cond_s, 0.0, rng.standard_normal(rho.size)) * 1e7)
+    r = torch.tensor(rho, requires_grad=True)
+    out = autodiff.ChargeIncrement.apply(r, b)
+    (out @ g).backward()
+    with fwAD.dual_level():
+        d = fwAD.make_dual(torch.tensor(rho), v)
+        jv = fwAD.unpack_dual(autodiff.ChargeIncrement.apply(d, b)).tangent
+    lhs = float(r.grad @ v)
+    rhs = float(g @ jv)
+    print(f"  <J^T g, v> = {lhs:.6e}, <g, J v> = {rhs:.6e}", flush=True)
+    assert abs(rhs) > 0
+    assert lhs == pytest.approx(rhs, rel=1e-9)

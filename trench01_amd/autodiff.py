@@ -1,0 +1,65 @@
+"""torch.autograd wrappers of the pathwise batch derivative.
+
+The device computes everything (tangent push, th_trace_tangent); torch only
+carries the result through autograd:
+
+* ``IonIncrement.apply(theta, batch, rho_s)``: the face-smoothed
+  surface-charge increment of ``batch`` (a tangent.PathwiseBatch) at the ion
+  distribution theta = (angular sigma [deg], energy shift [eV]); backward
+  (vector-Jacobian product) and forward-mode JVPs through the 2-column
+  Jacobian of two tangent pushes.
+* ``ChargeIncrement.apply(rho_s, batch)``: the increment as a function of the
+  surface charge; forward-mode JVPs by one tangent push along the tangent's
+  charge direction; backward by one tangent push per surface cell (exact,
+  m pushes).
+"""
+import numpy as np
+import torch
+
+
+class IonIncrement(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, batch, rho_s=None):
+        th = theta.detach().cpu().numpy().astype(float)
+        rho = None if rho_s is None else np.asarray(rho_s, dtype=float)
+        inc, jac = batch.ion_jacobian(rho, float(th[0]), float(th[1]))
+        jac_t = torch.as_tensor(jac, dtype=theta.dtype, device=theta.device)
+        ctx.save_for_backward(jac_t)
+        ctx.save_for_forward(jac_t)
+        return torch.as_tensor(inc, dtype=theta.dtype, device=theta.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        jac, = ctx.saved_tensors
+        return jac.T @ grad, None, None
+
+    @staticmethod
+    def jvp(ctx, dtheta, *_):
+        jac, = ctx.saved_tensors
+        return jac @ dtheta
+
+
+class ChargeIncrement(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho_s, batch):
+        ctx.batch = batch
+        ctx.rho = rho_s.detach().cpu().numpy().astype(float)
+        inc = batch.increment(ctx.rho)
+        return torch.as_tensor(inc, dtype=rho_s.dtype, device=rho_s.device)
+
+    @staticmethod
+    def jvp(ctx, drho, _):
+        d = drho.detach().cpu().numpy().astype(float)
+        dinc = ctx.batch.charge_jvp(ctx.rho, d)[1]
+        return torch.as_tensor(dinc, dtype=drho.dtype, device=drho.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        g = grad.detach().cpu().numpy().astype(float)
+        m = ctx.rho.size
+        out = np.empty(m)
+        for k in range(m):
+            e = np.zeros(m)
+            e[k] = 1.0
+            out[k] = ctx.batch.charge_jvp(ctx.rho, e)[1] @ g
+        return torch.as_tensor(out, dtype=grad.dtype, device=grad.device), None

@@ -177,3 +177,115 @@ class BatchDerivative:
                                  r["endpoints"], r["d_endpoints"], self.p.dx)
         r.update(deposit=dep, d_deposit=ddep)
         return r
+
+
+class PathwiseBatch:
+    """One batch of the charging loop at a given surface charge with its
+    random stream fixed (host injection, seed ``seed``): the face-smoothed
+    surface-charge increment [C/m^3 on the surface cells, conductor cells
+    zero] (the deposit of trench_charging_2d.py:1237-1243 with the face
+    deposit in place of the counts) and its derivatives along the ion
+    distribution (sigma, energy shift) and along the surface charge.
+
+    The state is the charge density on the surface cells, ``rho_s`` [m]; the
+    field follows the batch loop: phi = LU^-1(-rho / eps0) + phi_ext, E =
+    -grad(phi) (trench_charging_2d.py:1268-1269)."""
+
+    def __init__(self, p, seed=0, n=None, device=0):
+        from .charging import ChargingSetup
+        from .tracer import HipTracer
+        self.p = p
+        self.setup = setup = ChargingSetup(p)
+        self.geo = setup.geo
+        self.fields = setup.fields
+        self.tracer = HipTracer(self.geo.vac, device)
+        self.surface = self.tracer.surface_cells()
+        self.n = p.n_per_batch if n is None else int(n)
+        self.seed = int(seed)
+        # charge per traced particle: the batch's flux spread over the n traced particles per species
+        self.scale = setup.q_macro * p.n_per_batch / self.n / p.dx ** 2
+        self.cond_s = setup.cond_s.astype(bool)
+
+    def field(self, rho_s):
+        rho = np.zeros(self.geo.solid.size)
+        rho[self.surface] = rho_s
+        phi = self.fields.solve(-rho / EPS0).reshape(self.geo.solid.shape) + self.fields.external
+        ex, ey = model.compute_field(phi, self.geo.vac, self.p, self.fields.volts["top"])
+        return phi, ex, ey
+
+    def _trace(self, rho_s, sigma_deg, energy_shift, key=None, dfield=None):
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        parts, tan = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        barrier = (phi, self.fields.wface) if self.p.wall_model == "barrier" else None
+        dstate = None if key is None else (None, None, *tan[key])
+        bd = BatchDerivative(self.tracer, self.p, ex, ey, barrier)
+        r = bd(parts, dstate, dfield)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        dinc = (r["d_deposit"][0] - r["d_deposit"][1]) * self.scale
+        inc[self.cond_s] = 0.0
+        dinc[self.cond_s] = 0.0
+        return inc, dinc, r
+
+    def increment(self, rho_s=None, sigma_deg=None, energy_shift=0.0):
+        rho_s = np.zeros(self.surface.size) if rho_s is None else rho_s
+        return self._trace(rho_s, sigma_deg, energy_shift)[0]
+
+    def ion_jacobian(self, rho_s=None, sigma_deg=None, energy_shift=0.0):
+        """(increment [m], d increment / d (sigma [deg], energy shift [eV]) [m, 2])."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else rho_s
+        inc, d_sig, _ = self._trace(rho_s, sigma_deg, energy_shift, "sigma")
+        _, d_en, _ = self._trace(rho_s, sigma_deg, energy_shift, "energy")
+        return inc, np.column_stack((d_sig, d_en))
+
+    def charge_jvp(self, rho_s, drho_s, sigma_deg=None, energy_shift=0.0):
+        """(increment [m], d increment along the surface-charge direction drho_s [m])."""
+        drho = np.zeros(self.geo.solid.size)
+        drho[self.surface] = drho_s
+        _, dex, dey = charge_direction(self.fields, self.geo.vac, self.p, drho.reshape(self.geo.solid.shape))
+        inc, dinc, _ = self._trace(rho_s, sigma_deg, energy_shift, dfield=(dex, dey))
+        return inc, dinc
+
+
+def _cos(a, b):
+    na, nb = np.linalg.norm(a), np.linalg.norm(b)
+    return float(a @ b / (na * nb)) if na > 0 and nb > 0 else 0.0
+
+
+def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None, device=0):
+    """How well the pathwise derivative of the batch increment estimates the
+    derivative of its expectation, per direction (ion angular sigma, energy
+    shift, the surface charge's own direction rho_s): the cosine between the
+    pathwise derivatives of two random streams, between the central
+    differences (common random numbers, relative steps ``steps``) of the two
+    streams, and between pathwise derivative and difference, with their
+    norms.  The differences include the particles that change events; the
+    pathwise derivative leaves them out."""
+    batches = [PathwiseBatch(p, seed=s, n=n, device=device) for s in seeds]
+    sig = p.ion_angle_deg
+    e_mean = p.ion_energy_eV if p.ion_distribution == "mono" else 0.5 * (p.ion_energy_lo_eV + p.ion_energy_hi_eV)
+    out = {}
+    for name in ("sigma", "energy", "charge"):
+        pw, fds = [], []
+        for b in batches:
+            if name == "charge":
+                pw.append(b.charge_jvp(rho_s, rho_s)[1])
+                f = lambda e: b.increment(rho_s * (1.0 + e))
+                scale = 1.0
+            elif name == "sigma":
+                pw.append(b.ion_jacobian(rho_s, sig, 0.0)[1][:, 0])
+                f = lambda e: b.increment(rho_s, sig * (1.0 + e))
+                scale = sig
+            else:
+                pw.append(b.ion_jacobian(rho_s, sig, 0.0)[1][:, 1])
+                f = lambda e: b.increment(rho_s, sig, e * e_mean)
+                scale = e_mean
+            fds.append([(f(e) - f(-e)) / (2.0 * e * scale) for e in steps])
+        out[name] = dict(
+            pathwise_norm=float(np.linalg.norm(pw[0])), pathwise_seed_cosine=_cos(pw[0], pw[1]),
+            steps=list(steps), difference_norm=[float(np.linalg.norm(d)) for d in fds[0]],
+            difference_seed_cosine=[_cos(a, b) for a, b in zip(fds[0], fds[1])],
+            pathwise_difference_cosine=[_cos(pw[0] + pw[1], a + b) for a, b in zip(fds[0], fds[1])])
+    for b in batches:
+        b.tracer.close()
+    return out
