@@ -185,6 +185,12 @@ def main():
         out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
     if not a.no_steady:
         out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green)
+        if "cpu_baseline" in out:
+            # the plain run's batch count at the CPU push rate (push only; extrapolated, not run)
+            nb = out["time_to_steady"]["plain"]["batches"]
+            out["time_to_steady"]["plain_cpu_extrapolated"] = {
+                "wall_s": nb * n / out["cpu_baseline"]["value"], "cores": out["cpu_baseline"]["cores"],
+                "note": "extrapolated: plain batches x particles per batch / cpu_baseline rate (push only)"}
     if a.differentiation:
         out["differentiation"] = differentiation(p, dev, rho0 if rho0 is not None else np.zeros(m), push_ms / a.steps)
     if a.batching > 0:
@@ -340,9 +346,23 @@ def cpu_baseline(p, setup, parts, field, phi, threads):
         reps += 1
         steps += r["total_steps"]
     wall = time.perf_counter() - t0
+    # one thread, on every 64th particle of the same batch (same species mix and path lengths)
+    sub = [a[::64] for a in (x, y, vx, vy, qm, spc)]
+    reps1, steps1 = 0, 0
+    t1 = time.perf_counter()
+    while reps1 == 0 or time.perf_counter() - t1 < CPU_SAMPLE_S / 2:
+        r1 = trace_ref.trace_arrays(*sub, ex, ey, geo.vac, p.dx, p.cfl, p.max_steps, True,
+                                    (phi.reshape(geo.solid.shape), setup.fields.wface), threads=1)
+        reps1 += 1
+        steps1 += r1["total_steps"]
+    wall1 = time.perf_counter() - t1
     return {"value": reps * x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
             "sample": f"{reps} pass(es) over one whole batch ({x.size} particles, {r['total_steps']} "
-                      f"particle-steps each) of the timed workload, push only, {wall:.1f} s"}
+                      f"particle-steps each) of the timed workload, push only, {wall:.1f} s",
+            "single_thread": {"value": reps1 * sub[0].size / wall1, "unit": "particles/s", "cores": 1,
+                              "sample": f"{reps1} pass(es) over every 64th particle of the same batch "
+                                        f"({sub[0].size} particles, {r1['total_steps']} particle-steps each), "
+                                        f"{wall1:.1f} s"}}
 
 
 def time_to_steady(p, max_batches, log, push_precision=0, green=0):
