@@ -35,10 +35,14 @@ def _case(name):
     print(f"[gpu case] {name}", flush=True)
 
 
-@pytest.fixture(scope="module")
-def ar5():
-    """AR 5 trench with a charged barrier-model field, 4000 particles per species."""
-    p = cases.ar5_params()
+@pytest.fixture(scope="module", params=["ar5", "ar8"])
+def trench(request):
+    """AR 5 (mono 100 eV ions) and AR 8 (tapered walls, arcsine 40-260 eV ions)
+    trenches with a charged barrier-model field, 4000 particles per species."""
+    if request.param == "ar5":
+        p, ion = cases.ar5_params(), configs.AR5_ION
+    else:
+        p, ion = cases.ar8_params(), configs.AR8_ION
     c = cases.charged_case(p, n=10)
     solid, mask, i0, i1 = R.build_geometry(p)
     fields = R.setup_fields(p, solid, mask)
@@ -47,7 +51,6 @@ def ar5():
     rho[i1, p.floor_t:p.floor_t + p.trench_d] = -1e7
     dphi = fields["solve"]((-rho / R.EPS0).ravel()).reshape(solid.shape)
     dex, dey = tangent.linear_field(dphi, ~solid, p)
-    ion = configs.AR5_ION
     return dict(p=p, vac=c["vac"], Ex=c["Ex"], Ey=c["Ey"], phi=c["barrier"][0], wface=c["barrier"][1],
                 dphi=dphi, dEx=dex, dEy=dey, ion=ion, n=4000, seed=31)
 
@@ -70,12 +73,12 @@ def oracle_record(case, parts, Ex, Ey, phi):
                                   (phi, case["wface"]), record=True, probe_trajectories=1)
 
 
-def test_tangent_primal_is_the_record_trace(ar5):
+def test_tangent_primal_is_the_record_trace(trench):
     _case("tangent primal")
-    parts, tan = draw(ar5)
-    t = HipTracer(ar5["vac"])
-    r = t.tangent(*parts, ar5["Ex"], ar5["Ey"], ar5["p"], (ar5["phi"], ar5["wface"]), dstate=(None, None, *tan["sigma"]))
-    ref = oracle_record(ar5, parts, ar5["Ex"], ar5["Ey"], ar5["phi"])
+    parts, tan = draw(trench)
+    t = HipTracer(trench["vac"])
+    r = t.tangent(*parts, trench["Ex"], trench["Ey"], trench["p"], (trench["phi"], trench["wface"]), dstate=(None, None, *tan["sigma"]))
+    ref = oracle_record(trench, parts, trench["Ex"], trench["Ey"], trench["phi"])
     np.testing.assert_array_equal(r["status"], ref["status"])
     np.testing.assert_array_equal(r["hit"], ref["hit"])
     np.testing.assert_array_equal(r["steps"], ref["steps"])
@@ -107,13 +110,13 @@ def _directions(case):
 
 
 @pytest.mark.parametrize("direction", ["sigma", "energy", "charge"])
-def test_tangent_matches_central_differences(ar5, direction):
+def test_tangent_matches_central_differences(trench, direction):
     _case(f"tangent vs differences: {direction}")
-    states, flds, step, kw = _directions(ar5)[direction]
-    t = HipTracer(ar5["vac"])
-    p = ar5["p"]
-    r = t.tangent(*states[1], ar5["Ex"], ar5["Ey"], p, (ar5["phi"], ar5["wface"]), **kw)
-    lo, hi = (oracle_record(ar5, s, *f) for s, f in ((states[0], flds[0]), (states[2], flds[2])))
+    states, flds, step, kw = _directions(trench)[direction]
+    t = HipTracer(trench["vac"])
+    p = trench["p"]
+    r = t.tangent(*states[1], trench["Ex"], trench["Ey"], p, (trench["phi"], trench["wface"]), **kw)
+    lo, hi = (oracle_record(trench, s, *f) for s, f in ((states[0], flds[0]), (states[2], flds[2])))
     same = np.ones(r["status"].size, bool)
     for ref in (lo, hi):
         same &= (ref["status"] == r["status"]) & (ref["hit"] == r["hit"]) & (ref["steps"] == r["steps"])
@@ -162,17 +165,17 @@ def test_field_free_floor_crossing_closed_form():
     assert np.abs(re_["d_endpoints"][floor]).max() <= 1e-12 * np.abs(expect).max()
 
 
-def test_face_deposit_derivative_matches_its_differences(ar5):
+def test_face_deposit_derivative_matches_its_differences(trench):
     """d(face deposit)/d(theta) of the tangent push against the central
     difference of the face deposit over the particles that keep their events
     (device primal at theta +- h)."""
     _case("face deposit derivative")
-    states, flds, step, kw = _directions(ar5)["sigma"]
-    t = HipTracer(ar5["vac"])
-    p = ar5["p"]
-    bd = tangent.BatchDerivative(t, p, ar5["Ex"], ar5["Ey"], (ar5["phi"], ar5["wface"]))
+    states, flds, step, kw = _directions(trench)["sigma"]
+    t = HipTracer(trench["vac"])
+    p = trench["p"]
+    bd = tangent.BatchDerivative(t, p, trench["Ex"], trench["Ey"], (trench["phi"], trench["wface"]))
     mid = bd(states[1], **kw)
-    side = [t.tangent(*s, *f[:2], p, (f[2], ar5["wface"])) for s, f in ((states[0], flds[0]), (states[2], flds[2]))]
+    side = [t.tangent(*s, *f[:2], p, (f[2], trench["wface"])) for s, f in ((states[0], flds[0]), (states[2], flds[2]))]
     same = mid["status"] != 3
     for r in side:
         same &= (r["status"] == mid["status"]) & (r["hit"] == mid["hit"]) & (r["steps"] == mid["steps"]) & \
@@ -182,7 +185,7 @@ def test_face_deposit_derivative_matches_its_differences(ar5):
 
     def dep(r, keep):
         st = np.where(keep, r["status"], 0).astype(np.int8)
-        return tangent.face_deposit(ar5["vac"], surf, spc, st, r["hit"], r["face"], r["endpoints"],
+        return tangent.face_deposit(trench["vac"], surf, spc, st, r["hit"], r["face"], r["endpoints"],
                                     r["d_endpoints"], p.dx)
     fd = (dep(side[1], same)[0] - dep(side[0], same)[0]) / step
     tg = dep(mid, same)[1]
