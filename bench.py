@@ -223,7 +223,7 @@ def differentiation(p, dev, rho, push_ms):
         ms = ctypes.c_double(0.0)
         _lib.check(_lib.load().th_last_push_ms(b.tracer._h, ctypes.byref(ms)))
         times.append(ms.value)
-    b.tracer.close()
+    b.close()
     agreement = tangent.derivative_agreement(replace(p, n_per_batch=250_000), rho)
     return {"tangent_push_ms": min(times), "plain_push_ms": push_ms, "particles": n,
             "kernel": "push64_tangent_kernel", "agreement": agreement}

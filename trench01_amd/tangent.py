@@ -206,6 +206,10 @@ class PathwiseBatch:
         self.scale = setup.q_macro * p.n_per_batch / self.n / p.dx ** 2
         self.cond_s = setup.cond_s.astype(bool)
 
+    def close(self):
+        """Releases the tracer's device memory."""
+        self.tracer.close()
+
     def field(self, rho_s):
         rho = np.zeros(self.geo.solid.size)
         rho[self.surface] = rho_s
@@ -287,5 +291,5 @@ def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None
             difference_seed_cosine=[_cos(a, b) for a, b in zip(fds[0], fds[1])],
             pathwise_difference_cosine=[_cos(pw[0] + pw[1], a + b) for a, b in zip(fds[0], fds[1])])
     for b in batches:
-        b.tracer.close()
+        b.close()
     return out
