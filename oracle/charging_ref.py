@@ -15,7 +15,7 @@ voltages frozen at one phase) and "circuit_rf" (the push follows one RF period
 of voltages, launch phases drawn per batch); side boundaries periodic, fixed
 and pillar; no resume.
 """
-from dataclasses import dataclass, replace
+from dataclasses import dataclass
 
 import numpy as np
 import scipy.sparse as sps

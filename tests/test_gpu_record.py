@@ -6,7 +6,6 @@ trace_*.npz): status, hit cells, end points clipped to the crossed face,
 final velocities and probe paths.  The FP32 push is a precision variant: its
 tallies agree with the FP64 push statistically, not bit for bit."""
 import threading
-from dataclasses import replace
 
 import numpy as np
 import pytest
