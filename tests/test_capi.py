@@ -57,3 +57,26 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_every_entry_point_rejects_null_arguments():
+    """Called with NULL handles and pointers and zero sizes, every entry point
+    returns an error code with a message (the destroy functions return
+    quietly) -- before any GPU work, so this runs without a GPU."""
+    L = _lib.load()
+    for name, (res, args) in _lib.SIGNATURES.items():
+        if name == "th_last_error":
+            continue
+        call = []
+        for a in args:
+            if a in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64):
+                call.append(0)
+            elif a is ctypes.c_double:
+                call.append(0.0)
+            else:
+                call.append(None)
+        rc = getattr(L, name)(*call)
+        if res is None:
+            continue
+        assert rc != 0, name
+        assert L.th_last_error(), name
