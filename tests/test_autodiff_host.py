@@ -1,6 +1,7 @@
 """autograd plumbing of trench01_amd.autodiff with a stand-in batch whose
 Jacobian is known (the device batch is covered by test_gpu_autodiff.py)."""
 import numpy as np
+import pytest
 import torch
 import torch.autograd.forward_ad as fwAD
 
@@ -53,3 +54,27 @@ def test_charge_increment_backward_is_the_transpose_of_its_jvp():
         d = fwAD.make_dual(rho.detach(), v)
         tangent = fwAD.unpack_dual(autodiff.ChargeIncrement.apply(d, b)).tangent
     np.testing.assert_allclose(tangent.numpy(), b.B @ v.numpy(), rtol=1e-12)
+
+
+class ScoreBatch:
+    """increment = a sigma, gradient by the (here exact) likelihood ratio a."""
+
+    def __init__(self, m=4):
+        self.a = np.arange(1.0, m + 1.0)
+
+    def sigma_score_gradient(self, rho, sigma, shift):
+        return self.a * sigma, self.a.copy()
+
+
+def test_sigma_increment_backward_and_jvp():
+    b = ScoreBatch()
+    sigma = torch.tensor(3.0, dtype=torch.float64, requires_grad=True)
+    out = autodiff.SigmaIncrement.apply(sigma, b)
+    np.testing.assert_allclose(out.detach().numpy(), 3.0 * b.a)
+    w = torch.tensor([1.0, -1.0, 2.0, 0.5], dtype=torch.float64)
+    (out @ w).backward()
+    assert float(sigma.grad) == pytest.approx(float(b.a @ w.numpy()))
+    with fwAD.dual_level():
+        d = fwAD.make_dual(torch.tensor(3.0, dtype=torch.float64), torch.tensor(0.25, dtype=torch.float64))
+        tan = fwAD.unpack_dual(autodiff.SigmaIncrement.apply(d, b)).tangent
+    np.testing.assert_allclose(tan.numpy(), 0.25 * b.a)

@@ -163,3 +163,34 @@ def test_face_deposit_skips_live_and_escaped_particles():
     face = np.array([0, 0, -1], np.int8)
     dep, ddep = tangent.face_deposit(vac, surface, np.zeros(3, np.int8), st, hit, face, end, dend, 1.0)
     assert dep.sum() == pytest.approx(1.0)
+
+
+def test_sigma_score_is_the_derivative_of_the_angle_log_density():
+    p, _ = configs.product_params("AR8")
+    s = tangent.ion_sampler(p)
+    s(np.random.default_rng(3), 20000)
+    score = s.sigma_score()
+    sig, h = p.ion_angle_deg, 1e-5
+
+    def logpdf(sd):
+        r = np.radians(sd)
+        return -np.log(r) - s.theta ** 2 / (2 * r * r)
+    fd = (logpdf(sig + h) - logpdf(sig - h)) / (2 * h)
+    np.testing.assert_allclose(score, fd, rtol=1e-6, atol=1e-9)
+    assert abs(score.mean()) < 5 * score.std() / np.sqrt(score.size)   # zero mean
+
+
+def test_face_deposit_weights_scale_each_contribution():
+    vac, surface = _block()
+    rec = _record(vac, [((3, 2), False, (3.8, 3.0), (1.0, 0.0)), ((2, 1), True, (2.0, 1.2), (0.0, 2.0))])
+    spc = np.zeros(2, np.int8)
+    d1, dd1 = tangent.face_deposit(vac, surface, spc, *rec, 1.0)
+    d2, dd2 = tangent.face_deposit(vac, surface, spc, *rec, 1.0, weights=np.ones(2))
+    np.testing.assert_array_equal(d1, d2)
+    np.testing.assert_array_equal(dd1, dd2)
+    w = np.array([2.0, -0.5])
+    dw, ddw = tangent.face_deposit(vac, surface, spc, *rec, 1.0, weights=w)
+    a = tangent.face_deposit(vac, surface, spc, *(x[:1] for x in rec), 1.0)
+    b = tangent.face_deposit(vac, surface, spc[1:], *(x[1:] for x in rec), 1.0)
+    np.testing.assert_allclose(dw, 2.0 * a[0] - 0.5 * b[0], atol=1e-12)
+    np.testing.assert_allclose(ddw, 2.0 * a[1] - 0.5 * b[1], atol=1e-12)

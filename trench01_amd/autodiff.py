@@ -12,6 +12,10 @@ carries the result through autograd:
   surface charge; forward-mode JVPs by one tangent push along the tangent's
   charge direction; backward by one tangent push per surface cell (exact,
   m pushes).
+* ``SigmaIncrement.apply(sigma, batch, rho_s)``: the increment as a function
+  of the ion angular sigma [deg], differentiated by the likelihood ratio
+  (batch.sigma_score_gradient): an unbiased estimate of the derivative of
+  the expected increment, from the same trace as the forward value.
 """
 import numpy as np
 import torch
@@ -63,3 +67,25 @@ class ChargeIncrement(torch.autograd.Function):
             e[k] = 1.0
             out[k] = ctx.batch.charge_jvp(ctx.rho, e)[1] @ g
         return torch.as_tensor(out, dtype=grad.dtype, device=grad.device), None
+
+
+class SigmaIncrement(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sigma, batch, rho_s=None):
+        rho = None if rho_s is None else np.asarray(rho_s, dtype=float)
+        inc, grad = batch.sigma_score_gradient(rho, float(sigma.detach().cpu().reshape(-1)[0]), 0.0)
+        g = torch.as_tensor(grad, dtype=sigma.dtype, device=sigma.device)
+        ctx.save_for_backward(g)
+        ctx.save_for_forward(g)
+        ctx.shape = sigma.shape
+        return torch.as_tensor(inc, dtype=sigma.dtype, device=sigma.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        g, = ctx.saved_tensors
+        return (g @ grad).reshape(ctx.shape), None, None
+
+    @staticmethod
+    def jvp(ctx, dsigma, *_):
+        g, = ctx.saved_tensors
+        return g * dsigma.reshape(-1)[0]

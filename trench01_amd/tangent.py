@@ -69,7 +69,15 @@ class IonSampler:
         dtheta = theta / self.sigma_deg if self.sigma_deg else np.zeros(n)   # theta = sigma * z
         dv = E_CHARGE / (self.m_i * v)                                      # dv/dE
         self.tangents = {"sigma": (v * c * dtheta, v * s * dtheta), "energy": (s * dv, -c * dv)}
+        self.theta = theta
         return v * s, -v * c
+
+    def sigma_score(self):
+        """Score of the last draw's polar angles with respect to sigma [per
+        degree]: d/dsigma log N(theta; 0, sigma) = (theta^2 / sigma^2 - 1) / sigma
+        (sigma in radians, times the radians per degree)."""
+        s = np.radians(self.sigma_deg)
+        return (self.theta ** 2 / s ** 2 - 1.0) / s * np.radians(1.0)
 
 
 def ion_sampler(p, energy_shift=0.0, sigma_deg=None):
@@ -111,10 +119,11 @@ def charge_direction(fields, vac, p, drho):
     return dphi, dex, dey
 
 
-def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx):
+def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx, weights=None):
     """Face-smoothed deposit per species on the surface cells and its
     derivative: (dep, ddep), each [2, m] in particles (``surface`` = flat
-    indices of the surface cells, ascending)."""
+    indices of the surface cells, ascending).  ``weights`` (per particle)
+    scale each particle's contribution to both."""
     vac = np.asarray(vac, dtype=bool)
     nx, ny = vac.shape
     m = surface.size
@@ -130,6 +139,7 @@ def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, d
     xf = face[a] == 1
     e, de = endpoints[a], d_endpoints[a]
     sp = np.asarray(spc)[a].astype(np.int64)
+    wt = np.ones(a.size) if weights is None else np.asarray(weights, dtype=float)[a]
     # coordinate along the face and the side of the face the particle came from
     along = np.where(xf, e[:, 1] / dx - cy, np.mod(e[:, 0] / dx - cx, nx))
     across = np.where(xf, np.mod(e[:, 0] / dx - cx, nx), e[:, 1] / dx - cy)
@@ -153,8 +163,8 @@ def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, d
     k_hit, k_nb = smap[c], smap[np.where(valid, nb, c)]
     for s in (0, 1):
         sel = sp == s
-        dep[s] = np.bincount(k_hit[sel], 1.0 - w_nb[sel], m) + np.bincount(k_nb[sel], w_nb[sel], m)
-        ddep[s] = np.bincount(k_hit[sel], -dw_nb[sel], m) + np.bincount(k_nb[sel], dw_nb[sel], m)
+        dep[s] = np.bincount(k_hit[sel], wt[sel] * (1.0 - w_nb[sel]), m) + np.bincount(k_nb[sel], wt[sel] * w_nb[sel], m)
+        ddep[s] = np.bincount(k_hit[sel], -wt[sel] * dw_nb[sel], m) + np.bincount(k_nb[sel], wt[sel] * dw_nb[sel], m)
     return dep, ddep
 
 
@@ -242,6 +252,30 @@ class PathwiseBatch:
         _, d_en, _ = self._trace(rho_s, sigma_deg, energy_shift, "energy")
         return inc, np.column_stack((d_sig, d_en))
 
+    def sigma_score_gradient(self, rho_s=None, sigma_deg=None, energy_shift=0.0):
+        """Likelihood-ratio (score-function) estimate of the derivative of the
+        expected increment with respect to the ion angular sigma [per degree],
+        from one trace: each ion's face deposit weighted by the score of its
+        polar angle (IonSampler.sigma_score), whose mean is zero.  Unbiased:
+        unlike the pathwise derivative it includes the particles that change
+        events.  Returns (increment [m], d E[increment] / d sigma [m])."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else rho_s
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        parts, _ = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        barrier = (phi, self.fields.wface) if self.p.wall_model == "barrier" else None
+        bd = BatchDerivative(self.tracer, self.p, ex, ey, barrier)
+        r = bd(parts)
+        score = np.zeros(parts[0].size)
+        score[:self.n] = sampler.sigma_score()   # ions first; electrons do not depend on sigma
+        dep, _ = face_deposit(self.tracer.vac, self.surface, parts[5], r["status"], r["hit"], r["face"],
+                              r["endpoints"], r["d_endpoints"], self.p.dx, weights=score)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        grad = (dep[0] - dep[1]) * self.scale
+        inc[self.cond_s] = 0.0
+        grad[self.cond_s] = 0.0
+        return inc, grad
+
     def charge_jvp(self, rho_s, drho_s, sigma_deg=None, energy_shift=0.0):
         """(increment [m], d increment along the surface-charge direction drho_s [m])."""
         drho = np.zeros(self.geo.solid.size)
@@ -259,20 +293,25 @@ def _cos(a, b):
 def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None, device=0):
     """How well the pathwise derivative of the batch increment estimates the
     derivative of its expectation, per direction (ion angular sigma, energy
-    shift, the surface charge's own direction rho_s): the cosine between the
-    pathwise derivatives of two random streams, between the central
-    differences (common random numbers, relative steps ``steps``) of the two
-    streams, and between pathwise derivative and difference, with their
-    norms.  The differences include the particles that change events; the
-    pathwise derivative leaves them out."""
+    shift, the surface charge's own direction rho_s), and the likelihood-ratio
+    estimate for sigma ("sigma_score"): the cosine between the estimates of
+    two random streams, between the central differences (common random
+    numbers, relative steps ``steps``) of the two streams, and between
+    estimate and difference, with their norms.  The differences include the
+    particles that change events; the pathwise derivative leaves them out,
+    the likelihood ratio does not."""
     batches = [PathwiseBatch(p, seed=s, n=n, device=device) for s in seeds]
     sig = p.ion_angle_deg
     e_mean = p.ion_energy_eV if p.ion_distribution == "mono" else 0.5 * (p.ion_energy_lo_eV + p.ion_energy_hi_eV)
     out = {}
-    for name in ("sigma", "energy", "charge"):
+    for name in ("sigma", "sigma_score", "energy", "charge"):
         pw, fds = [], []
         for b in batches:
-            if name == "charge":
+            if name == "sigma_score":
+                pw.append(b.sigma_score_gradient(rho_s, sig, 0.0)[1])
+                f = lambda e: b.increment(rho_s, sig * (1.0 + e))
+                scale = sig
+            elif name == "charge":
                 pw.append(b.charge_jvp(rho_s, rho_s)[1])
                 f = lambda e: b.increment(rho_s * (1.0 + e))
                 scale = 1.0
@@ -286,10 +325,10 @@ def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None
                 scale = e_mean
             fds.append([(f(e) - f(-e)) / (2.0 * e * scale) for e in steps])
         out[name] = dict(
-            pathwise_norm=float(np.linalg.norm(pw[0])), pathwise_seed_cosine=_cos(pw[0], pw[1]),
+            estimate_norm=float(np.linalg.norm(pw[0])), estimate_seed_cosine=_cos(pw[0], pw[1]),
             steps=list(steps), difference_norm=[float(np.linalg.norm(d)) for d in fds[0]],
             difference_seed_cosine=[_cos(a, b) for a, b in zip(fds[0], fds[1])],
-            pathwise_difference_cosine=[_cos(pw[0] + pw[1], a + b) for a, b in zip(fds[0], fds[1])])
+            estimate_difference_cosine=[_cos(pw[0] + pw[1], a + b) for a, b in zip(fds[0], fds[1])])
     for b in batches:
         b.close()
     return out
