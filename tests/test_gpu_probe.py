@@ -1,0 +1,117 @@
+"""The vertical-ion diagnostic on the device (trench01_amd.probe), against the
+reference's own known answers for it (tests/test_features.py:135-171,
+tests/test_taper.py:132-156) and against the oracle's record-mode C push on
+the same beam and field."""
+import threading
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from oracle import trace_ref
+from tests.test_probe_host import small_params, steady_result
+from trench01_amd import model, probe
+from trench01_amd.tracer import tapered_of
+
+pytestmark = pytest.mark.gpu
+
+TR_ANGLE = np.degrees(np.arctan(0.25))
+MASK_ANGLE = np.degrees(np.arctan(0.5))
+
+
+def _case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def taper_params(**changes):
+    """The reference's tests/test_taper.py:19-24."""
+    base = replace(model.Params(), nx=40, trench_w=16, trench_d=20, floor_t=4, mask_t=12, n_vac=10,
+                   trench_taper_deg=TR_ANGLE, mask_taper_deg=MASK_ANGLE, bias="dc", sigma_s=0, n_batches=3,
+                   n_per_batch=256, max_steps=800, probe_ions=224, probe_trajectories=12)
+    return replace(base, **changes)
+
+
+def test_vertical_probe_is_exactly_vertical_and_does_not_charge():
+    _case("probe vertical")
+    p = small_params()
+    res = steady_result(p)
+    before = res["rho_raw"].copy()
+    d = probe.probe_vertical_ions(res, log=lambda _: None)
+    np.testing.assert_array_equal(before, res["rho_raw"])
+    assert d["backend_used"] == "hip"
+    for label in ("charged", "uncharged"):
+        r = d[label]
+        assert r["bottom_fraction"] == 1 and r["lost"] == 0 and r["escaped"] == 0
+        assert len(r["paths"]) == p.probe_trajectories
+        for points in r["paths"]:
+            np.testing.assert_array_equal(points[:, 0], np.full(len(points), points[0, 0]))
+            np.testing.assert_allclose(points[-1, 1], p.floor_t * p.dx, atol=1e-18)
+        np.testing.assert_array_equal(r["angle_deg"], np.zeros(p.probe_ions))
+        np.testing.assert_allclose(r["bottom_flux"], p.flux)
+
+
+def test_charge_deflects_vertical_ions_and_preserves_particle_accounting():
+    _case("probe deflection")
+    p = small_params()
+    res = steady_result(p)
+    res["rho_raw"][res["i0"] - 1, p.floor_t:p.floor_t + p.trench_d] = 1e7
+    before = res["rho_raw"].copy()
+    d = probe.probe_vertical_ions(res, log=lambda _: None)
+    np.testing.assert_array_equal(res["rho_raw"], before)
+    charged = d["charged"]
+    assert np.any(np.abs(charged["angle_deg"]) > 0.1)
+    assert charged["hits"].sum() + charged["escaped"] + charged["lost"] == p.probe_ions
+    assert d["uncharged"]["bottom_fraction"] == 1
+    # the same beam through the same field, by the oracle's record-mode push
+    geo = model.Geometry(p)
+    i0, i1 = d["entrance_i0"], d["entrance_i1"]
+    n = p.probe_ions
+    x = (i0 + (np.arange(n) + 0.5) * (i1 - i0) / n) * p.dx
+    y = np.full(n, (geo.ny - 1e-3) * p.dx)
+    vy = np.full(n, -np.sqrt(2 * p.probe_energy_eV * model.E_CHARGE / (p.ion_mass_amu * model.AMU)))
+    qm = np.full(n, model.E_CHARGE / (p.ion_mass_amu * model.AMU))
+    wface = model.Fields(p, geo).wface
+    ref = trace_ref.trace_arrays(x, y, np.zeros(n), vy, qm, np.zeros(n, np.int8), charged["Ex"], charged["Ey"],
+                                 geo.vac, p.dx, p.cfl, p.max_steps, tapered_of(p), (charged["phi"], wface),
+                                 record=True, probe_trajectories=p.probe_trajectories)
+    np.testing.assert_array_equal(charged["status"], ref["status"])
+    np.testing.assert_array_equal(charged["endpoints"], ref["endpoints"])
+    np.testing.assert_array_equal(charged["final_velocity"], ref["final_velocity"])
+    np.testing.assert_array_equal(charged["hits"], ref["cnt_i"])
+    for a, b in zip(charged["paths"], ref["paths"]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_probe_supports_cancellation():
+    _case("probe cancel")
+    p = small_params()
+    stop = threading.Event()
+    stop.set()
+    d = probe.probe_vertical_ions(steady_result(p), log=lambda _: None, stop=stop)
+    assert d["cancelled"]
+
+
+@pytest.mark.parametrize("trench_angle,mask_angle,mask_t,bottom_width,entrance_width,fraction", [
+    (TR_ANGLE, MASK_ANGLE, 12, 6, 28, 6 / 28),
+    (-TR_ANGLE, MASK_ANGLE, 12, 26, 28, 16 / 28),
+    (TR_ANGLE, -MASK_ANGLE, 12, 6, 4, 1),
+    (TR_ANGLE, 0, 0, 6, 16, 6 / 16)])
+def test_vertical_beam_matches_ballistic_aperture_and_flux_conservation(
+        trench_angle, mask_angle, mask_t, bottom_width, entrance_width, fraction):
+    _case(f"probe aperture {trench_angle:.1f} {mask_angle:.1f} {mask_t}")
+    p = taper_params(trench_taper_deg=trench_angle, mask_taper_deg=mask_angle, mask_t=mask_t,
+                     probe_ions=entrance_width * 8)
+    res = steady_result(p)
+    before = res["rho_raw"].copy()
+    d = probe.probe_vertical_ions(res, log=lambda _: None)
+    np.testing.assert_array_equal(res["rho_raw"], before)
+    for label in ("charged", "uncharged"):
+        r = d[label]
+        assert r["bottom_fraction"] == fraction
+        assert r["escaped"] == r["lost"] == 0
+        assert r["hits"].sum() == p.probe_ions
+        assert len(r["bottom_flux"]) == bottom_width
+        assert r["bottom_flux"].sum() / p.flux == pytest.approx(entrance_width * fraction)
+        for path in r["paths"]:
+            np.testing.assert_array_equal(path[:, 0], np.full(len(path), path[0, 0]))
+        np.testing.assert_array_equal(r["angle_deg"], np.zeros(p.probe_ions))
