@@ -115,3 +115,30 @@ def test_vertical_beam_matches_ballistic_aperture_and_flux_conservation(
         for path in r["paths"]:
             np.testing.assert_array_equal(path[:, 0], np.full(len(path), path[0, 0]))
         np.testing.assert_array_equal(r["angle_deg"], np.zeros(p.probe_ions))
+
+
+def test_rf_cycle_probe_survives_the_result_file(tmp_path):
+    """The reference's tests/test_rf_cycle.py:221-231 on the reference's own
+    circuit_rf result file: the diagnostic over the RF cycle is saved with the
+    result, read back with its launch phases and trajectories, and running it
+    again on the restored result gives the same hits."""
+    _case("probe rf round trip")
+    from pathlib import Path
+    from trench01_amd import results
+    res, p = results.load_results(Path(__file__).resolve().parent / "golden" / "result_circuit_rf.npz")
+    res["t_conv"] = 1e-6            # take the stored charge as the saturated state
+    p = replace(p, probe_ions=400, probe_trajectories=6, probe_energy_eV=1.5)
+    res["ion_probe"] = probe.probe_vertical_ions(res, p, log=lambda _: None)
+    assert res["ion_probe"]["rf_cycle"]
+    restored, q = results.load_results(results.save_results(res, str(tmp_path / "rf_cycle")))
+    assert q.field_mode == "circuit_rf" and restored["ion_probe"]["rf_cycle"]
+    a, b = restored["ion_probe"]["charged"], res["ion_probe"]["charged"]
+    np.testing.assert_array_equal(a["launch_phase"], b["launch_phase"])
+    np.testing.assert_array_equal(a["hits"], b["hits"])
+    assert len(a["paths"]) == len(b["paths"]) == 6
+    for x, y in zip(a["paths"], b["paths"]):
+        np.testing.assert_array_equal(x, y)
+    restored["t_conv"] = res["t_conv"]   # the short stored history fails the steady re-check on loading
+    again = probe.probe_vertical_ions(restored, p, log=lambda _: None)
+    np.testing.assert_array_equal(again["charged"]["hits"], b["hits"])
+    np.testing.assert_array_equal(again["uncharged"]["hits"], res["ion_probe"]["uncharged"]["hits"])

@@ -49,7 +49,8 @@ def probe_vertical_ions(res, p=None, phase=None, log=print, stop=None):
     if geo.solid.shape != solid.shape or not np.array_equal(geo.solid, solid):
         raise ValueError("the result's geometry does not match its parameters")
     fields = model.Fields(p, geo, res.get("circuit"))
-    tracer, backend_used = make_particle_tracer(p, vac, log)
+    # the diagnostic runs on the device whichever backend computed the stored charge
+    tracer, backend_used = make_particle_tracer(replace(p, backend="hip"), vac, log)
     ny = solid.shape[1]
     i0, i1 = geo.opening_span(p.floor_t + p.trench_d + p.mask_t - 1)
     b0, b1 = geo.opening_span(p.floor_t)
