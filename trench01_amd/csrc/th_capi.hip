@@ -857,7 +857,14 @@ int th_charging_set_green(th_charging *c, int32_t mode) {
     if (mode != th::kGreenF64 && c->green_built != mode) {
         size_t count = (size_t)d.n_rows * d.n_pad;
         int rc;
-        if (!c->g_hi && ((rc = dev_alloc(&c->g_hi, count)) || (rc = dev_alloc(&c->g_lo, count)))) return rc;
+        if (!c->g_hi || !c->g_lo) {   // both halves or neither: a failed allocation leaves no half behind
+            if ((rc = dev_alloc(&c->g_hi, count)) || (rc = dev_alloc(&c->g_lo, count))) {
+                if (c->g_hi) (void)hipFree(c->g_hi);
+                c->g_hi = nullptr;
+                c->g_lo = nullptr;
+                return rc;
+            }
+        }
         int blocks = (int)((count + 255) / 256);
         if (mode == th::kGreenBF16)
             hipLaunchKernelGGL(th::green_split_kernel<th::kGreenBF16>, dim3(blocks), dim3(256), 0, t->stream, c->green,
@@ -1332,10 +1339,6 @@ extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const 
         if (bottom_s[j] < 0 || bottom_s[j] >= t->m) return fail(TH_EINVAL, "floor index out of range");
     TH_HIP(hipSetDevice(t->device));
     TH_HIP(hipStreamSynchronize(t->stream));
-    if (c->mon_buf) (void)hipFree(c->mon_buf);
-    if (c->mon_out) (void)hipFree(c->mon_out);
-    c->mon_buf = nullptr;
-    c->mon_out = nullptr;
     size_t total = (size_t)n_probes + n_regions + 1 + n_idx + n_bottom;
     std::vector<int32_t> buf(std::max<size_t>(total, 1));
     size_t o = 0;
@@ -1346,12 +1349,25 @@ extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const 
     if (n_idx) memcpy(buf.data() + o_idx, region_idx, sizeof(int32_t) * n_idx);
     size_t o_bot = o_idx + n_idx;
     if (n_bottom) memcpy(buf.data() + o_bot, bottom_s, sizeof(int32_t) * n_bottom);
-    int rc;
-    if ((rc = dev_alloc(&c->mon_buf, buf.size()))) return rc;
-    size_t n_out = (size_t)n_probes + n_regions + 3;
-    if ((rc = dev_alloc(&c->mon_out, n_out))) return rc;
-    TH_HIP(hipMemcpy(c->mon_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
-    TH_HIP(hipMemset(c->mon_out, 0, sizeof(double) * n_out));
+    // the new buffers replace the old ones only once they are complete: on an error the handle keeps its
+    // previous monitors
+    int32_t *mon_buf = nullptr;
+    double *mon_out = nullptr;
+    const size_t n_out = (size_t)n_probes + n_regions + 3;
+    int rc = dev_alloc(&mon_buf, buf.size());
+    if (!rc) rc = dev_alloc(&mon_out, n_out);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(mon_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemset(mon_out, 0, sizeof(double) * n_out);
+    if (rc || e != hipSuccess) {
+        if (mon_buf) (void)hipFree(mon_buf);
+        if (mon_out) (void)hipFree(mon_out);
+        return rc ? rc : fail(TH_EHIP, "monitor upload: %s", hipGetErrorString(e));
+    }
+    if (c->mon_buf) (void)hipFree(c->mon_buf);
+    if (c->mon_out) (void)hipFree(c->mon_out);
+    c->mon_buf = mon_buf;
+    c->mon_out = mon_out;
     c->mon.n_probes = n_probes;
     c->mon.n_regions = n_regions;
     c->mon.abs_region = abs_region;
