@@ -304,6 +304,20 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
  * counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, double *ms);
 
+/* Solves L U X = B in FP64 on `device` for k right-hand sides: B and X are
+ * n-by-k, row-major, on the host.  L is unit lower triangular, given by its
+ * strictly lower part in CSR (l_ptr[n+1], l_idx, l_val); U is upper
+ * triangular, given by its strictly upper part in CSR (u_ptr, u_idx, u_val)
+ * and its diagonal u_diag[n].  The caller applies the LU's row and column
+ * permutations.  Replaces the host LU solve of the Green's columns, one
+ * Poisson system per surface-charge column (make_solver's lu.solve on the
+ * unit-charge block, trench_charging_2d.py:297-309, :795-797); the
+ * factorisation itself stays on the host.  *ms (nullable) receives the
+ * kernel time. */
+int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, const int32_t *l_idx, const double *l_val,
+                const int32_t *u_ptr, const int32_t *u_idx, const double *u_val, const double *u_diag, int32_t k,
+                const double *b, double *x, double *ms);
+
 /* ---------------------------------------------------------------------------
  * Ensemble push: many trench instances (own grid, fields, taper, particle
  * count, ion distributions) traced together by one persistent launch per

@@ -63,6 +63,7 @@ def test_secant_pairs_carry_over_to_a_perturbed_map():
     assert warm < cold
 
 
+@pytest.mark.gpu                  # the set-up solves its Green's columns on the device
 @pytest.mark.parametrize("name", ["taper", "ar5"])
 def test_potential_coordinates_round_trip(name):
     rp = {"taper": cases.taper_params, "ar5": cases.ar5_params}[name]()

@@ -92,6 +92,8 @@ SIGNATURES = {
     "th_last_push_ms": (ctypes.c_int, [vp, dp]),
     "th_selftest_math": (ctypes.c_int, [vp, dp, dp, ctypes.c_int64, dp]),
     "th_probe_copy": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32, dp]),
+    "th_lu_solve": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, i32p, i32p, dp, i32p, i32p, dp, dp, ctypes.c_int32,
+                                   dp, dp, dp]),
     "th_inject": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(InjectParams), ctypes.c_uint64,
                                  ctypes.c_uint64]),
     "th_charging_create": (ctypes.c_int, [vp, ctypes.POINTER(ChargingSetupC), ctypes.POINTER(vp)]),

@@ -4,8 +4,9 @@
 ``run()`` (trench_charging_2d.py:1045-1369): same batch order (inject, trace,
 deposit, leak, floating conductor, Poisson, field, record, steady test), same
 histories, probes, charge monitors, steady criterion and result keys.  The
-difference is where the batch runs: after a one-off host set-up (LU of the
-Poisson operator, the Green's matrix columns, the leak operator), every batch
+difference is where the batch runs: after a one-off set-up (LU of the
+Poisson operator on the host, the Green's matrix columns by FP64 triangular
+solves on the device, the leak operator), every batch
 stays on the device -- push, deposit tallies, leak, conductor step,
 potential (phi = G Q + phi_ext) and field -- and only a few kilobytes of
 diagnostics come back per batch.
@@ -27,11 +28,60 @@ from .tracer import HipTracer
 STEADY_CRITERION_VERSION = 2
 
 
-class ChargingSetup:
-    """Host-side, once per geometry: geometry, LU, Green's matrix on the rows
-    the push reads, leak operator, floating-conductor response, monitors."""
+def lu_factors(lu):
+    """SuperLU's factors (Pr A Pc = L U) in the form th_lu_solve takes: the
+    strictly lower part of the unit-diagonal L and the strictly upper part of
+    U in CSR, and U's diagonal."""
+    n = lu.shape[0]
+    Lc, Uc = lu.L.tocsr(), lu.U.tocsr()
+    Lc.sort_indices()
+    Uc.sort_indices()
+    last, first = Lc.indptr[1:] - 1, Uc.indptr[:-1]
+    rows = np.arange(n)
+    if (np.any(np.diff(Lc.indptr) < 1) or np.any(np.diff(Uc.indptr) < 1)
+            or not np.array_equal(Lc.indices[last], rows) or not np.array_equal(Uc.indices[first], rows)):
+        raise RuntimeError("LU factors without a stored diagonal")
+    keep_l = np.ones(Lc.nnz, dtype=bool)
+    keep_l[last] = False
+    keep_u = np.ones(Uc.nnz, dtype=bool)
+    keep_u[first] = False
+    shift = np.arange(n + 1)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    return dict(l_ptr=i32(Lc.indptr - shift), l_idx=i32(Lc.indices[keep_l]), l_val=f64(Lc.data[keep_l]),
+                u_ptr=i32(Uc.indptr - shift), u_idx=i32(Uc.indices[keep_u]), u_val=f64(Uc.data[keep_u]),
+                u_diag=f64(Uc.data[first]))
 
-    def __init__(self, p, circuit=None):
+
+def device_solve(solver, B, device=0):
+    """``solver.solve(B)`` (a model.PoissonSolver) for a block of right-hand
+    sides, with the triangular solves in FP64 on the device (th_lu_solve).
+    The LU factorisation and the conductor mapping stay on the host.  Returns
+    (solution, kernel ms)."""
+    lu = solver.lu
+    r = np.asarray(solver.reduce(B), dtype=np.float64)
+    if r.ndim != 2:
+        raise ValueError("device_solve takes a block of right-hand sides")
+    n, k = r.shape
+    z = np.empty((n, k))
+    z[lu.perm_r] = r                      # Pr r
+    w = np.empty((n, k))
+    f = lu_factors(lu)
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().th_lu_solve(int(device), n, *(_lib.ptr(f[key], _lib.i32p if key.endswith(("ptr", "idx"))
+                                                                 else _lib.dp)
+                                                         for key in ("l_ptr", "l_idx", "l_val", "u_ptr", "u_idx",
+                                                                     "u_val", "u_diag")),
+                                       k, _lib.ptr(z, _lib.dp), _lib.ptr(w, _lib.dp), ctypes.byref(ms)))
+    return solver.expand(w[lu.perm_c]), ms.value   # Pc w
+
+
+class ChargingSetup:
+    """Once per geometry: geometry, LU (host), Green's matrix on the rows the
+    push reads (columns solved on ``device``), leak operator,
+    floating-conductor response, monitors."""
+
+    def __init__(self, p, circuit=None, device=0):
         if p.wall_model not in model.WALL_MODELS:
             raise ValueError(f"wall_model must be one of {model.WALL_MODELS}: {p.wall_model!r}")
         self.p = p
@@ -62,7 +112,7 @@ class ChargingSetup:
         self.cond_col = cond_col
         B = np.zeros((N, len(reps)))
         B[surf[reps], np.arange(len(reps))] = -1.0 / (p.dx ** 2 * EPS0)   # 1 C/m per column (:795-797)
-        G_full = fields.solve(B)
+        G_full, self.green_solve_ms = device_solve(fields.solver, B, device)
         # rows the push reads: vacuum cells (field) and surface cells (barrier)
         keep = geo.vac.ravel().copy()
         keep[surf] = True
@@ -344,7 +394,7 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
         ion_velocity = None
         ion_energy = lambda rng_, n_: np.full(n_, p.electron_temp_eV / 2)
     t_setup = time.perf_counter()
-    s = ChargingSetup(p, circuit)
+    s = ChargingSetup(p, circuit, device)
     geo, fields = s.geo, s.fields
     nx, ny = geo.solid.shape
     dev = DeviceCharging(s, device=device)

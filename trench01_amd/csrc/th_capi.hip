@@ -11,6 +11,7 @@
 #include "../../include/trench_hip.h"
 #include "th_charge.hpp"
 #include "th_inject.hpp"
+#include "th_lu.hpp"
 #include "th_push.hpp"
 
 namespace {
@@ -1066,6 +1067,93 @@ extern "C" int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, doubl
     (void)hipFree(b);
     if (e != hipSuccess) return fail(TH_EHIP, "copy probe: %s", hipGetErrorString(e));
     *ms = f / repeats;
+    return TH_OK;
+}
+
+namespace {
+
+// A triangular factor's off-diagonal part in CSR: ptr from 0 to nnz, not
+// decreasing; row i's column indices below i (lower) or above i (upper).
+bool triangle_ok(int32_t n, const int32_t *ptr, const int32_t *idx, const double *val, bool lower) {
+    if (ptr[0] != 0) return false;
+    for (int32_t i = 0; i < n; ++i) {
+        if (ptr[i + 1] < ptr[i]) return false;
+        for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
+            if (lower ? (idx[q] < 0 || idx[q] >= i) : (idx[q] <= i || idx[q] >= n)) return false;
+            if (!std::isfinite(val[q])) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, const int32_t *l_idx,
+                           const double *l_val, const int32_t *u_ptr, const int32_t *u_idx, const double *u_val,
+                           const double *u_diag, int32_t k, const double *b, double *x, double *ms) {
+    if (n < 1 || k < 0 || !l_ptr || !u_ptr || !u_diag || (k > 0 && (!b || !x))) return fail(TH_EINVAL, "invalid argument");
+    if ((l_ptr[n] > 0 && (!l_idx || !l_val)) || (u_ptr[n] > 0 && (!u_idx || !u_val)))
+        return fail(TH_EINVAL, "factor array is NULL");
+    if (!triangle_ok(n, l_ptr, l_idx, l_val, true)) return fail(TH_EINVAL, "L is not strictly lower triangular CSR");
+    if (!triangle_ok(n, u_ptr, u_idx, u_val, false)) return fail(TH_EINVAL, "U is not strictly upper triangular CSR");
+    for (int32_t i = 0; i < n; ++i)
+        if (!(std::fabs(u_diag[i]) > 0.0) || !std::isfinite(u_diag[i])) return fail(TH_EINVAL, "U has a zero pivot");
+    if (ms) *ms = 0.0;
+    if (k == 0) return TH_OK;
+    TH_HIP(hipSetDevice(device));
+    const int slabs = (k + th::kLuLanes - 1) / th::kLuLanes;
+    const size_t ld = (size_t)slabs * th::kLuLanes;
+    const size_t nl = (size_t)l_ptr[n], nu = (size_t)u_ptr[n];
+    int32_t *d_lp = nullptr, *d_li = nullptr, *d_up = nullptr, *d_ui = nullptr;
+    double *d_lv = nullptr, *d_uv = nullptr, *d_ud = nullptr, *d_x = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        for (void *q : {(void *)d_lp, (void *)d_li, (void *)d_up, (void *)d_ui, (void *)d_lv, (void *)d_uv,
+                        (void *)d_ud, (void *)d_x})
+            if (q) (void)hipFree(q);
+        for (hipEvent_t v : ev)
+            if (v) (void)hipEventDestroy(v);
+    };
+    int rc;
+    if ((rc = dev_alloc(&d_lp, (size_t)n + 1)) || (rc = dev_alloc(&d_li, nl)) || (rc = dev_alloc(&d_lv, nl)) ||
+        (rc = dev_alloc(&d_up, (size_t)n + 1)) || (rc = dev_alloc(&d_ui, nu)) || (rc = dev_alloc(&d_uv, nu)) ||
+        (rc = dev_alloc(&d_ud, (size_t)n)) || (rc = dev_alloc(&d_x, (size_t)n * ld))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(d_lp, l_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nl) e = hipMemcpy(d_li, l_idx, sizeof(int32_t) * nl, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nl) e = hipMemcpy(d_lv, l_val, sizeof(double) * nl, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_up, u_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nu) e = hipMemcpy(d_ui, u_idx, sizeof(int32_t) * nu, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nu) e = hipMemcpy(d_uv, u_val, sizeof(double) * nu, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_ud, u_diag, sizeof(double) * n, hipMemcpyHostToDevice);
+    // rows padded to ld on the host (one linear copy each way); the padding columns start at zero and stay zero
+    std::vector<double> pad;
+    if (ld != (size_t)k) {
+        pad.assign((size_t)n * ld, 0.0);
+        for (size_t i = 0; i < (size_t)n; ++i) memcpy(pad.data() + i * ld, b + i * k, sizeof(double) * k);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(d_x, pad.empty() ? b : pad.data(), sizeof(double) * n * ld, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], nullptr);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(th::lu_solve_kernel, dim3(slabs), dim3(th::kLuLanes), 0, nullptr, n, d_lp, d_li, d_lv,
+                           d_up, d_ui, d_uv, d_ud, (int)ld, d_x);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[1], nullptr);
+    if (e == hipSuccess)
+        e = hipMemcpy(pad.empty() ? x : pad.data(), d_x, sizeof(double) * n * ld, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !pad.empty())
+        for (size_t i = 0; i < (size_t)n; ++i) memcpy(x + i * k, pad.data() + i * ld, sizeof(double) * k);
+    float f = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, ev[0], ev[1]);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "LU solve: %s", hipGetErrorString(e));
+    if (ms) *ms = f;
     return TH_OK;
 }
 

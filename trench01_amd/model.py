@@ -280,13 +280,25 @@ class PoissonSolver:
             self.free = None
 
     def solve(self, b):
+        return self.expand(self.lu.solve(self.reduce(b)))
+
+    def reduce(self, b):
+        """The factored system's right-hand side for ``b`` (one column or a block)."""
         if self.free is not None:
-            out = np.zeros_like(b, dtype=float)
-            out[self.free] = self.lu.solve(b[self.free])
+            return b[self.free]
+        if self.E is not None:
+            return self.E.T @ b
+        return b
+
+    def expand(self, y):
+        """The potential on every cell from a solution of the factored system."""
+        if self.free is not None:
+            out = np.zeros((self.free.size,) + y.shape[1:])
+            out[self.free] = y
             return out
         if self.E is not None:
-            return self.E @ self.lu.solve(self.E.T @ b)
-        return self.lu.solve(b)
+            return self.E @ y
+        return y
 
 
 def circuit_voltages(c, phase):

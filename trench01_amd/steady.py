@@ -137,7 +137,7 @@ class _Session:
     def __init__(self, p, device, setup, push_precision, green_precision):
         self.t_start = time.perf_counter()
         self.p = p
-        self.s = s = setup or ChargingSetup(p)
+        self.s = s = setup or ChargingSetup(p, device=device)
         self.dev = dev = DeviceCharging(s, device=device)
         dev.set_schedule(precision=push_precision)
         if green_precision:

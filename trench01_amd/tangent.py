@@ -205,7 +205,7 @@ class PathwiseBatch:
         from .charging import ChargingSetup
         from .tracer import HipTracer
         self.p = p
-        self.setup = setup = ChargingSetup(p)
+        self.setup = setup = ChargingSetup(p, device=device)
         self.geo = setup.geo
         self.fields = setup.fields
         self.tracer = HipTracer(self.geo.vac, device)
