@@ -312,8 +312,9 @@ int th_probe_copy(th_tracer *t, int64_t bytes, int32_t repeats, double *ms);
  * permutations.  Replaces the host LU solve of the Green's columns, one
  * Poisson system per surface-charge column (make_solver's lu.solve on the
  * unit-charge block, trench_charging_2d.py:297-309, :795-797); the
- * factorisation itself stays on the host.  *ms (nullable) receives the
- * kernel time. */
+ * factorisation itself stays on the host.  Each triangle's rows are grouped
+ * by dependency level and solved one level per launch.  *ms (nullable)
+ * receives the time of the launches. */
 int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, const int32_t *l_idx, const double *l_val,
                 const int32_t *u_ptr, const int32_t *u_idx, const double *u_val, const double *u_diag, int32_t k,
                 const double *b, double *x, double *ms);

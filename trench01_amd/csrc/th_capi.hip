@@ -1086,6 +1086,30 @@ bool triangle_ok(int32_t n, const int32_t *ptr, const int32_t *idx, const double
     return true;
 }
 
+// Rows of a validated triangle grouped by dependency level: level(i) = 1 +
+// the largest level of the rows its entries read (0 for none), rows visited
+// in dependency order.  order holds the rows level by level; level_ptr has
+// one more entry than there are levels.
+void triangle_levels(int32_t n, const int32_t *ptr, const int32_t *idx, bool upper, std::vector<int32_t> &order,
+                     std::vector<int32_t> &level_ptr) {
+    std::vector<int32_t> level((size_t)n, 0);
+    int32_t top = 0;
+    for (int32_t r = 0; r < n; ++r) {
+        const int32_t i = upper ? n - 1 - r : r;
+        int32_t l = 0;
+        for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) l = std::max(l, level[idx[q]] + 1);
+        level[i] = l;
+        top = std::max(top, l);
+    }
+    level_ptr.assign((size_t)top + 2, 0);
+    for (int32_t i = 0; i < n; ++i) ++level_ptr[level[i] + 1];
+    for (size_t l = 1; l < level_ptr.size(); ++l) level_ptr[l] += level_ptr[l - 1];
+    order.resize((size_t)n);
+    std::vector<int32_t> fill(level_ptr.begin(), level_ptr.end() - 1);
+    for (int32_t i = 0; i < n; ++i) order[fill[level[i]]++] = i;
+    // every level holds a row: a row of level l > 0 reads a row of level l - 1
+}
+
 }  // namespace
 
 extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, const int32_t *l_idx,
@@ -1104,12 +1128,15 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
     const int slabs = (k + th::kLuLanes - 1) / th::kLuLanes;
     const size_t ld = (size_t)slabs * th::kLuLanes;
     const size_t nl = (size_t)l_ptr[n], nu = (size_t)u_ptr[n];
-    int32_t *d_lp = nullptr, *d_li = nullptr, *d_up = nullptr, *d_ui = nullptr;
+    std::vector<int32_t> l_order, l_levels, u_order, u_levels;
+    triangle_levels(n, l_ptr, l_idx, false, l_order, l_levels);
+    triangle_levels(n, u_ptr, u_idx, true, u_order, u_levels);
+    int32_t *d_lp = nullptr, *d_li = nullptr, *d_up = nullptr, *d_ui = nullptr, *d_lo = nullptr, *d_uo = nullptr;
     double *d_lv = nullptr, *d_uv = nullptr, *d_ud = nullptr, *d_x = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     auto release = [&]() {
         for (void *q : {(void *)d_lp, (void *)d_li, (void *)d_up, (void *)d_ui, (void *)d_lv, (void *)d_uv,
-                        (void *)d_ud, (void *)d_x})
+                        (void *)d_ud, (void *)d_x, (void *)d_lo, (void *)d_uo})
             if (q) (void)hipFree(q);
         for (hipEvent_t v : ev)
             if (v) (void)hipEventDestroy(v);
@@ -1117,7 +1144,8 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
     int rc;
     if ((rc = dev_alloc(&d_lp, (size_t)n + 1)) || (rc = dev_alloc(&d_li, nl)) || (rc = dev_alloc(&d_lv, nl)) ||
         (rc = dev_alloc(&d_up, (size_t)n + 1)) || (rc = dev_alloc(&d_ui, nu)) || (rc = dev_alloc(&d_uv, nu)) ||
-        (rc = dev_alloc(&d_ud, (size_t)n)) || (rc = dev_alloc(&d_x, (size_t)n * ld))) {
+        (rc = dev_alloc(&d_ud, (size_t)n)) || (rc = dev_alloc(&d_x, (size_t)n * ld)) ||
+        (rc = dev_alloc(&d_lo, (size_t)n)) || (rc = dev_alloc(&d_uo, (size_t)n))) {
         release();
         return rc;
     }
@@ -1128,6 +1156,8 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
     if (e == hipSuccess && nu) e = hipMemcpy(d_ui, u_idx, sizeof(int32_t) * nu, hipMemcpyHostToDevice);
     if (e == hipSuccess && nu) e = hipMemcpy(d_uv, u_val, sizeof(double) * nu, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_ud, u_diag, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_lo, l_order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_uo, u_order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice);
     // rows padded to ld on the host (one linear copy each way); the padding columns start at zero and stay zero
     std::vector<double> pad;
     if (ld != (size_t)k) {
@@ -1139,9 +1169,15 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
     if (e == hipSuccess) e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
     if (e == hipSuccess) e = hipEventRecord(ev[0], nullptr);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(th::lu_solve_kernel, dim3(slabs), dim3(th::kLuLanes), 0, nullptr, n, d_lp, d_li, d_lv,
-                           d_up, d_ui, d_uv, d_ud, (int)ld, d_x);
+    // one stream-ordered launch per level: L's levels, then U's; a level's rows run in parallel
+    for (size_t l = 0; l + 1 < l_levels.size() && e == hipSuccess; ++l) {
+        hipLaunchKernelGGL(th::lu_level_kernel, dim3(l_levels[l + 1] - l_levels[l], slabs), dim3(th::kLuLanes), 0,
+                           nullptr, d_lo + l_levels[l], d_lp, d_li, d_lv, (const double *)nullptr, (int)ld, d_x);
+        e = hipGetLastError();
+    }
+    for (size_t l = 0; l + 1 < u_levels.size() && e == hipSuccess; ++l) {
+        hipLaunchKernelGGL(th::lu_level_kernel, dim3(u_levels[l + 1] - u_levels[l], slabs), dim3(th::kLuLanes), 0,
+                           nullptr, d_uo + u_levels[l], d_up, d_ui, d_uv, (const double *)d_ud, (int)ld, d_x);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev[1], nullptr);

@@ -48,48 +48,25 @@ __device__ inline LuChunk load_chunk(int base, int end, const int *__restrict__ 
     return LuChunk{cnt, lane < cnt ? idx[base + lane] : 0, lane < cnt ? val[base + lane] : 0.0};
 }
 
-// Solves one triangle in place, rows in dependency order (first, first +
-// step, ... for n rows).  The first chunk of the next row is loaded while the
-// current row is being reduced; rows longer than a chunk load the rest as
-// they go.
-template <bool UPPER>
-__device__ inline void triangle(int n, const int *__restrict__ ptr, const int *__restrict__ idx,
-                                const double *__restrict__ val, const double *__restrict__ diag, double *xc,
-                                int ld) {
-    int i = UPPER ? n - 1 : 0;
-    LuChunk c = load_chunk(ptr[i], ptr[i + 1], idx, val);
-    for (int r = 0; r < n; ++r, i += UPPER ? -1 : 1) {
-        const int q0 = ptr[i], q1 = ptr[i + 1];
-        const int j = UPPER ? i - 1 : i + 1;
-        const LuChunk next = r + 1 < n ? load_chunk(ptr[j], ptr[j + 1], idx, val) : LuChunk{0, 0, 0.0};
-        double acc = chunk_update(xc[(size_t)i * ld], c.cnt, c.my_i, c.my_v, xc, ld);
-        for (int base = q0 + kLuLanes; base < q1; base += kLuLanes) {
-            const LuChunk more = load_chunk(base, q1, idx, val);
-            acc = chunk_update(acc, more.cnt, more.my_i, more.my_v, xc, ld);
-        }
-        xc[(size_t)i * ld] = UPPER ? acc / diag[i] : acc;
-        c = next;
+// One level of a triangle: rows[blockIdx.x] for the 64 columns of slab
+// blockIdx.y.  A level's rows depend only on rows of earlier levels, which
+// earlier launches on the same stream have finished; x[i] becomes
+// x[i] - sum over the row of val * x[idx], divided by diag[i] for U (diag
+// nullptr: L's unit diagonal).  The host builds the levels from the
+// validated factors, so every row and index is in [0, n).
+__global__ __launch_bounds__(kLuLanes) void lu_level_kernel(const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                            const int *__restrict__ idx,
+                                                            const double *__restrict__ val,
+                                                            const double *__restrict__ diag, int ld, double *x) {
+    const int i = rows[blockIdx.x];
+    double *xc = x + (size_t)blockIdx.y * kLuLanes + threadIdx.x;
+    const int q1 = ptr[i + 1];
+    double acc = xc[(size_t)i * ld];
+    for (int base = ptr[i]; base < q1; base += kLuLanes) {
+        const LuChunk c = load_chunk(base, q1, idx, val);
+        acc = chunk_update(acc, c.cnt, c.my_i, c.my_v, xc, ld);
     }
-}
-
-// x holds n rows of ld right-hand-side values (row-major, ld a multiple of
-// kLuLanes) and is overwritten by U^-1 L^-1 x.  L is unit lower triangular
-// (its strictly lower part in CSR: lp, li, lv), U upper triangular (strictly
-// upper part in CSR: up, ui, uv; diagonal ud).  Each lane owns one column of
-// x and walks the rows in dependency order, so every x[j] it reads is final
-// and was written by the lane itself.  The host checks that li[q] < i and
-// ui[q] > i for the entries of row i, which also keeps every access in
-// bounds.
-__global__ __launch_bounds__(kLuLanes) void lu_solve_kernel(int n, const int *__restrict__ lp,
-                                                            const int *__restrict__ li,
-                                                            const double *__restrict__ lv,
-                                                            const int *__restrict__ up,
-                                                            const int *__restrict__ ui,
-                                                            const double *__restrict__ uv,
-                                                            const double *__restrict__ ud, int ld, double *x) {
-    double *xc = x + (size_t)blockIdx.x * kLuLanes + threadIdx.x;
-    triangle<false>(n, lp, li, lv, nullptr, xc, ld);
-    triangle<true>(n, up, ui, uv, ud, xc, ld);
+    xc[(size_t)i * ld] = diag ? acc / diag[i] : acc;
 }
 
 }  // namespace th
