@@ -52,3 +52,24 @@ def test_setup_green_matrix_equals_the_host_solve():
     P[:, s.col_of >= 0] = G[s.geo.surface][:, s.col_of[s.col_of >= 0]]
     np.testing.assert_allclose(s.surface_response, P, rtol=0, atol=1e-12 * np.abs(P).max())
     np.testing.assert_allclose(s.g_cond.ravel(), G[:, s.cond_col], rtol=0, atol=1e-12 * np.abs(G).max())
+
+
+def test_tiny_system_matches_dense_solve():
+    """A 3x3 system written out by hand (L's first row and U's last row hold
+    no off-diagonal entry, so each triangle has three levels): the device
+    solve equals the dense solve of (L U) x = b for two right-hand sides."""
+    _case("lu solve 3x3")
+    from tests.test_green_host import GOOD
+    from trench01_amd import _lib
+    L = np.array([[1.0, 0, 0], [2.0, 1, 0], [0, 3.0, 1]])
+    U = np.array([[4.0, 1.0, 0], [0, 5.0, 2.0], [0, 0, 6.0]])
+    b = np.array([[1.0, -2.0], [0.5, 3.0], [2.0, 1.0]])
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    arrs = [(i32(GOOD["lp"]), _lib.i32p), (i32(GOOD["li"]), _lib.i32p), (f64(GOOD["lv"]), _lib.dp),
+            (i32(GOOD["up"]), _lib.i32p), (i32(GOOD["ui"]), _lib.i32p), (f64(GOOD["uv"]), _lib.dp),
+            (f64(GOOD["ud"]), _lib.dp)]
+    x = np.empty_like(b)
+    _lib.check(_lib.load().th_lu_solve(0, 3, *(_lib.ptr(a, t) for a, t in arrs), 2, _lib.ptr(b, _lib.dp),
+                                       _lib.ptr(x, _lib.dp), None))
+    np.testing.assert_allclose(x, np.linalg.solve(L @ U, b), rtol=1e-14, atol=1e-14)
