@@ -86,3 +86,34 @@ def test_lu_solve_rejects_malformed_factors(change, message):
 def test_lu_solve_accepts_no_right_hand_sides():
     rc, _ = _call(**GOOD, k=0)
     assert rc == 0
+
+
+@pytest.mark.parametrize("name", list(SOLVERS))
+def test_unit_source_solve_gathers_the_solver_columns(name, monkeypatch):
+    """unit_source_solve's host side (the right-hand side built permuted and
+    padded, potentials read back through the cell-to-unknown map): with
+    scipy's triangular solves standing in for th_lu_solve, every cell's
+    potential equals solver.solve of the unit-source block, including a
+    floating conductor's shared unknown and a driven conductor's fixed cells."""
+    def host_lu_solve(lu, z, device):
+        n = lu.shape[0]
+        f = charging.lu_factors(lu)
+        L = sps.csr_matrix((f["l_val"], f["l_idx"], f["l_ptr"]), shape=(n, n)) + sps.identity(n)
+        U = sps.csr_matrix((f["u_val"], f["u_idx"], f["u_ptr"]), shape=(n, n)) + sps.diags(f["u_diag"])
+        y = spsolve_triangular(L.tocsr(), z, lower=True, unit_diagonal=True)
+        return spsolve_triangular(U.tocsr(), y, lower=False), 0.0
+
+    monkeypatch.setattr(charging, "_lu_solve", host_lu_solve)
+    s = SOLVERS[name]()
+    n_cells = s.A.shape[0]
+    cells = np.flatnonzero(s.unknown_of() >= 0)[::301][:6]
+    potential, _ = charging.unit_source_solve(s, cells, 2.5)
+    B = np.zeros((n_cells, cells.size))
+    B[cells, np.arange(cells.size)] = 2.5
+    ref = s.solve(B)
+    every = np.arange(n_cells)
+    np.testing.assert_allclose(potential(every), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    np.testing.assert_array_equal(potential(every, 1), potential(every)[:, 1])
+    if s.free is not None:
+        with pytest.raises(ValueError, match="driven"):
+            charging.unit_source_solve(s, np.flatnonzero(~s.free)[:1], 1.0)

@@ -53,6 +53,21 @@ def lu_factors(lu):
                 u_diag=f64(Uc.data[first]))
 
 
+def _lu_solve(lu, z, device):
+    """Solves L U w = z on the device (th_lu_solve) for a C-ordered block z."""
+    n, k = z.shape
+    w = np.empty((n, k))
+    f = lu_factors(lu)
+    ms = ctypes.c_double()
+    _lib.check(_lib.load().th_lu_solve(int(device), n, *(_lib.ptr(f[key], _lib.i32p if key.endswith(("ptr", "idx"))
+                                                                 else _lib.dp)
+                                                         for key in ("l_ptr", "l_idx", "l_val", "u_ptr", "u_idx",
+                                                                     "u_val", "u_diag")),
+                                       k, _lib.ptr(np.ascontiguousarray(z), _lib.dp), _lib.ptr(w, _lib.dp),
+                                       ctypes.byref(ms)))
+    return w, ms.value
+
+
 def device_solve(solver, B, device=0):
     """``solver.solve(B)`` (a model.PoissonSolver) for a block of right-hand
     sides, with the triangular solves in FP64 on the device (th_lu_solve).
@@ -62,18 +77,38 @@ def device_solve(solver, B, device=0):
     r = np.asarray(solver.reduce(B), dtype=np.float64)
     if r.ndim != 2:
         raise ValueError("device_solve takes a block of right-hand sides")
-    n, k = r.shape
-    z = np.empty((n, k))
-    z[lu.perm_r] = r                      # Pr r
-    w = np.empty((n, k))
-    f = lu_factors(lu)
-    ms = ctypes.c_double()
-    _lib.check(_lib.load().th_lu_solve(int(device), n, *(_lib.ptr(f[key], _lib.i32p if key.endswith(("ptr", "idx"))
-                                                                 else _lib.dp)
-                                                         for key in ("l_ptr", "l_idx", "l_val", "u_ptr", "u_idx",
-                                                                     "u_val", "u_diag")),
-                                       k, _lib.ptr(z, _lib.dp), _lib.ptr(w, _lib.dp), ctypes.byref(ms)))
-    return solver.expand(w[lu.perm_c]), ms.value   # Pc w
+    z = np.empty(r.shape)
+    z[lu.perm_r] = r                      # Pr r  (SuperLU: Pr A Pc = L U)
+    w, ms = _lu_solve(lu, z, device)
+    return solver.expand(w[lu.perm_c]), ms   # Pc w
+
+
+def unit_source_solve(solver, cells, value, device=0):
+    """The potentials of unit sources, one column per source: column k solves
+    the Poisson system whose right-hand side is ``value`` at cell ``cells[k]``
+    and zero elsewhere (solver.solve of that block), with the triangular
+    solves on the device.  The right-hand side is built already permuted and
+    padded to whole waves, and the potentials are read only where asked:
+    returns (potential, kernel ms), where potential(cells_out) is the
+    [len(cells_out), k] array and potential(cells_out, j) its column j."""
+    lu = solver.lu
+    unknown = solver.unknown_of()
+    n, k = lu.shape[0], len(cells)
+    src = unknown[np.asarray(cells)]
+    if np.any(src < 0):
+        raise ValueError("a unit source on a driven conductor's cell")
+    z = np.zeros((n, -(-k // 64) * 64))
+    z[lu.perm_r[src], np.arange(k)] = value
+    w, ms = _lu_solve(lu, z, device)
+
+    def potential(cells_out, column=None):
+        u = unknown[np.asarray(cells_out)]
+        at = lu.perm_c[np.maximum(u, 0)]
+        out = w[at, :k] if column is None else w[at, column]
+        out[u < 0] = 0.0
+        return out
+
+    return potential, ms
 
 
 class ChargingSetup:
@@ -110,21 +145,21 @@ class ChargingSetup:
                 reps.append(k)
         self.col_of = col_of
         self.cond_col = cond_col
-        B = np.zeros((N, len(reps)))
-        B[surf[reps], np.arange(len(reps))] = -1.0 / (p.dx ** 2 * EPS0)   # 1 C/m per column (:795-797)
-        G_full, self.green_solve_ms = device_solve(fields.solver, B, device)
+        # 1 C/m per column (:795-797)
+        green_at, self.green_solve_ms = unit_source_solve(fields.solver, surf[reps], -1.0 / (p.dx ** 2 * EPS0),
+                                                          device)
         # rows the push reads: vacuum cells (field) and surface cells (barrier)
         keep = geo.vac.ravel().copy()
         keep[surf] = True
         self.rows = np.flatnonzero(keep).astype(np.int32)
-        self.green = np.ascontiguousarray(G_full[self.rows])
+        self.green = np.ascontiguousarray(green_at(self.rows))
         # surface response P (m x m) for the leak operator (:795-798)
         P = np.zeros((m, m))
         has = col_of >= 0
-        P[:, has] = G_full[surf][:, col_of[has]]
+        P[:, has] = green_at(surf)[:, col_of[has]]
         self.surface_response = P      # surface potentials per unit surface charge [V per C/m]
         self.leak = self._leakage(P) if p.sigma_s > 0 else None
-        self.g_cond = G_full[:, cond_col].reshape(geo.solid.shape) if fields.floating else None
+        self.g_cond = green_at(np.arange(N), cond_col).reshape(geo.solid.shape) if fields.floating else None
         if fields.floating:
             self.C_cond = 1.0 / self.g_cond[fields.cond][0]                      # :1089
             c_cell = np.flatnonzero(geo.surface_mask() & fields.cond)[0]        # :1090

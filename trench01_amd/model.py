@@ -290,6 +290,19 @@ class PoissonSolver:
             return self.E.T @ b
         return b
 
+    def unknown_of(self):
+        """Per cell, the unknown of the factored system that holds its potential
+        (the shared conductor unknown for a floating conductor's cells), or -1
+        for a driven conductor's cells (fixed potential, zero in ``expand``)."""
+        n_cells = self.A.shape[0]
+        if self.free is not None:
+            out = np.full(n_cells, -1, dtype=np.int64)
+            out[self.free] = np.arange(int(self.free.sum()))
+            return out
+        if self.E is not None:
+            return self.E.indices.astype(np.int64)      # one entry per row of E
+        return np.arange(n_cells, dtype=np.int64)
+
     def expand(self, y):
         """The potential on every cell from a solution of the factored system."""
         if self.free is not None:
