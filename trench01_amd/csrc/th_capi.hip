@@ -1134,7 +1134,12 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
     int32_t *d_lp = nullptr, *d_li = nullptr, *d_up = nullptr, *d_ui = nullptr, *d_lo = nullptr, *d_uo = nullptr;
     double *d_lv = nullptr, *d_uv = nullptr, *d_ud = nullptr, *d_x = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    auto release = [&]() {
+    hipStream_t st = nullptr;   // its own stream: the legacy null stream would wait for every blocking stream
+    auto release = [&]() {   // after an error too: nothing enqueued may still use the buffers
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
         for (void *q : {(void *)d_lp, (void *)d_li, (void *)d_up, (void *)d_ui, (void *)d_lv, (void *)d_uv,
                         (void *)d_ud, (void *)d_x, (void *)d_lo, (void *)d_uo})
             if (q) (void)hipFree(q);
@@ -1149,7 +1154,8 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
         release();
         return rc;
     }
-    hipError_t e = hipMemcpy(d_lp, l_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+    hipError_t e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipMemcpy(d_lp, l_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess && nl) e = hipMemcpy(d_li, l_idx, sizeof(int32_t) * nl, hipMemcpyHostToDevice);
     if (e == hipSuccess && nl) e = hipMemcpy(d_lv, l_val, sizeof(double) * nl, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_up, u_ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
@@ -1168,21 +1174,22 @@ extern "C" int th_lu_solve(int32_t device, int32_t n, const int32_t *l_ptr, cons
         e = hipMemcpy(d_x, pad.empty() ? b : pad.data(), sizeof(double) * n * ld, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
-    if (e == hipSuccess) e = hipEventRecord(ev[0], nullptr);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], st);
     // one stream-ordered launch per level: L's levels, then U's; a level's rows run in parallel
     for (size_t l = 0; l + 1 < l_levels.size() && e == hipSuccess; ++l) {
         hipLaunchKernelGGL(th::lu_level_kernel, dim3(l_levels[l + 1] - l_levels[l], slabs), dim3(th::kLuLanes), 0,
-                           nullptr, d_lo + l_levels[l], d_lp, d_li, d_lv, (const double *)nullptr, (int)ld, d_x);
+                           st, d_lo + l_levels[l], d_lp, d_li, d_lv, (const double *)nullptr, (int)ld, d_x);
         e = hipGetLastError();
     }
     for (size_t l = 0; l + 1 < u_levels.size() && e == hipSuccess; ++l) {
         hipLaunchKernelGGL(th::lu_level_kernel, dim3(u_levels[l + 1] - u_levels[l], slabs), dim3(th::kLuLanes), 0,
-                           nullptr, d_uo + u_levels[l], d_up, d_ui, d_uv, (const double *)d_ud, (int)ld, d_x);
+                           st, d_uo + u_levels[l], d_up, d_ui, d_uv, (const double *)d_ud, (int)ld, d_x);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipEventRecord(ev[1], nullptr);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
     if (e == hipSuccess)
-        e = hipMemcpy(pad.empty() ? x : pad.data(), d_x, sizeof(double) * n * ld, hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(pad.empty() ? x : pad.data(), d_x, sizeof(double) * n * ld, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess && !pad.empty())
         for (size_t i = 0; i < (size_t)n; ++i) memcpy(x + i * k, pad.data() + i * ld, sizeof(double) * k);
     float f = 0.0f;
