@@ -89,3 +89,27 @@ def test_reference_result_file_loads_and_resumes():
     assert more["start_batch"] == 50 and len(more["hist"]["t"]) == 53
     np.testing.assert_array_equal(more["hist"]["t"][:50], res["hist"]["t"])
     assert np.isfinite(more["hist"]["bottom center"]).all()
+
+
+def test_resume_fills_histories_the_stored_result_lacks():
+    """A stored result without charge histories or a probe's history resumes
+    with NaN in their place and their last sample recomputed from the stored
+    charge (trench_charging_2d.py:1175-1197); the steady checks then skip the
+    unknown prefix (:1034-1041)."""
+    _case("resume with missing histories")
+    res, p = results.load_results(GOLDEN / "result_ar5.npz")
+    from trench01_amd import configs, monitors
+    sampler = configs.ion_velocity_sampler(p.ion_mass_amu, **configs.AR5_ION)
+    res["charge_hist"] = {}
+    res["charge_monitor_hist"] = {}
+    stored_bottom = res["hist"].pop("bottom center")
+    more = charging.run(replace(p, n_batches=2), log=None, resume=res, ion_velocity=sampler)
+    h = more["hist"]["bottom center"]
+    assert np.isnan(h[:49]).all() and np.isfinite(h[49:]).all()
+    # the last stored sample is the potential of the stored charge
+    assert abs(h[49] - stored_bottom[49]) <= 1e-9 * max(1.0, abs(stored_bottom[49]))
+    setup = charging.ChargingSetup(p)
+    initial = monitors.sample_charges(res["rho_raw"].ravel(), setup.regions, p.dx)
+    for key, q in more["charge_monitor_hist"].items():
+        assert np.isnan(q[:49]).all() and q[49] == initial[key], key
+    assert monitors.history_start(more["hist"], more["probes"], more["charge_monitor_hist"]) == 49

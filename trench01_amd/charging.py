@@ -371,6 +371,10 @@ class DeviceCharging:
 RESUME_EDITABLE = {"until_steady", "n_batches", "max_batches", "steady_window", "steady_rtol",
                    "steady_charge_rtol", "steady_charge_atol", "steady_hold", "backend", "cuda_device",
                    "probe_ions", "probe_energy_eV", "probe_trajectories"}
+# Settings of the steady criterion: a resumed run keeps its count of passed
+# checks only if none of them changed (trench_charging_2d.py:955-956, :1203-1207).
+STEADY_SETTINGS = ("until_steady", "steady_window", "steady_rtol", "steady_charge_rtol", "steady_charge_atol",
+                   "steady_hold")
 
 
 def resume_params(res, p):
@@ -390,6 +394,34 @@ def resume_params(res, p):
     if changed:
         raise ValueError("a resumed run must keep the geometry and physics of the stored one: " + ", ".join(changed))
     return replace(source, **{k: getattr(p, k) for k in editable if hasattr(p, k)})
+
+
+def resumed_steady_count(resume, p, start_batch, history_start, hist, probes, charge_monitor_hist,
+                         steady_reference=None):
+    """Consecutive passed steady checks a resumed run starts from
+    (trench_charging_2d.py:1200-1222): the stored count if the stored run used
+    the same criterion version and steady settings and its convergence was not
+    revoked on load; else, for a result without a checkpoint, the passes
+    re-counted backwards over its histories; otherwise 0."""
+    source = resume["params"]
+    same = (resume.get("steady_criterion_version") == STEADY_CRITERION_VERSION
+            and not resume.get("convergence_note")
+            and all(source.get(k) == getattr(p, k) for k in STEADY_SETTINGS))
+    if not same:
+        return 0
+    checkpoint = resume.get("checkpoint")
+    if checkpoint:
+        return int(checkpoint["steady_n_ok"])
+    n_ok, W = 0, p.steady_window
+    for size in range(start_batch - start_batch % W, history_start + 2 * W - 1, -W):
+        h = {k: np.asarray(v[:size], dtype=float) for k, v in hist.items()}
+        q = {k: np.asarray(v[:size], dtype=float) for k, v in charge_monitor_hist.items()}
+        if not (monitors.check_steady(h, probes, p, steady_reference)[0] and monitors.check_charge_steady(q, p)["ok"]):
+            break
+        n_ok += 1
+        if n_ok >= p.steady_hold:
+            break
+    return n_ok
 
 
 def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=None, ion_velocity=None,
@@ -457,7 +489,6 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
             if not checkpoint.get("phase_rng_state"):
                 raise ValueError("an RF-cycle run needs the stored phase random state to resume")
             phase_rng.bit_generator.state = checkpoint["phase_rng_state"]
-        n_ok = int(checkpoint.get("steady_n_ok", 0) or 0)
     dev.set_rho(rho_full[geo.surface])
     n = 2 * p.n_per_batch
     dev.reserve(n)
@@ -472,17 +503,36 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     charge_hist = {"left": [], "right": []}
     charge_monitor_hist = {k: [] for k in (*s.regions, "surface absolute")}
     if resume is not None:
+        # Histories a stored result lacks are NaN, except their last sample,
+        # which is recomputed from the stored charge (trench_charging_2d.py:1175-1197).
+        phi0 = fields.solve((-rho_full / EPS0)) + fields.external.ravel()
+        initial_charge = monitors.sample_charges(rho_full, s.regions, p.dx)
         for key in hist:
             hist[key] = list(np.asarray(resume["hist"].get(key, np.full(start_batch, np.nan)), dtype=float))
+            if key in probes and key not in resume["hist"]:
+                ix, iy = probes[key]
+                hist[key][-1] = float(phi0[ix * ny + iy])
+        stored_charge = resume.get("charge_hist") or {}
         for key in charge_hist:
-            charge_hist[key] = list(np.asarray(resume["charge_hist"][key], dtype=float))
+            charge_hist[key] = list(np.asarray(stored_charge.get(key, np.full(start_batch, np.nan)), dtype=float))
+            if key not in stored_charge:
+                charge_hist[key][-1] = initial_charge[key]
+        stored_monitors = resume.get("charge_monitor_hist") or {}
         for key in charge_monitor_hist:
-            charge_monitor_hist[key] = list(np.asarray(resume["charge_monitor_hist"][key], dtype=float))
+            saved = stored_monitors.get(key)
+            if saved is None:
+                saved = charge_hist[key] if key in charge_hist else np.full(start_batch, np.nan)
+            charge_monitor_hist[key] = list(np.asarray(saved, dtype=float))
+            charge_monitor_hist[key][-1] = initial_charge[key]
     frac_open = (s.b1 - s.b0) / nx
     if device_monitors:
         dev.set_monitors(probes, s.regions)
     n_max = p.max_batches if p.until_steady else start_batch + p.n_batches
     converged_at, stopped = None, False
+    history_start = monitors.history_start(hist, probes, charge_monitor_hist)
+    if resume is not None and p.until_steady:
+        n_ok = resumed_steady_count(resume, p, start_batch, history_start, hist, probes, charge_monitor_hist,
+                                    steady_reference)
     push_ms = update_ms = 0.0
     phi = fields.external.ravel().copy()
     t0 = time.time()
@@ -533,7 +583,7 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
                 f"({time.time() - t0:.1f}s)")
         W = p.steady_window
         check_info = None
-        if p.until_steady and (b + 1) % W == 0 and b + 1 >= 2 * W:
+        if p.until_steady and (b + 1) % W == 0 and (b + 1) - history_start >= 2 * W:
             v_ok, name, dvolt, tol = monitors.check_steady(hist, probes, p, steady_reference)
             charge = monitors.check_charge_steady(charge_monitor_hist, p)
             ok = bool(v_ok and charge["ok"])

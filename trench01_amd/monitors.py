@@ -55,6 +55,19 @@ def sample_charges(rho, regions, dx):
     return out
 
 
+def history_start(hist, probe_names, charge_monitor_hist):
+    """First batch from which every probe and charge-monitor history holds
+    finite values: result files of older runs lack some histories, which a
+    resume fills with NaN, and the steady checks only look at the part where
+    all of them are known (trench_charging_2d.py:1034-1041)."""
+    first = 0
+    for values in [*(hist[k] for k in probe_names), *charge_monitor_hist.values()]:
+        missing = np.flatnonzero(~np.isfinite(np.asarray(values, dtype=float)))
+        if missing.size:
+            first = max(first, int(missing[-1]) + 1)
+    return first
+
+
 def mean_and_se(x):
     """Block mean and its standard error from linearly detrended residuals,
     with the effective sample size of a lag-1 autocorrelated series (:877-890)."""

@@ -4,8 +4,7 @@ without plots.
 ``save_results(res, out)`` writes ``out.npz`` with the reference's keys (fields,
 raw charge, histories, charge monitors, steady values, the run's parameters
 as JSON, the checkpoint with the host random state), so the reference's
-``load_results`` reads it; a vertical-ion diagnostic (``res["ion_probe"]``)
-is stored as the reference stores it.  ``load_results(path)`` reads files written by
+``load_results`` reads it.  ``load_results(path)`` reads files written by
 either side without unpickling anything (``allow_pickle=False``).  Parameters
 that the reference's ``Params`` does not have (this package's device-injection
 fields) are kept under ``params_extra_json``.
@@ -56,40 +55,6 @@ def _circuit_from(d):
     return c
 
 
-def _probe_arrays(probe):
-    """The vertical-ion diagnostic (``res["ion_probe"]``) as ``probe_*`` keys,
-    :1709-1719: the scalars as JSON, each beam's arrays under
-    ``probe_{label}_*``, its trajectories concatenated with their lengths."""
-    if probe is None or probe["cancelled"]:
-        return {}
-    out = {"probe_meta_json": np.array(json.dumps({k: v for k, v in probe.items()
-                                                   if k not in ("charged", "uncharged")}))}
-    for label in ("charged", "uncharged"):
-        d = probe[label]
-        out.update({f"probe_{label}_{k}": v for k, v in d.items() if isinstance(v, np.ndarray)})
-        out[f"probe_{label}_path_lengths"] = np.array([len(v) for v in d["paths"]])
-        out[f"probe_{label}_paths"] = (np.concatenate(d["paths"]) if d["paths"] else np.zeros((0, 2)))
-        out[f"probe_{label}_meta_json"] = np.array(json.dumps({k: v for k, v in d.items()
-                                                               if not isinstance(v, np.ndarray) and k != "paths"}))
-    return out
-
-
-def _probe_from(d):
-    """Inverse of _probe_arrays (:1775-1784, :1800)."""
-    if "probe_meta_json" not in d:
-        return None
-    probe = json.loads(str(d["probe_meta_json"]))
-    for label in ("charged", "uncharged"):
-        prefix = f"probe_{label}_"
-        record = json.loads(str(d[prefix + "meta_json"]))
-        record.update({k[len(prefix):]: d[k].copy() for k in d.files
-                       if k.startswith(prefix) and k != prefix + "meta_json"})
-        lengths = record.pop("path_lengths")
-        record["paths"] = list(np.split(record["paths"], np.cumsum(lengths)[:-1])) if lengths.size else []
-        probe[label] = record
-    return probe
-
-
 def save_results(res, out):
     """Write ``out + '.npz'`` (trench_charging_2d.py:1687-1734, the arrays only;
     with an RF circuit its waveform as well)."""
@@ -109,8 +74,7 @@ def save_results(res, out):
              **{f"qmonitor_{k}": v for k, v in res.get("charge_monitor_hist", {}).items()},
              **{f"charge_{k}": v for k, v in res["charge_hist"].items()},
              **{f"hist_{k}": v for k, v in res["hist"].items()},
-             **{f"steady_{k}": v for k, v in res["steady"].items()}, **_circuit_arrays(res.get("circuit")),
-             **_probe_arrays(res.get("ion_probe")))
+             **{f"steady_{k}": v for k, v in res["steady"].items()}, **_circuit_arrays(res.get("circuit")))
     return path
 
 
@@ -152,9 +116,6 @@ def load_results(filename):
         if conv.get("previous_t_conv") is not None:
             res["previous_t_conv"] = conv["previous_t_conv"]
         res["circuit"] = _circuit_from(d)
-        probe = _probe_from(d)
-        if probe is not None:
-            res["ion_probe"] = probe
     # A stored "converged" flag is re-checked against the charge histories
     # (trench_charging_2d.py:1780-1800).
     if res["converged"]:
@@ -170,6 +131,4 @@ def load_results(filename):
             res["previous_t_conv"] = res["t_conv"]
             res["t_conv"], res["converged"] = np.nan, False
             res["convergence_note"] = "stored charge is not saturated under the charge criterion"
-            if "ion_probe" in res:
-                res["ion_probe"]["source_converged"] = False
     return res, p
