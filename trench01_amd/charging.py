@@ -505,6 +505,9 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     if resume is not None:
         # Histories a stored result lacks are NaN, except their last sample,
         # which is recomputed from the stored charge (trench_charging_2d.py:1175-1197).
+        # A stored charge-monitor history keeps its own last sample: it is the
+        # same integral of the same charge, summed on the device in the run
+        # that wrote it, so the resumed run continues it bit for bit.
         phi0 = fields.solve((-rho_full / EPS0)) + fields.external.ravel()
         initial_charge = monitors.sample_charges(rho_full, s.regions, p.dx)
         for key in hist:
@@ -520,10 +523,11 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
         stored_monitors = resume.get("charge_monitor_hist") or {}
         for key in charge_monitor_hist:
             saved = stored_monitors.get(key)
+            charge_monitor_hist[key] = list(np.asarray(
+                saved if saved is not None else charge_hist[key] if key in charge_hist else np.full(start_batch, np.nan),
+                dtype=float))
             if saved is None:
-                saved = charge_hist[key] if key in charge_hist else np.full(start_batch, np.nan)
-            charge_monitor_hist[key] = list(np.asarray(saved, dtype=float))
-            charge_monitor_hist[key][-1] = initial_charge[key]
+                charge_monitor_hist[key][-1] = initial_charge[key]
     frac_open = (s.b1 - s.b0) / nx
     if device_monitors:
         dev.set_monitors(probes, s.regions)
