@@ -288,6 +288,18 @@ int th_charging_monitors(const th_charging *c, double *out);
  * (trench_charging_2d.py:1268, :795-797). */
 int th_charging_green_adjoint(th_charging *c, const double *w, double *out);
 
+/* th_charging_green_adjoint without any device allocation: same inputs,
+ * outputs and kernels (bit-identical results), with the weights, the partial
+ * sums and the result held in a fixed per-device array of 8 MiB (2^20
+ * doubles) that the library keeps from load until the process ends.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Rejects with TH_EINVAL a handle whose nx*ny +
+ * (ceil(n_rows/64) + 1) * n_cols exceeds 2^20.  Calls on one device share that
+ * array, so calls that overlap on one device (from several threads or
+ * handles) race; sequential calls do not. */
+int th_charging_green_adjoint_noalloc(th_charging *c, const double *w, double *out);
+
 /* Reads the state: rho (m), potential on the grid (nx*ny; cells outside the
  * kept rows read 0), and the last batch's per-surface-cell counts (2m). */
 int th_charging_read(th_charging *c, double *rho_s, double *phi /* nullable */, int32_t *counts /* nullable */);

@@ -107,6 +107,7 @@ SIGNATURES = {
                                                 ctypes.c_int32, ctypes.c_int32, i32p]),
     "th_charging_monitors": (ctypes.c_int, [vp, dp]),
     "th_charging_green_adjoint": (ctypes.c_int, [vp, dp, dp]),
+    "th_charging_green_adjoint_noalloc": (ctypes.c_int, [vp, dp, dp]),
     "th_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(vp)]),
     "th_ensemble_destroy": (None, [vp]),
     "th_ensemble_add": (ctypes.c_int, [vp, u8p, ctypes.c_int32, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_int32,

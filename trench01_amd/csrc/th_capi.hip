@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/trench_hip.h"
@@ -39,6 +40,33 @@ int dev_alloc(T **p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
     TH_HIP(hipMalloc((void **)p, count * sizeof(T)));
+    return TH_OK;
+}
+
+// Device memory of th_charging_green_adjoint_noalloc: the weights staged from
+// the host, the per-workgroup partial sums and the result, 8 MiB per device,
+// held from module load until the process ends.  Host code never passes
+// `scratch` itself to a kernel or a copy (on the host it names a stand-in,
+// not device memory): it uses the address hipGetSymbolAddress returns for each
+// device, resolved once per device with that device current.
+constexpr size_t kScratchDoubles = (size_t)1 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+int scratch_address(int device, double **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return fail(TH_EINVAL, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        TH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(TH_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
     return TH_OK;
 }
 
@@ -272,6 +300,8 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
     if (hipMemcpy(t->d_vac, t->vac.data(), t->N, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(t->d_smap, smap.data(), sizeof(int32_t) * t->N, hipMemcpyHostToDevice) != hipSuccess)
         return bad(fail(TH_EHIP, "upload of the vacuum map failed"));
+    double *sp = nullptr;   // per-device set-up of the allocation-free entry point's scratch
+    if ((rc = scratch_address(device, &sp))) return bad(rc);
     *out = t;
     return TH_OK;
 }
@@ -962,16 +992,25 @@ int th_charging_read(th_charging *c, double *rho_s, double *phi, int32_t *counts
     return TH_OK;
 }
 
-int th_charging_green_adjoint(th_charging *c, const double *w, double *out) {
-    if (!c || !w || !out) return fail(TH_EINVAL, "invalid argument");
+}  // extern "C"
+
+namespace {
+
+// G^T w with buf = [N] weights, [blocks][n_cols] partial sums, [n_cols] result
+// (green_adjoint_buffer doubles): upload, two kernels, readback, wait.  Every
+// entry of buf that is read is written first in the same call.
+int green_adjoint_rows_per_block() { return 64; }
+
+size_t green_adjoint_buffer(const th_charging *c) {
+    const int rpb = green_adjoint_rows_per_block();
+    const size_t blocks = (size_t)std::max(1, (c->d.n_rows + rpb - 1) / rpb);
+    return (size_t)c->t->N + blocks * c->d.n_cols + c->d.n_cols;
+}
+
+int green_adjoint_into(th_charging *c, const double *w, double *out, double *buf) {
     th_tracer *t = c->t;
-    TH_HIP(hipSetDevice(t->device));
-    const int rows_per_block = 64;
+    const int rows_per_block = green_adjoint_rows_per_block();
     const int blocks = std::max(1, (c->d.n_rows + rows_per_block - 1) / rows_per_block);
-    double *buf = nullptr;   // [N] weights, [blocks][n_cols] partial sums, [n_cols] result
-    const size_t n_buf = (size_t)t->N + (size_t)blocks * c->d.n_cols + c->d.n_cols;
-    int rc = dev_alloc(&buf, n_buf);
-    if (rc) return rc;
     double *dw = buf, *partial = buf + t->N, *dout = partial + (size_t)blocks * c->d.n_cols;
     hipError_t e = hipMemcpyAsync(dw, w, sizeof(double) * t->N, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess) {
@@ -984,9 +1023,39 @@ int th_charging_green_adjoint(th_charging *c, const double *w, double *out) {
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, dout, sizeof(double) * c->d.n_cols, hipMemcpyDeviceToHost, t->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
-    if (e != hipSuccess) rc = fail(TH_EHIP, "green adjoint: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(TH_EHIP, "green adjoint: %s", hipGetErrorString(e));
+    return TH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int th_charging_green_adjoint(th_charging *c, const double *w, double *out) {
+    if (!c || !w || !out) return fail(TH_EINVAL, "invalid argument");
+    TH_HIP(hipSetDevice(c->t->device));
+    double *buf = nullptr;
+    int rc = dev_alloc(&buf, green_adjoint_buffer(c));
+    if (rc) return rc;
+    rc = green_adjoint_into(c, w, out, buf);
+    if (rc) (void)hipStreamSynchronize(c->t->stream);   // nothing may still use buf when it is freed
     (void)hipFree(buf);
     return rc;
+}
+
+int th_charging_green_adjoint_noalloc(th_charging *c, const double *w, double *out) {
+    if (!c || !w || !out) return fail(TH_EINVAL, "invalid argument");
+    const size_t need = green_adjoint_buffer(c);
+    if (need > kScratchDoubles)
+        return fail(TH_EINVAL,
+                    "G^T w on %d cells x %d columns needs %zu doubles of scratch; the allocation-free variant holds "
+                    "%zu (use th_charging_green_adjoint)",
+                    c->t->N, c->d.n_cols, need, kScratchDoubles);
+    TH_HIP(hipSetDevice(c->t->device));
+    double *buf = nullptr;
+    int rc = scratch_address(c->t->device, &buf);
+    if (rc) return rc;
+    return green_adjoint_into(c, w, out, buf);
 }
 
 int th_inject(th_tracer *t, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t batch) {
