@@ -62,6 +62,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all cores this process may use")
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady measurement")
     ap.add_argument("--max-batches", type=int, default=5000)
+    ap.add_argument("--steady-seeds", default="0,1,2,3", help="seeds of the time-to-steady comparison")
+    ap.add_argument("--long-run", type=int, default=300,
+                    help="plain batches run on past each plain convergence for the long-run time-marched mean")
+    ap.add_argument("--no-host-array", action="store_true", help="skip the host-array (PCIe-inclusive) tracer rate")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
     ap.add_argument("--green", type=int, default=0, choices=(0, 1, 2),
                     help="potential update: 0 FP64 Green's matrix, 1 BF16 hi/lo split, 2 FP16 hi/lo split")
@@ -136,9 +140,17 @@ def main():
         dev.inject_device(0, p.n_per_batch, p.seed, b)
         dev.step(0, n)
         dev.monitors()
-    # keep the state of the first timed batch for the CPU baseline
+    # The first timed batch, traced once beforehand to keep its inputs and the
+    # device's tallies for the parity check and the CPU baseline; the charge
+    # is then put back, so the timed loop starts from the same state.
+    rho_before = dev.read(phi=False, counts=False)[0]
     dev.inject_device(0, p.n_per_batch, p.seed, a.warmup)
     cpu_inputs = (dev.tracer.read_particles(0, n), dev.tracer.read_field(), dev.read(counts=False)[1])
+    d_first = dev.step(0, n)
+    gpu_first = dict(counts=dev.read(phi=False)[2], escaped=list(d_first.tally.escaped), lost=d_first.tally.lost,
+                     steps=d_first.tally.steps, off_surface=d_first.tally.off_surface)
+    dev.set_rho(rho_before)
+    host_array = host_array_rate(p, setup, *cpu_inputs) if not a.no_host_array else None
     steps_total = 0
     push_ms = update_ms = 0.0
     t0 = time.perf_counter()
@@ -160,31 +172,37 @@ def main():
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if a.precision == 0 else "f32",
         "data": "synthetic (device-injected particles, Philox)",
-        "config": {"workload": "AR8 trench 120x320, tapers 2/5 deg, conductive floating mask 40, barrier wall, "
-                               "sigma_s 1e-15; 1e6 particles per batch; full batch on device (inject, FP64 push, "
-                               "deposit, leak, conductor, Green's potential, field); multi-pass push schedule",
+        "config": {"workload": f"AR8 trench 120x320 (dx 5 nm), trench 20x160 with 2/5 deg tapers, conductive "
+                               f"floating mask 40, barrier wall, sigma_s 1e-15 S; ions arcsine IED 40-260 eV with "
+                               f"4 deg polar sigma, electrons isotropic Te 4 eV; 1e6 particles per batch "
+                               f"(5e5 per species), seed {p.seed}; full batch on device (inject, FP64 push, "
+                               f"deposit, leak, conductor, Green's potential, field); multi-pass push schedule",
                    "particles_per_batch": n, "grid": [setup.geo.nx, setup.geo.ny], "surface_cells": m,
                    "green_rows_cols": list(setup.green.shape), "precharge_batches": a.precharge,
                    "max_steps": p.max_steps, "mean_steps_per_particle": steps_total / (a.steps * n),
                    "push_ms_per_batch": push_ms / a.steps, "update_ms_per_batch": update_ms / a.steps},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu" if a.precision == 0 else "fp32-valu", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "push64_kernel" if a.precision == 0 else "push32_kernel",
                      "kernel_ms_per_batch": push_ms / a.steps,
                      "launches_per_batch": len(pass_caps(dev.prm.pass_steps, p.max_steps)),
                      "flop_per_particle_step": FLOP_PER_STEP,
-                     "note": "vector-ALU roof of the push's precision (FP64 78.6 TFLOP/s = FP64 MFMA peak; FP32 "
+                     "note": "vector-ALU roof of the push's precision (FP64 vector 78.6 TFLOP/s; FP32 vector "
                              "157.3); the push is ALU/latency bound, not HBM: algorithmic HBM traffic is 78 B/particle; "
                              "one batch's push is launches_per_batch stream-ordered launches (multi-pass schedule), "
                              "timed together with HIP events on the library stream; traffic is HBM bytes per batch",
                      "hbm_gbs_algorithmic": BYTES_PER_PARTICLE * n / (push_ms / a.steps * 1e-3) / 1e9},
     }
+    if host_array is not None:
+        out["host_array_tracer"] = host_array
     if precision is not None:
         out["precision"] = precision
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(p, setup, *cpu_inputs, a.cpu_threads, gpu_first)
     if not a.no_steady:
-        out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green)
+        out["time_to_steady"] = time_to_steady(p, a.max_batches, log, a.precision, a.green,
+                                               [int(x) for x in a.steady_seeds.split(",")], a.long_run)
         if "cpu_baseline" in out:
             # the plain run's batch count at the CPU push rate (push only; extrapolated, not run)
             nb = out["time_to_steady"]["plain"]["batches"]
@@ -262,6 +280,7 @@ def batching(count, batches, log, schedule=None):
     _, scal = ens.totals()
     steps = int(scal[:, 3].sum())
     nb_last = max(1, batches // 2)
+    parity = ensemble_parity(ens, specs, fields, prm, log)
     ens.close()
     steps_per_batch = steps / nb_last
     achieved = FLOP_PER_STEP * steps_per_batch / (ms["one"] * 1e-3) / 1e12
@@ -279,13 +298,54 @@ def batching(count, batches, log, schedule=None):
             "schedule": best, "tuned_candidates": len(table),
             "ms_per_batch": ms, "particles_per_s": {k: n / (v * 1e-3) for k, v in ms.items()},
             "speedup_vs_per_instance": ms["per_instance"] / ms["one"],
-            "graph_vs_eager": ms["one"] / ms["graph"], "setup_s": setup_s,
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "graph_vs_eager": ms["one"] / ms["graph"], "setup_s": setup_s, "parity": parity,
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                          "traffic_source": os.path.relpath(ENSEMBLE_TRAFFIC_FILE, ROOT) if traffic else None,
                          "kernel": "ensemble_push_kernel",
                          "note": "FP64 FLOP at 69 per particle-step over the eager batch time (one push launch plus "
                                  "injection, counter clears and the end-record reduction, about 1% of it)"}}
+
+
+def ensemble_parity(ens, specs, fields, prm, log, sample=4, max_particles=400_000):
+    """One more device-injected batch of the whole ensemble at full size: every
+    instance's accounting (absorbed + escaped + lost = its particle count, none
+    absorbed off the surface cells), and a sample of instances -- the largest
+    aspect ratio, the smallest and largest particle counts up to
+    ``max_particles``, and one drawn at random -- traced again by the oracle's
+    C push from the batch's own initial states and compared tally by tally."""
+    from oracle import trace_ref
+    from trench01_amd.tracer import tapered_of
+    ens.run(prm, seed=7, first_batch=0, batches=1, inject=True, mode="one")
+    counts, scal = ens.totals()
+    sizes = np.array([2 * s.p.n_per_batch for s in specs])
+    offs = np.cumsum([0] + [ens.surface(i).size for i in range(len(specs))])
+    absorbed = np.array([counts[:, offs[i]:offs[i + 1]].sum() for i in range(len(specs))])
+    accounted = absorbed + scal[:, 0] + scal[:, 1] + scal[:, 2] + scal[:, 4]
+    accounting_ok = bool(np.array_equal(accounted, sizes) and not scal[:, 4].any())
+    small = [i for i in range(len(specs)) if sizes[i] <= max_particles]
+    rng = np.random.default_rng(5)
+    pick = {int(np.argmax([s.p.trench_d / s.p.trench_w if sizes[i] <= max_particles else 0
+                           for i, s in enumerate(specs)])),
+            int(min(small, key=lambda i: sizes[i])), int(max(small, key=lambda i: sizes[i]))}
+    pick.add(int(rng.choice([i for i in small if i not in pick])))
+    cores = len(os.sched_getaffinity(0))
+    checked = []
+    for i in sorted(pick)[:sample]:
+        p = specs[i].p
+        vac, ex, ey, phi, wf = fields[i]
+        got = ens.read(i)
+        ref = trace_ref.trace_arrays(*ens.read_particles(i), ex, ey, vac, p.dx, p.cfl, p.max_steps, tapered_of(p),
+                                     (phi, wf), threads=cores)
+        ok = (np.array_equal(got["cnt_i"], ref["cnt_i"]) and np.array_equal(got["cnt_e"], ref["cnt_e"])
+              and np.array_equal(got["escaped"], ref["escaped"]) and got["lost"] == ref["lost"]
+              and got["steps"] == ref["total_steps"])
+        checked.append({"instance": i, "particles": int(sizes[i]), "aspect_ratio": p.trench_d / p.trench_w,
+                        "bit_exact": bool(ok)})
+    log(f"batching parity: accounting {'ok' if accounting_ok else 'FAILED'} over {len(specs)} instances; "
+        f"oracle sample {[c['instance'] for c in checked]} bit-exact {all(c['bit_exact'] for c in checked)}")
+    return {"accounting_ok": accounting_ok, "instances": len(specs), "particles": int(sizes.sum()),
+            "oracle_sample": checked, "bit_exact": accounting_ok and all(c["bit_exact"] for c in checked)}
 
 
 GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums"}
@@ -328,10 +388,37 @@ def measured_traffic(precision):
     return t["fetch_bytes_per_batch"] + t["write_bytes_per_batch"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
-def cpu_baseline(p, setup, parts, field, phi, threads):
+def host_array_rate(p, setup, parts, field, phi, calls=5):
+    """The tracer call as the reference defines it: host arrays in (particle
+    states and fields uploaded over PCIe), push, per-cell counts back on the
+    host (HipTracer.__call__, the CudaTracer signature), on the first timed
+    batch's particles and field; the best of ``calls`` calls after a warm-up."""
+    from trench01_amd import HipTracer
+    geo = setup.geo
+    ex, ey = field
+    barrier = (phi.reshape(geo.solid.shape), setup.fields.wface)
+    t = HipTracer(geo.vac)
+    t(*parts, ex, ey, geo.vac, p, barrier)
+    best = np.inf
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        ci, ce, esc, lost = t(*parts, ex, ey, geo.vac, p, barrier)
+        best = min(best, time.perf_counter() - t0)
+    t.close()
+    n = parts[0].size
+    return {"value": n / best, "unit": "particles/s", "ms_per_call": best * 1e3, "particles": n,
+            "note": "HipTracer called with host arrays (x, y, vx, vy, qm, spc, Ex, Ey, phi, wface uploaded; counts "
+                    "read back), the reference tracer call's definition of particles/s; value above is the "
+                    "device-resident batch"}
+
+
+def cpu_baseline(p, setup, parts, field, phi, threads, gpu=None):
     """The oracle's C push (OpenMP) over whole batches of the timed workload
     (same particle states and field as the GPU's first timed batch), repeated
-    until at least CPU_SAMPLE_S seconds of work."""
+    until at least CPU_SAMPLE_S seconds of work.  With ``gpu`` (the device's
+    tallies of that batch), also returns the parity of the two: per-cell ion
+    and electron counts on every surface cell, escapes per species, losses and
+    total particle-steps."""
     from oracle import trace_ref
     geo = setup.geo
     cores = threads or len(os.sched_getaffinity(0))
@@ -340,9 +427,11 @@ def cpu_baseline(p, setup, parts, field, phi, threads):
     x, y, vx, vy, qm, spc = parts
     reps, steps = 0, 0
     t0 = time.perf_counter()
+    first = None
     while reps == 0 or time.perf_counter() - t0 < CPU_SAMPLE_S:
         r = trace_ref.trace_arrays(x, y, vx, vy, qm, spc, ex, ey, geo.vac, p.dx, p.cfl, p.max_steps,
                                    True, (phi.reshape(geo.solid.shape), setup.fields.wface), threads=cores)
+        first = first or r
         reps += 1
         steps += r["total_steps"]
     wall = time.perf_counter() - t0
@@ -356,52 +445,104 @@ def cpu_baseline(p, setup, parts, field, phi, threads):
         reps1 += 1
         steps1 += r1["total_steps"]
     wall1 = time.perf_counter() - t1
-    return {"value": reps * x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
+    parity = None
+    if gpu is not None:
+        r0 = first
+        m = geo.surface.size
+        ci = r0["cnt_i"].ravel()
+        ce = r0["cnt_e"].ravel()
+        off = int(ci.sum() + ce.sum() - ci[geo.surface].sum() - ce[geo.surface].sum())
+        checks = {"counts_ions": bool(np.array_equal(gpu["counts"][:m], ci[geo.surface])),
+                  "counts_electrons": bool(np.array_equal(gpu["counts"][m:], ce[geo.surface])),
+                  "escaped": [int(v) for v in r0["escaped"]] == [int(v) for v in gpu["escaped"]],
+                  "lost": int(r0["lost"]) == int(gpu["lost"]),
+                  "steps": int(r0["total_steps"]) == int(gpu["steps"]),
+                  "off_surface": off == 0 and int(gpu["off_surface"]) == 0}
+        parity = {"bit_exact": all(checks.values()), "checks": checks, "particles": int(x.size),
+                  "absorbed": int(ci.sum() + ce.sum()), "escaped": [int(v) for v in r0["escaped"]],
+                  "lost": int(r0["lost"]), "particle_steps": int(r0["total_steps"]),
+                  "cells_differing": int(np.count_nonzero(gpu["counts"][:m] != ci[geo.surface])
+                                         + np.count_nonzero(gpu["counts"][m:] != ce[geo.surface])),
+                  "against": "oracle C push (FP64, contraction off, glibc hypot) on the same initial states and "
+                             "field as the device's first timed batch"}
+    base = {"value": reps * x.size / wall, "unit": "particles/s", "cores": cores, "kind": "port",
             "sample": f"{reps} pass(es) over one whole batch ({x.size} particles, {r['total_steps']} "
                       f"particle-steps each) of the timed workload, push only, {wall:.1f} s",
             "single_thread": {"value": reps1 * sub[0].size / wall1, "unit": "particles/s", "cores": 1,
                               "sample": f"{reps1} pass(es) over every 64th particle of the same batch "
                                         f"({sub[0].size} particles, {r1['total_steps']} particle-steps each), "
                                         f"{wall1:.1f} s"}}
+    return base, parity
 
 
-def time_to_steady(p, max_batches, log, push_precision=0, green=0):
+def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, seeds=(0, 1, 2, 3), long_run=300):
     """Time-to-steady from zero charge at this workload (device injection,
-    1e6 particles per batch): plain time-marching until the reference's
-    criterion passes (W = 30, hold = 2), and the chosen method (Anderson
-    solve of the batch map's fixed point, then plain batches to the same
-    criterion), and the chosen method with the precision variants (FP32 push,
-    BF16 split Green's update, unless others are given) with its drift.  Wall
-    times include each method's host set-up."""
+    1e6 particles per batch), per seed: plain time-marching until the
+    reference's criterion passes (W = 30, hold = 2), and the chosen method
+    (Anderson solve of the batch map's fixed point, then plain batches to the
+    same criterion), with the bottom-centre potential of each.  Each plain run
+    is then continued ``long_run`` batches: the mean over all seeds'
+    continuations is the long-run time-marched value (the plain run stops
+    while still drifting up towards it).  The chosen method with the
+    precision variants (FP32 push, BF16 split Green's update, unless others
+    are given) runs on the first seed.  Wall times include each method's host
+    set-up."""
     from trench01_amd import charging, steady
-    ps = replace(p, until_steady=True, max_batches=max_batches)
-    t0 = time.perf_counter()
-    res = charging.run(ps, log=None, injection="device")
-    wall = time.perf_counter() - t0
-    nb = res["timing"]["batches"]
-    plain = {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb, "wall_s": wall,
-             "setup_s": res["timing"]["setup_s"], "bottom_center_V": res["steady"]["bottom center"],
-             "t_conv_s": res["t_conv"]}
-    log(f"time-to-steady, plain: converged={res['converged']} after {nb} batches in {wall:.1f}s, "
-        f"bottom centre {plain['bottom_center_V']:.3f} V")
-    r = steady.anderson(ps)
-    chosen = {"method": r["method"], "converged": bool(r["converged"]), "batches": r["batches"],
-              "wall_s": r["wall_s"], "setup_s": r["setup_s"], "phases": r["phases"],
-              "bottom_center_V": r["steady"]["bottom center"]}
-    log(f"time-to-steady, chosen: converged={r['converged']} after {r['batches']} batches in {r['wall_s']:.1f}s, "
-        f"bottom centre {chosen['bottom_center_V']:.3f} V")
-    out = {"plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
-           "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]}
+    per_seed, tails = [], []
+    for seed in seeds:
+        ps = replace(p, seed=seed, until_steady=True, max_batches=max_batches)
+        t0 = time.perf_counter()
+        res = charging.run(ps, log=None, injection="device")
+        wall = time.perf_counter() - t0
+        nb = res["timing"]["batches"]
+        plain = {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb, "wall_s": wall,
+                 "setup_s": res["timing"]["setup_s"], "bottom_center_V": res["steady"]["bottom center"],
+                 "t_conv_s": res["t_conv"]}
+        if long_run > 0:
+            more = charging.run(replace(ps, until_steady=False, n_batches=long_run), log=None, injection="device",
+                                resume=res)
+            tail = np.asarray(more["hist"]["bottom center"][nb:])
+            tails.append(tail)
+            plain["continued_mean_V"] = float(tail.mean())
+        log(f"time-to-steady seed {seed}, plain: converged={res['converged']} after {nb} batches in {wall:.1f}s, "
+            f"bottom centre {plain['bottom_center_V']:.3f} V")
+        r = steady.anderson(ps)
+        chosen = {"method": r["method"], "converged": bool(r["converged"]), "batches": r["batches"],
+                  "wall_s": r["wall_s"], "setup_s": r["setup_s"], "phases": r["phases"],
+                  "bottom_center_V": r["steady"]["bottom center"]}
+        log(f"time-to-steady seed {seed}, chosen: converged={r['converged']} after {r['batches']} batches in "
+            f"{r['wall_s']:.1f}s, bottom centre {chosen['bottom_center_V']:.3f} V")
+        per_seed.append({"seed": seed, "plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
+                         "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]})
+    out = {"seeds": per_seed, "plain": per_seed[0]["plain"], "chosen": per_seed[0]["chosen"],
+           "speedup": per_seed[0]["speedup"], "bottom_center_diff_V": per_seed[0]["bottom_center_diff_V"],
+           "speedup_min": min(e["speedup"] for e in per_seed),
+           "speedup_mean": float(np.mean([e["speedup"] for e in per_seed])),
+           "max_abs_diff_vs_plain_V": max(abs(e["bottom_center_diff_V"]) for e in per_seed),
+           "all_converged": all(e["plain"]["converged"] and e["chosen"]["converged"] for e in per_seed)}
+    if tails:
+        allv = np.concatenate(tails)
+        # standard error over the seeds' continuation means (independent chains)
+        means = [t.mean() for t in tails]
+        long = {"value_V": float(allv.mean()), "batches": int(allv.size), "chains": len(tails),
+                "se_V": float(np.std(means, ddof=1) / np.sqrt(len(means))) if len(means) > 1 else None}
+        out["long_run_time_marched"] = long
+        for e in per_seed:
+            e["chosen_minus_long_run_V"] = e["chosen"]["bottom_center_V"] - long["value_V"]
+            e["plain_minus_long_run_V"] = e["plain"]["bottom_center_V"] - long["value_V"]
+        out["max_abs_diff_vs_long_run_V"] = max(abs(e["chosen_minus_long_run_V"]) for e in per_seed)
     # the precision pattern in the steady solve: FP32 push + BF16 split Green's unless asked otherwise
-    if not (push_precision or green):
-        push_precision, green = 1, 1
-    rv = steady.anderson(ps, push_precision=push_precision, green_precision=green)
+    if not (push_precision or green_precision):
+        push_precision, green_precision = 1, 1
+    ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
+    rv = steady.anderson(ps, push_precision=push_precision, green_precision=green_precision)
+    plain0, chosen0 = per_seed[0]["plain"], per_seed[0]["chosen"]
     out["chosen_precision_variant"] = {
-        "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green],
+        "seed": seeds[0], "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green_precision],
         "converged": bool(rv["converged"]), "batches": rv["batches"], "wall_s": rv["wall_s"],
-        "speedup_vs_plain": wall / rv["wall_s"], "bottom_center_V": rv["steady"]["bottom center"],
-        "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain["bottom_center_V"],
-        "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen["bottom_center_V"]}
+        "speedup_vs_plain": plain0["wall_s"] / rv["wall_s"], "bottom_center_V": rv["steady"]["bottom center"],
+        "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain0["bottom_center_V"],
+        "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen0["bottom_center_V"]}
     log(f"time-to-steady, chosen with precision variants: {rv['wall_s']:.1f}s, bottom centre "
         f"{rv['steady']['bottom center']:.3f} V")
     return out

@@ -158,6 +158,27 @@ int th_trace_tangent(th_tracer *t, const double *x, const double *y, const doubl
                      int32_t *steps, int8_t *face, double *endpoints, double *d_endpoints, double *final_velocity,
                      double *d_final_velocity);
 
+/* Adjoint (reverse-mode, pathwise) push: the transpose of th_trace_tangent's
+ * derivative.  For n particles traced through the fields last set, given
+ * the cotangent of each recorded end point bar_end[n][2] and of each final
+ * velocity bar_vel[n][2] (each nullable = 0), returns the cotangent of each
+ * initial state bar_state[n][4] (x, y, vx, vy) and, if bar_field is non-NULL,
+ * the cotangent of the vacuum-masked field {Ex*vac, Ey*vac} per cell
+ * bar_field[nx*ny][2], summed over all particles and steps: for any
+ * direction, sum over particles of <bar_end, d end> + <bar_vel, d vel> equals
+ * <bar_state, d state0> + <bar_field, d field>.  steps[n] are the particles'
+ * path lengths from th_trace_tangent (or th_trace_record) of the same
+ * particles and field; a particle whose re-traced length differs is an
+ * error.  The discrete events of each path are held fixed, as in
+ * th_trace_tangent.  Each path is re-traced once forward onto a tape in
+ * device memory (32 bytes per step; in chunks of at most 4 GiB) and swept
+ * back.  *ms (nullable) receives the kernels' time.  No reference
+ * counterpart, as th_trace_tangent (built on the push of
+ * trench_trace.cu:17-185). */
+int th_trace_adjoint(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                     const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p, const int32_t *steps,
+                     const double *bar_end, const double *bar_vel, double *bar_state, double *bar_field, double *ms);
+
 /* Device-resident particle pool, so a batch can be traced with its states
  * already in HBM.  th_particles_reserve sizes the pool (contents are lost);
  * th_particles_upload copies n host particles to pool[offset, offset+n).
@@ -260,6 +281,12 @@ void th_charging_destroy(th_charging *c);
  * Recomputes the potential and field of the current charge. */
 int th_charging_set_green(th_charging *c, int32_t mode);
 
+/* Charge per macro-particle of the following batches [C/m] (the set-up's
+ * q_macro, trench_charging_2d.py:1093-1094): a batch of f * n_per_batch
+ * particles per species at q_macro / f has the expected deposit of the full
+ * batch.  No reference counterpart (the reference's batch size is fixed). */
+int th_charging_set_q_macro(th_charging *c, double q_macro);
+
 /* Sets the surface charge density rho (m values, C/m^3) and recomputes the
  * potential and the push's field records from it. */
 int th_charging_set_rho(th_charging *c, const double *rho_s);
@@ -267,6 +294,22 @@ int th_charging_set_rho(th_charging *c, const double *rho_s);
 /* One batch: traces pool[offset, offset+n) through the current field, then
  * deposit, leak, conductor, potential and field update, on the device. */
 int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, th_batch_diag *out);
+
+/* Replicas of one batch from the current charge: pool[offset + r n,
+ * offset + (r+1) n) holds replica r's particles (r < replicas).  One push
+ * traces all of them through the current field; each replica's tallies then
+ * update its own copy of the charge (deposit, leak, conductor), exactly as
+ * th_charging_step would from the same charge: rho_out[replicas][m] receives
+ * the replicas' charge densities, bit-identical with `replicas` separate
+ * th_charging_step calls each preceded by th_charging_set_rho of the current
+ * charge.  scal_out (nullable) receives per replica {escaped ions, escaped
+ * electrons, lost, absorbed off the surface cells}; out the push's totals.
+ * The charging state (charge, potential, field) is left unchanged.  No
+ * reference counterpart: the batch body of trench_charging_2d.py:1225-1266
+ * run for several particle sets at once, so the push's tail of long paths
+ * is paid once for all of them. */
+int th_charging_step_replicas(th_charging *c, int64_t offset, int64_t n, int32_t replicas,
+                              const th_trace_params *p, double *rho_out, int64_t *scal_out, th_batch_diag *out);
 
 /* Per-batch monitors on the device (trench_charging_2d.py:1272-1285): after
  * every th_charging_step the library gathers the potential at n_probes cells,

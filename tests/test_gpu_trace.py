@@ -50,6 +50,7 @@ CASES = {
                                               q_left=4e7, q_right=-2e7),
     "ar5-marched": lambda: cases.marched_case(cases.ar5_params(), 30, 2000, 5000, ion=configs.AR5_ION),
     "ar8-marched": lambda: cases.marched_case(cases.ar8_params(), 30, 5000, 20000, ion=configs.AR8_ION),
+    "ar8-marched-1e6": lambda: cases.marched_case(cases.ar8_params(), 30, 5000, 500_000, ion=configs.AR8_ION),
     "ar8-marched-short": lambda: cases.marched_case(cases.ar8_params(max_steps=450), 30, 5000, 20000,
                                                     ion=configs.AR8_ION),
     "regression-marched-absorb": lambda: cases.marched_case(cases.regression_params(), 40, 3000, 3000),

@@ -81,6 +81,8 @@ SIGNATURES = {
     "th_trace_tangent": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                         dp, dp, dp, dp, dp, dp, ctypes.POINTER(Tally), i8p, i32p, i32p, i8p, dp, dp,
                                         dp, dp]),
+    "th_trace_adjoint": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                        i32p, dp, dp, dp, dp, dp]),
     "th_particles_upload_times": (ctypes.c_int, [vp, ctypes.c_int64, dp, ctypes.c_int64]),
     "th_particles_reserve": (ctypes.c_int, [vp, ctypes.c_int64]),
     "th_particles_upload": (ctypes.c_int, [vp, ctypes.c_int64, dp, dp, dp, dp, dp, i8p, ctypes.c_int64]),
@@ -99,10 +101,13 @@ SIGNATURES = {
     "th_charging_create": (ctypes.c_int, [vp, ctypes.POINTER(ChargingSetupC), ctypes.POINTER(vp)]),
     "th_charging_destroy": (None, [vp]),
     "th_charging_set_rho": (ctypes.c_int, [vp, dp]),
+    "th_charging_set_q_macro": (ctypes.c_int, [vp, ctypes.c_double]),
     "th_charging_set_green": (ctypes.c_int, [vp, ctypes.c_int32]),
     "th_charging_step": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.POINTER(TraceParams), ctypes.POINTER(BatchDiag)]),
     "th_charging_read": (ctypes.c_int, [vp, dp, dp, i32p]),
+    "th_charging_step_replicas": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                                 ctypes.POINTER(TraceParams), dp, i64p, ctypes.POINTER(BatchDiag)]),
     "th_charging_set_monitors": (ctypes.c_int, [vp, ctypes.c_int32, i32p, ctypes.c_int32, i32p, i32p,
                                                 ctypes.c_int32, ctypes.c_int32, i32p]),
     "th_charging_monitors": (ctypes.c_int, [vp, dp]),

@@ -324,6 +324,25 @@ class DeviceCharging:
                                                 ctypes.byref(d)))
         return d
 
+    def set_q_macro(self, q_macro):
+        """Charge per macro-particle of the following batches [C/m]."""
+        _lib.check(_lib.load().th_charging_set_q_macro(self._h, float(q_macro)))
+
+    def step_replicas(self, offset, n, replicas):
+        """``replicas`` batches from the current charge in one push (replica r's
+        particles at pool[offset + r n, offset + (r+1) n)), each updating its
+        own copy of the charge (th_charging_step_replicas).  Returns (rho
+        [replicas][m], per-replica {escaped ions, escaped electrons, lost,
+        off-surface} [replicas][4], push totals); the device state is left
+        at the current charge."""
+        rho = np.empty((replicas, self.m))
+        scal = np.empty((replicas, 4), np.int64)
+        d = _lib.BatchDiag()
+        _lib.check(_lib.load().th_charging_step_replicas(self._h, int(offset), int(n), int(replicas),
+                                                         ctypes.byref(self.prm), _lib.ptr(rho, _lib.dp),
+                                                         _lib.ptr(scal, _lib.i64p), ctypes.byref(d)))
+        return rho, scal, d
+
     def set_monitors(self, probes, regions, abs_key="surface"):
         """Per-batch device monitors: potential at the probe cells, charge per
         region (regions: name -> flat cells), |charge| of ``abs_key``, and the

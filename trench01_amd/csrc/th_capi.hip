@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/trench_hip.h"
+#include "th_adjoint.hpp"
 #include "th_charge.hpp"
 #include "th_inject.hpp"
 #include "th_lu.hpp"
@@ -745,6 +746,11 @@ struct th_charging {
     uint16_t *g_hi = nullptr, *g_lo = nullptr;
     int green_built = 0;
     double green_absmax = 0.0;
+    // replicas of one batch (th_charging_step_replicas), grown on demand
+    int rep_cap = 0;
+    uint32_t *rep_counts = nullptr;
+    unsigned long long *rep_scal = nullptr;
+    double *rep_rho = nullptr, *rep_prev = nullptr, *rep_diag = nullptr, *rep_qcol = nullptr;
     // monitors
     th::MonitorDev mon{};
     int32_t *mon_buf = nullptr;     // probe cells, region ptr, region idx, bottom (one allocation)
@@ -801,7 +807,8 @@ void th_charging_destroy(th_charging *c) {
     for (void *p : {(void *)c->col_of, (void *)c->rows, (void *)c->surf, (void *)c->green, (void *)c->external,
                     (void *)c->leak_M, (void *)c->leak_c, (void *)c->g_cond_s, (void *)c->region,
                     (void *)c->cond_s, (void *)c->rho, (void *)c->rho_prev, (void *)c->qcol, (void *)c->phi,
-                    (void *)c->diag, (void *)c->g_hi, (void *)c->g_lo})
+                    (void *)c->diag, (void *)c->g_hi, (void *)c->g_lo, (void *)c->rep_counts, (void *)c->rep_scal,
+                    (void *)c->rep_rho, (void *)c->rep_prev, (void *)c->rep_diag, (void *)c->rep_qcol})
         if (p) (void)hipFree(p);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->mon_buf) (void)hipFree(c->mon_buf);
@@ -917,6 +924,12 @@ int th_charging_set_green(th_charging *c, int32_t mode) {
     return TH_OK;
 }
 
+int th_charging_set_q_macro(th_charging *c, double q_macro) {
+    if (!c || !(q_macro > 0.0) || !std::isfinite(q_macro)) return fail(TH_EINVAL, "q_macro must be positive");
+    c->d.q_macro = q_macro;
+    return TH_OK;
+}
+
 int th_charging_set_rho(th_charging *c, const double *rho_s) {
     if (!c || !rho_s) return fail(TH_EINVAL, "invalid argument");
     th_tracer *t = c->t;
@@ -971,6 +984,84 @@ int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_p
     out->dv = diag[th::kDiagDV];
     out->n_i_cond = diag[th::kDiagNi];
     out->n_e_cond = diag[th::kDiagNe];
+    float a = 0.0f, b = 0.0f;
+    TH_HIP(hipEventElapsedTime(&a, t->ev0, t->ev1));
+    TH_HIP(hipEventElapsedTime(&b, t->ev1, c->ev2));
+    out->push_ms = a;
+    out->update_ms = b;
+    t->push_ms = a;
+    return TH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int reserve_replicas(th_charging *c, int R) {
+    if (R <= c->rep_cap) return TH_OK;
+    for (void *p : {(void *)c->rep_counts, (void *)c->rep_scal, (void *)c->rep_rho, (void *)c->rep_prev,
+                    (void *)c->rep_diag, (void *)c->rep_qcol})
+        if (p) (void)hipFree(p);
+    c->rep_counts = nullptr;
+    c->rep_scal = nullptr;
+    c->rep_rho = c->rep_prev = c->rep_diag = c->rep_qcol = nullptr;
+    c->rep_cap = 0;
+    const size_t m = (size_t)std::max(c->d.m, 1);
+    int rc;
+    if ((rc = dev_alloc(&c->rep_counts, (size_t)R * 2 * m)) || (rc = dev_alloc(&c->rep_scal, (size_t)R * 4)) ||
+        (rc = dev_alloc(&c->rep_rho, (size_t)R * m)) || (rc = dev_alloc(&c->rep_prev, (size_t)R * m)) ||
+        (rc = dev_alloc(&c->rep_diag, (size_t)R * th::kDiagCount)) ||
+        (rc = dev_alloc(&c->rep_qcol, (size_t)R * c->d.n_cols)))
+        return rc;
+    c->rep_cap = R;
+    return TH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int th_charging_step_replicas(th_charging *c, int64_t offset, int64_t n, int32_t replicas, const th_trace_params *p,
+                              double *rho_out, int64_t *scal_out, th_batch_diag *out) {
+    if (!c || !p || !rho_out || !out || offset < 0 || n < 0 || replicas < 1 || replicas > 4096)
+        return fail(TH_EINVAL, "invalid argument");
+    th_tracer *t = c->t;
+    const int64_t total = n * replicas;
+    if (offset + total > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    TH_HIP(hipSetDevice(t->device));
+    if ((rc = reserve_replicas(c, replicas))) return rc;
+    memset(out, 0, sizeof *out);
+    const int m = t->m;
+    int64_t traced = offset;
+    // one push over every replica's particles, keeping each particle's end
+    if ((rc = launch_trace(t, offset, total, p, nullptr, true, true, &traced, &out->tally))) return rc;
+    TH_HIP(hipMemsetAsync(c->rep_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(m, 1) * replicas, t->stream));
+    TH_HIP(hipMemsetAsync(c->rep_scal, 0, sizeof(unsigned long long) * 4 * (size_t)replicas, t->stream));
+    if (n > 0 && m > 0) {
+        const int per = (int)std::min<int64_t>((n + 255) / 256, std::max(1, 4 * t->cus / replicas));
+        hipLaunchKernelGGL(th::replica_tally_kernel, dim3(per, replicas), dim3(256), sizeof(uint32_t) * 2 * m,
+                           t->stream, t->d_status, t->d_hit, t->d_spc, t->d_smap, m, offset, n, c->rep_counts,
+                           c->rep_scal);
+        TH_HIP(hipGetLastError());
+    }
+    th::ReplicaDev r{c->rep_counts, c->rep_rho, c->rep_prev, c->rep_diag, c->rep_qcol};
+    hipLaunchKernelGGL(th::charge_update_replicas_kernel, dim3(replicas), dim3(th::kUpdBlock), c->update_lds,
+                       t->stream, c->d, r);
+    TH_HIP(hipGetLastError());
+    TH_HIP(hipEventRecord(c->ev2, t->stream));
+    unsigned long long scal[th::kScalCount];
+    std::vector<unsigned long long> rscal(4 * (size_t)replicas);
+    TH_HIP(hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipMemcpyAsync(rscal.data(), c->rep_scal, sizeof(unsigned long long) * rscal.size(),
+                          hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipMemcpyAsync(rho_out, c->rep_rho, sizeof(double) * (size_t)m * replicas, hipMemcpyDeviceToHost,
+                          t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    fill_tally(scal, offset + total - traced, &out->tally);
+    if (scal_out)
+        for (size_t k = 0; k < rscal.size(); ++k) scal_out[k] = (int64_t)rscal[k];
     float a = 0.0f, b = 0.0f;
     TH_HIP(hipEventElapsedTime(&a, t->ev0, t->ev1));
     TH_HIP(hipEventElapsedTime(&b, t->ev1, c->ev2));
@@ -1515,6 +1606,122 @@ extern "C" int th_trace_tangent(th_tracer *t, const double *x, const double *y, 
     TH_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     t->push_ms = ms;
     fill_tally(scal, n - traced, tally);
+    return TH_OK;
+}
+
+extern "C" int th_trace_adjoint(th_tracer *t, const double *x, const double *y, const double *vx, const double *vy,
+                                const double *qm, const int8_t *spc, int64_t n, const th_trace_params *p,
+                                const int32_t *steps, const double *bar_end, const double *bar_vel, double *bar_state,
+                                double *bar_field, double *ms) {
+    if (!t || !p || !steps || !bar_state) return fail(TH_EINVAL, "invalid argument");
+    if (n < 0) return fail(TH_EINVAL, "invalid particle count");
+    if (n > 0 && (!x || !y || !vx || !vy || !qm || !spc)) return fail(TH_EINVAL, "particle array is NULL");
+    if (p->precision != 0 || p->rf) return fail(TH_EINVAL, "the adjoint push follows the FP64 push without RF");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; ++k) {
+        if (spc[k] != 0 && spc[k] != 1) return fail(TH_EINVAL, "species must be 0 (ion) or 1 (electron)");
+        if (steps[k] < 1 || steps[k] > p->max_steps)
+            return fail(TH_EINVAL, "steps[%lld] = %d is outside [1, max_steps]", (long long)k, steps[k]);
+    }
+    if (bar_field) memset(bar_field, 0, sizeof(double) * 2 * (size_t)t->N);
+    if (ms) *ms = 0.0;
+    if (n == 0) {
+        return TH_OK;
+    }
+    TH_HIP(hipSetDevice(t->device));
+    if ((rc = reserve_particles(t, n))) return rc;
+    if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
+    // Lanes take particles longest first (waves of similar path lengths); the
+    // tape is filled chunk by chunk, at most kTapeSteps steps (32 bytes each) at once.
+    const size_t nn = (size_t)n;
+    std::vector<int32_t> order(nn);
+    for (size_t k = 0; k < nn; ++k) order[k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
+    constexpr int64_t kTapeSteps = (int64_t)1 << 27;   // 4 GiB of tape per chunk
+    std::vector<int64_t> off(nn);
+    std::vector<std::pair<int64_t, int64_t>> chunks;
+    int64_t run = 0, begin = 0, tape_steps = 0;
+    for (size_t o = 0; o < nn; ++o) {
+        const int64_t len = steps[order[o]];
+        if (run + len > kTapeSteps && run > 0) {
+            chunks.emplace_back(begin, (int64_t)o);
+            tape_steps = std::max(tape_steps, run);
+            begin = (int64_t)o;
+            run = 0;
+        }
+        off[order[o]] = run;
+        run += len;
+    }
+    chunks.emplace_back(begin, (int64_t)nn);
+    tape_steps = std::max(tape_steps, run);
+    int32_t *d_order = nullptr, *d_steps = nullptr;
+    int64_t *d_off = nullptr;
+    double *d_bar = nullptr, *d_field = nullptr;
+    double4 *d_tape = nullptr;
+    unsigned long long *d_mis = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(t->stream);
+        for (void *q : {(void *)d_order, (void *)d_steps, (void *)d_off, (void *)d_bar, (void *)d_field,
+                        (void *)d_tape, (void *)d_mis})
+            if (q) (void)hipFree(q);
+    };
+    if ((rc = dev_alloc(&d_order, nn)) || (rc = dev_alloc(&d_steps, nn)) || (rc = dev_alloc(&d_off, nn)) ||
+        (rc = dev_alloc(&d_bar, 8 * nn)) || (bar_field && (rc = dev_alloc(&d_field, 2 * (size_t)t->N))) ||
+        (rc = dev_alloc(&d_tape, (size_t)tape_steps)) || (rc = dev_alloc(&d_mis, 1))) {
+        release();
+        return rc;
+    }
+    double *d_end = d_bar, *d_vel = d_bar + 2 * nn, *d_state = d_bar + 4 * nn;
+    hipError_t e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_steps, steps, sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && bar_end)
+        e = hipMemcpyAsync(d_end, bar_end, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && bar_vel)
+        e = hipMemcpyAsync(d_vel, bar_vel, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && d_field) e = hipMemsetAsync(d_field, 0, sizeof(double) * 2 * (size_t)t->N, t->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_mis, 0, sizeof(unsigned long long), t->stream);
+    if (e != hipSuccess) {
+        release();
+        return fail(TH_EHIP, "adjoint upload: %s", hipGetErrorString(e));
+    }
+    th::Grid g = make_grid(t, p);
+    th::AdjointIO a{t->d_x, t->d_y, t->d_vx, t->d_vy, t->d_qm, d_order, d_steps, d_off, d_tape,
+                    bar_end ? d_end : nullptr, bar_vel ? d_vel : nullptr, d_state, d_field, d_mis, 0, 0,
+                    p->max_steps};
+    e = hipEventRecord(t->ev0, t->stream);
+    for (const auto &ch : chunks) {
+        if (e != hipSuccess) break;
+        a.begin = ch.first;
+        a.end = ch.second;
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ch.second - ch.first + 255) / 256,
+                                                                          (int64_t)t->cus * 8));
+        auto kernel = p->barrier ? (p->tapered ? th::push64_adjoint_kernel<true, true>
+                                               : th::push64_adjoint_kernel<true, false>)
+                                 : (p->tapered ? th::push64_adjoint_kernel<false, true>
+                                               : th::push64_adjoint_kernel<false, false>);
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, t->stream, g, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess) e = hipEventRecord(t->ev1, t->stream);
+    unsigned long long mismatch = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bar_state, d_state, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && bar_field)
+        e = hipMemcpyAsync(bar_field, d_field, sizeof(double) * 2 * (size_t)t->N, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&mismatch, d_mis, sizeof mismatch, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    float el = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&el, t->ev0, t->ev1);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "adjoint push: %s", hipGetErrorString(e));
+    if (ms) *ms = el;
+    if (mismatch)
+        return fail(TH_EINVAL, "%llu particle(s) took a different number of steps than `steps` gives: the steps "
+                               "must come from a trace of the same particles and field",
+                    mismatch);
     return TH_OK;
 }
 

@@ -91,10 +91,9 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
     return scratch[32];
 }
 
-// Deposit, leak, conductor and column charges: one workgroup over the m-vector.
-__global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {
-    extern __shared__ double sh[];          // q_old[m], q_new[m]
-    __shared__ double scratch[40];
+// Deposit, leak, conductor and column charges of one batch over the m-vector
+// (one workgroup).
+__device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, double *scratch) {
     double *q_old = sh, *q_new = sh + d.m;
     const double dx2 = d.dx * d.dx;
     const int m = d.m;
@@ -161,6 +160,74 @@ __global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {
     if (threadIdx.x == 0)
         for (int k = 0; k < m; ++k)
             if (d.col_of[k] >= 0) d.qcol[d.col_of[k]] += d.rho[k] * dx2;
+}
+
+__global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {
+    extern __shared__ double sh[];          // q_old[m], q_new[m]
+    __shared__ double scratch[40];
+    charge_update(d, sh, scratch);
+}
+
+// Replicas of one batch from one charge state: workgroup r applies the update
+// of replica r's tallies to its own copy of the charge, so each replica's
+// result equals that of charge_update_kernel run on its tallies alone.
+struct ReplicaDev {
+    const uint32_t *counts;     // [R][2][m]
+    double *rho, *rho_prev;     // [R][m]
+    double *diag;               // [R][kDiagCount]
+    double *qcol;               // [R][n_cols]
+};
+
+__global__ void __launch_bounds__(kUpdBlock) charge_update_replicas_kernel(ChargeDev d, ReplicaDev r) {
+    extern __shared__ double sh[];
+    __shared__ double scratch[40];
+    const int b = blockIdx.x;
+    ChargeDev e = d;
+    e.rho = r.rho + (int64_t)b * d.m;
+    e.rho_prev = r.rho_prev + (int64_t)b * d.m;
+    e.counts = r.counts + (int64_t)b * 2 * d.m;
+    e.diag = r.diag + (int64_t)b * kDiagCount;
+    e.qcol = r.qcol + (int64_t)b * d.n_cols;
+    for (int k = threadIdx.x; k < d.m; k += blockDim.x) e.rho[k] = d.rho[k];
+    __syncthreads();
+    charge_update(e, sh, scratch);
+}
+
+// Per-replica tallies from the push's per-particle end records: particles
+// [offset + r n, offset + (r+1) n) belong to replica r (blockIdx.y).  Counts
+// [R][2][m] per surface cell; scal [R][4] = {escaped ions, escaped electrons,
+// lost, absorbed off the surface cells}.  Both are cleared by the caller.
+__global__ void __launch_bounds__(256) replica_tally_kernel(const int8_t *__restrict__ status,
+                                                            const int32_t *__restrict__ hit,
+                                                            const int8_t *__restrict__ spc,
+                                                            const int32_t *__restrict__ smap, int m, int64_t offset,
+                                                            int64_t n, uint32_t *counts, unsigned long long *scal) {
+    extern __shared__ uint32_t lds[];   // [2][m]
+    __shared__ unsigned long long lscal[4];
+    const int r = blockIdx.y;
+    for (int k = threadIdx.x; k < 2 * m; k += blockDim.x) lds[k] = 0u;
+    if (threadIdx.x < 4) lscal[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int64_t base = offset + (int64_t)r * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = base + i;
+        const int st = status[q];
+        const int sp = spc[q] != 0;
+        if (st == 1) {
+            const int k = smap[hit[q]];
+            if (k >= 0) atomicAdd(&lds[sp * m + k], 1u);
+            else atomicAdd(&lscal[3], 1ull);
+        } else if (st == 2) {
+            atomicAdd(&lscal[sp], 1ull);
+        } else {
+            atomicAdd(&lscal[2], 1ull);
+        }
+    }
+    __syncthreads();
+    uint32_t *out = counts + (int64_t)r * 2 * m;
+    for (int k = threadIdx.x; k < 2 * m; k += blockDim.x)
+        if (lds[k]) atomicAdd(&out[k], lds[k]);
+    if (threadIdx.x < 4 && lscal[threadIdx.x]) atomicAdd(&scal[4 * r + threadIdx.x], lscal[threadIdx.x]);
 }
 
 // phi[rows[r]] = sum_c G[r][c] qcol[c] + external[rows[r]]: one wave per row,

@@ -29,6 +29,14 @@ of the map itself:
   sampling error) and carry the secant pairs over: the first stage finds the
   fixed point of a small particle sample, later ones refine it for the
   sampling error of R*K batches in a few evaluations;
+* a stage may trace a fraction f of the particles per batch at 1/f the charge
+  per particle: the expected deposit, and so the expected map and its
+  Jacobian, are those of the full batch (the conductor's one-batch step has
+  the same root), at a fraction of the cost where a batch is short enough
+  for the push's tail of long paths to dominate;
+* the R replicas of a one-batch evaluation start from the same charge, so
+  they share its field and are traced by a single push
+  (th_charging_step_replicas), which pays the tail once;
 * then plain batches with fresh particles run until the reference criterion
   (window W, hold) passes; the reported steady values are their last-window
   means, as the plain method reports its own (:1342-1343).
@@ -45,7 +53,22 @@ import numpy as np
 from . import monitors
 from .charging import ChargingSetup, DeviceCharging
 
-DEFAULT_STAGES = ((1, 1, 0.02), (1, 4, 0.003), (1, 8, 0.002))
+# (K batches per replica, R replicas, tolerance [V], particle fraction): a
+# coarse approach from zero charge with a quarter of the particles per batch
+# (at four times the charge each: the same expected batch map), a refinement
+# at half, and the final solve at full size over 24 replicas.
+DEFAULT_STAGES = ((1, 2, 0.01, 0.25), (1, 4, 0.003, 0.5), (1, 24, 0.001, 1.0))
+# Batch size (particles per species) the default tolerances are set for; the
+# sampling noise of a batch map scales as 1 / sqrt(n_per_batch), and so do
+# the default tolerances for other batch sizes.
+DEFAULT_STAGES_N = 500_000
+
+
+def default_stages(p):
+    """DEFAULT_STAGES with the tolerances scaled to ``p.n_per_batch``'s
+    sampling noise (never below the defaults)."""
+    scale = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
+    return tuple((K, R, tol * scale, f) for K, R, tol, f in DEFAULT_STAGES)
 
 
 def anderson_update(pairs, f, gx, reg=1e-12):
@@ -134,7 +157,7 @@ class _Session:
     with common random numbers, the potential coordinates, and the plain
     phase with the reference criterion."""
 
-    def __init__(self, p, device, setup, push_precision, green_precision):
+    def __init__(self, p, device, setup, push_precision, green_precision, replicas=1, merge=True):
         self.t_start = time.perf_counter()
         self.p = p
         self.s = s = setup or ChargingSetup(p, device=device)
@@ -144,24 +167,44 @@ class _Session:
             dev.set_green_precision(green_precision)
         dev.set_monitors(s.probes, s.regions)
         self.n = 2 * p.n_per_batch
-        dev.reserve(self.n)
+        self.merge = merge
+        dev.reserve(self.n * (max(1, replicas) if merge else 1))
         self.to_u, self.to_rho = potential_map(s)
         self.setup_s = time.perf_counter() - self.t_start
-        self.traced = 0
+        self.traced = 0        # batches traced, in units of full batches
+        self.frac = 1.0
+
+    def set_fraction(self, frac):
+        """Batches of ``frac`` times the particles at 1/frac the charge per
+        particle (the same expected deposit per batch)."""
+        self.frac = frac
+        self.dev.set_q_macro(self.s.q_macro / frac)
 
     def evaluate(self, rho, first, K, R):
         """The batch map from charge ``rho``: R replicas of K batches (device
         injection, batch indices first .. first + R*K - 1), surface potentials
         averaged over the replicas.  Returns (charge of the last replica,
-        mean surface potentials)."""
+        mean surface potentials).  With K = 1 the replicas share the field of
+        ``rho`` and are traced by one push (th_charging_step_replicas): the
+        same results as R separate batches, with the push's tail of long
+        paths paid once."""
         p, dev = self.p, self.dev
+        n_per = max(1, int(round(p.n_per_batch * self.frac)))
+        n = 2 * n_per
+        if K == 1 and self.merge:
+            dev.set_rho(rho)
+            for r in range(R):
+                dev.inject_device(r * n, n_per, p.seed, first + r)
+            rhos, _, _ = dev.step_replicas(0, n, R)
+            self.traced += R * self.frac
+            return rhos[-1], np.mean([self.to_u(x) for x in rhos], axis=0)
         us = []
         for r in range(R):
             dev.set_rho(rho)
             for b in range(K):
-                dev.inject_device(0, p.n_per_batch, p.seed, first + r * K + b)
-                dev.step(0, self.n)
-            self.traced += K
+                dev.inject_device(0, n_per, p.seed, first + r * K + b)
+                dev.step(0, n)
+            self.traced += K * self.frac
             rho_g = dev.read(phi=False, counts=False)[0]
             us.append(self.to_u(rho_g))
         return rho_g, np.mean(us, axis=0)
@@ -172,6 +215,7 @@ class _Session:
         are the last-window means (:1342-1343)."""
         p, dev, s = self.p, self.dev, self.s
         probes = s.probes
+        self.set_fraction(1.0)
         dev.set_rho(rho)
         hist = {k: [] for k in probes}
         qhist = {k: [] for k in (*s.regions, "surface absolute")}
@@ -208,26 +252,36 @@ class _Session:
         self.dev.close()
 
 
-def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None,
-             push_precision=0, green_precision=0):
+def anderson(p, stages=None, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None,
+             push_precision=0, green_precision=0, merge=True, secant_push_precision=None):
     """Steady state by multilevel Anderson acceleration of the batch map with
     common random numbers, then plain batches to the reference criterion.
-    ``stages`` lists (K, R, tol): K batches per replica, R replicas averaged
-    per evaluation, stop when no surface potential moves by tol volts (or
-    after ``patience`` evaluations without progress).  ``push_precision`` /
+    ``stages`` lists (K, R, tol[, f]): K batches per replica, R replicas
+    averaged per evaluation, stop when no surface potential moves by tol
+    volts (or after ``patience`` evaluations without progress); f (default 1)
+    the fraction of the batch's particles traced, at 1/f the charge each.
+    Default: ``default_stages(p)``.  ``push_precision`` /
     ``green_precision`` select the precision variants of the batch (FP32
-    push; BF16 / FP16 split Green's matrix).
+    push; BF16 / FP16 split Green's matrix).  ``merge``: trace the replicas
+    of a K = 1 evaluation in one push (same results).
+    ``secant_push_precision`` (if given) is the push precision of the secant
+    stages only; the plain phase then runs with ``push_precision``.
     Returns a dict: converged, batches (all traced), phases, steady (probe ->
     last-window mean), hist (probe histories of the plain phase), rho_s,
     wall_s and setup_s."""
     log = log or (lambda s: None)
-    ss = _Session(p, device, setup, push_precision, green_precision)
+    stages = default_stages(p) if stages is None else stages
+    ss = _Session(p, device, setup, push_precision, green_precision, max(st[1] for st in stages), merge)
     pairs = []
     rho = np.zeros(ss.dev.m)
     x = ss.to_u(rho)
     evals_per_stage = []
     first = 0
-    for K, R, tol in stages:
+    if secant_push_precision is not None:
+        ss.dev.set_schedule(precision=secant_push_precision)
+    for stage in stages:
+        K, R, tol = stage[:3]
+        ss.set_fraction(stage[3] if len(stage) > 3 else 1.0)
         box = {"like": rho}
 
         def g(x, first=first, K=K, R=R):
@@ -243,6 +297,8 @@ def anderson(p, stages=DEFAULT_STAGES, memory=10, max_evals=40, patience=10, log
         evals_per_stage.append(e)
         first += K * R
     secant_batches = ss.traced
+    if secant_push_precision is not None:
+        ss.dev.set_schedule(precision=push_precision)
     done, b2, steady, hist, rho_s = ss.plain_phase(rho, log)
     wall = time.perf_counter() - ss.t_start
     ss.close()
@@ -340,7 +396,7 @@ def newton_krylov(p, K=1, R=4, h_volts=0.5, krylov=30, max_newton=3, tol=0.003, 
     evaluation with common random numbers), then plain batches to the
     reference criterion.  Same return dict as ``anderson``."""
     log = log or (lambda s: None)
-    ss = _Session(p, device, setup, 0, 0)
+    ss = _Session(p, device, setup, 0, 0, R)
     rho = np.zeros(ss.dev.m)
     box = {"like": rho}
 

@@ -326,6 +326,40 @@ class HipTracer:
                    total_steps=int(tally.steps))
         return out
 
+    def adjoint(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, steps, bar_end=None, bar_vel=None, field=True):
+        """Reverse-mode pathwise derivative (th_trace_adjoint): the transpose
+        of ``tangent``.  ``steps`` are the particles' path lengths from a
+        ``tangent`` (or record) trace of the same particles and field;
+        ``bar_end`` / ``bar_vel`` ([n, 2] each, None = 0) the cotangents of the
+        recorded end points and final velocities.  Returns (bar_state [n, 4]
+        = cotangent of (x, y, vx, vy) at launch, bar_field [nx, ny, 2] =
+        cotangent of the vacuum-masked (Ex, Ey) per cell, or None)."""
+        L = _lib.load()
+        n = len(x)
+        nx, ny = self.vac.shape
+        if any(np.shape(v) != (n,) for v in (x, y, vx, vy, qm, spc, steps)):
+            raise ValueError("particle arrays differ in length")
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        self.set_fields(Ex, Ey, barrier)
+        xs = [f64(v) for v in (x, y, vx, vy, qm)]
+        spc = np.ascontiguousarray(spc, dtype=np.int8)
+        steps = np.ascontiguousarray(steps, dtype=np.int32)
+        be, bv = f64(bar_end), f64(bar_vel)
+        for b in (be, bv):
+            if b is not None and b.shape != (n, 2):
+                raise ValueError("cotangents must be [n, 2]")
+        prm = self.params_for(p, barrier is not None)
+        prm.precision = 0
+        bar_state = np.empty((n, 4))
+        bar_field = np.empty((nx, ny, 2)) if field else None
+        ms = ctypes.c_double(0.0)
+        _lib.check(L.th_trace_adjoint(self._h, *(_lib.ptr(v, _lib.dp) for v in xs), _lib.ptr(spc, _lib.i8p), n,
+                                      ctypes.byref(prm), _lib.ptr(steps, _lib.i32p), _lib.ptr(be, _lib.dp),
+                                      _lib.ptr(bv, _lib.dp), _lib.ptr(bar_state, _lib.dp),
+                                      _lib.ptr(bar_field, _lib.dp), ctypes.byref(ms)))
+        self.last_adjoint_ms = ms.value
+        return bar_state, bar_field
+
     def selftest_math(self, a, b):
         """Device hypot / sqrt / divide / fmod on arrays (for arithmetic pinning)."""
         a = np.ascontiguousarray(a, dtype=np.float64)
