@@ -63,8 +63,9 @@ def parse():
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady measurement")
     ap.add_argument("--max-batches", type=int, default=5000)
     ap.add_argument("--steady-seeds", default="0,1,2,3", help="seeds of the time-to-steady comparison")
-    ap.add_argument("--long-run", type=int, default=300,
-                    help="plain batches run on past each plain convergence for the long-run time-marched mean")
+    ap.add_argument("--long-run", type=int, default=2000,
+                    help="plain batches run on past the first seed's plain convergence; the mean over the last "
+                         "three quarters is the long-run time-marched value (0: skip)")
     ap.add_argument("--no-host-array", action="store_true", help="skip the host-array (PCIe-inclusive) tracer rate")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
     ap.add_argument("--green", type=int, default=0, choices=(0, 1, 2),
@@ -480,10 +481,10 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
     1e6 particles per batch), per seed: plain time-marching until the
     reference's criterion passes (W = 30, hold = 2), and the chosen method
     (Anderson solve of the batch map's fixed point, then plain batches to the
-    same criterion), with the bottom-centre potential of each.  Each plain run
-    is then continued ``long_run`` batches: the mean over all seeds'
-    continuations is the long-run time-marched value (the plain run stops
-    while still drifting up towards it).  The chosen method with the
+    same criterion), with the bottom-centre potential of each.  The first
+    seed's plain run is then continued ``long_run`` batches: the mean over
+    the last three quarters is the long-run time-marched value (the plain run
+    stops while still drifting up towards it).  The chosen method with the
     precision variants (FP32 push, BF16 split Green's update, unless others
     are given) runs on the first seed.  Wall times include each method's host
     set-up."""
@@ -498,12 +499,12 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
         plain = {"method": "plain time-marching", "converged": bool(res["converged"]), "batches": nb, "wall_s": wall,
                  "setup_s": res["timing"]["setup_s"], "bottom_center_V": res["steady"]["bottom center"],
                  "t_conv_s": res["t_conv"]}
-        if long_run > 0:
+        if long_run > 0 and not tails:
+            # one long time-marched chain: the first seed's plain run continued
+            # long_run batches; the first quarter is left to relax
             more = charging.run(replace(ps, until_steady=False, n_batches=long_run), log=None, injection="device",
                                 resume=res)
-            tail = np.asarray(more["hist"]["bottom center"][nb:])
-            tails.append(tail)
-            plain["continued_mean_V"] = float(tail.mean())
+            tails.append(np.asarray(more["hist"]["bottom center"][nb + long_run // 4:]))
         log(f"time-to-steady seed {seed}, plain: converged={res['converged']} after {nb} batches in {wall:.1f}s, "
             f"bottom centre {plain['bottom_center_V']:.3f} V")
         r = steady.anderson(ps)
@@ -521,11 +522,12 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
            "max_abs_diff_vs_plain_V": max(abs(e["bottom_center_diff_V"]) for e in per_seed),
            "all_converged": all(e["plain"]["converged"] and e["chosen"]["converged"] for e in per_seed)}
     if tails:
-        allv = np.concatenate(tails)
-        # standard error over the seeds' continuation means (independent chains)
-        means = [t.mean() for t in tails]
-        long = {"value_V": float(allv.mean()), "batches": int(allv.size), "chains": len(tails),
-                "se_V": float(np.std(means, ddof=1) / np.sqrt(len(means))) if len(means) > 1 else None}
+        tail = tails[0]
+        # standard error of the mean of an autocorrelated series by batch means (10 blocks)
+        blocks = np.array([b.mean() for b in np.array_split(tail, 10)])
+        long = {"value_V": float(tail.mean()), "batches_averaged": int(tail.size), "seed": seeds[0],
+                "batches_after_plain_stop": [int(long_run // 4), int(long_run)],
+                "se_V": float(blocks.std(ddof=1) / np.sqrt(blocks.size))}
         out["long_run_time_marched"] = long
         for e in per_seed:
             e["chosen_minus_long_run_V"] = e["chosen"]["bottom_center_V"] - long["value_V"]

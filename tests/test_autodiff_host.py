@@ -26,6 +26,9 @@ class LinearBatch:
     def charge_jvp(self, rho, drho):
         return self.B @ rho, self.B @ drho
 
+    def vjp(self, rho, g):
+        return self.B @ rho, self.B.T @ g, self.A.T @ g
+
 
 def test_ion_increment_backward_and_jvp():
     b = LinearBatch()

@@ -194,3 +194,56 @@ def test_face_deposit_weights_scale_each_contribution():
     b = tangent.face_deposit(vac, surface, spc[1:], *(x[1:] for x in rec), 1.0)
     np.testing.assert_allclose(dw, 2.0 * a[0] - 0.5 * b[0], atol=1e-12)
     np.testing.assert_allclose(ddw, 2.0 * a[1] - 0.5 * b[1], atol=1e-12)
+
+
+def test_face_deposit_vjp_is_the_transpose_of_its_derivative():
+    """sum(gbar * ddep) = sum(bar_end * d_endpoints) for random cotangents and
+    end-point tangents over absorbed, escaped and live particles on x- and
+    y-faces of a real tapered geometry, with and without weights."""
+    p = replace(model.Params(), **{**configs.AR8, "nx": 120})
+    geo = model.Geometry(p)
+    vac, surface = geo.vac, geo.surface
+    nx, ny = vac.shape
+    rng = np.random.default_rng(4)
+    n = 4000
+    hit = rng.choice(surface, n).astype(np.int32)
+    cx, cy = hit // ny, hit % ny
+    face = rng.integers(0, 2, n).astype(np.int8)
+    u = rng.uniform(-0.2, 1.2, n)                       # some crossing points clipped to the face's end
+    side = rng.integers(0, 2, n)
+    ex = np.where(face == 1, (cx + side) % nx, cx + u) * p.dx
+    ey = np.where(face == 1, cy + u, cy + side) * p.dx
+    end = np.stack([ex, ey], axis=1)
+    status = rng.choice(np.array([1, 1, 1, 2, 3], np.int8), n)
+    spc = rng.integers(0, 2, n).astype(np.int8)
+    dend = rng.standard_normal((n, 2)) * p.dx
+    gbar = rng.standard_normal((2, surface.size))
+    for w in (None, rng.uniform(0.5, 2.0, n)):
+        _, ddep = tangent.face_deposit(vac, surface, spc, status, hit, face, end, dend, p.dx, weights=w)
+        bar = tangent.face_deposit_vjp(vac, surface, spc, status, hit, face, end, p.dx, gbar, weights=w)
+        lhs, rhs = float((gbar * ddep).sum()), float((bar * dend).sum())
+        assert lhs == pytest.approx(rhs, rel=1e-12)
+        assert np.abs(bar[status != 1]).max() == 0.0
+        assert np.count_nonzero(bar) > 100                 # a non-trivial share of the particles
+
+
+@pytest.mark.parametrize("top_bc", ["dirichlet", "neumann"])
+def test_linear_field_transpose_is_the_adjoint_of_the_field_stencil(top_bc):
+    """<bE, linear_field(phi)> = <linear_field_transpose(bE), phi> on a
+    tapered AR 8 geometry (periodic x, substrate and top ghost rows)."""
+    p = replace(model.Params(), **{**configs.AR8, "top_bc": top_bc})
+    geo = model.Geometry(p)
+    rng = np.random.default_rng(8)
+    phi = rng.standard_normal(geo.solid.shape)
+    bex, bey = rng.standard_normal((2, *geo.solid.shape))
+    ex, ey = tangent.linear_field(phi, geo.vac, p)
+    bphi = tangent.linear_field_transpose(bex, bey, geo.vac, p)
+    lhs, rhs = float((bex * ex).sum() + (bey * ey).sum()), float((bphi * phi).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    # one unit cotangent at a time reproduces a column of the stencil
+    e = np.zeros(geo.solid.shape)
+    ix, iy = 60, p.floor_t + p.trench_d // 2
+    e[ix, iy] = 1.0
+    col = tangent.linear_field_transpose(e, np.zeros_like(e), geo.vac, p)
+    phi2 = rng.standard_normal(geo.solid.shape)
+    assert float((col * phi2).sum()) == pytest.approx(tangent.linear_field(phi2, geo.vac, p)[0][ix, iy], rel=1e-12)

@@ -10,8 +10,8 @@ carries the result through autograd:
   Jacobian of two tangent pushes.
 * ``ChargeIncrement.apply(rho_s, batch)``: the increment as a function of the
   surface charge; forward-mode JVPs by one tangent push along the tangent's
-  charge direction; backward by one tangent push per surface cell (exact,
-  m pushes).
+  charge direction; backward (vector-Jacobian product) by one adjoint push
+  (batch.vjp: the transpose of the tangent, for all surface cells at once).
 * ``SigmaIncrement.apply(sigma, batch, rho_s)``: the increment as a function
   of the ion angular sigma [deg], differentiated by the likelihood ratio
   (batch.sigma_score_gradient): an unbiased estimate of the derivative of
@@ -60,12 +60,7 @@ class ChargeIncrement(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         g = grad.detach().cpu().numpy().astype(float)
-        m = ctx.rho.size
-        out = np.empty(m)
-        for k in range(m):
-            e = np.zeros(m)
-            e[k] = 1.0
-            out[k] = ctx.batch.charge_jvp(ctx.rho, e)[1] @ g
+        out = ctx.batch.vjp(ctx.rho, g)[1]
         return torch.as_tensor(out, dtype=grad.dtype, device=grad.device), None
 
 

@@ -119,32 +119,30 @@ def charge_direction(fields, vac, p, drho):
     return dphi, dex, dey
 
 
-def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx, weights=None):
-    """Face-smoothed deposit per species on the surface cells and its
-    derivative: (dep, ddep), each [2, m] in particles (``surface`` = flat
-    indices of the surface cells, ascending).  ``weights`` (per particle)
-    scale each particle's contribution to both."""
+def _face_geometry(vac, surface, spc, status, hit, face, endpoints, dx, weights=None):
+    """Per absorbed particle on a surface cell: its index, species, weight,
+    hit and neighbour surface indices, the neighbour's share |u - 1/2| and
+    the slope of that share with respect to the crossing point's coordinate
+    along the face (d share / d coordinate [1/m], zero where the crossing
+    point is clipped to the face's end or no neighbour takes a share), and
+    which end-point coordinate runs along the face (1: y, for an x-face)."""
     vac = np.asarray(vac, dtype=bool)
     nx, ny = vac.shape
     m = surface.size
     smap = np.full(vac.size, -1, dtype=np.int64)
     smap[surface] = np.arange(m)
-    dep, ddep = np.zeros((2, m)), np.zeros((2, m))
     a = np.flatnonzero(status == 1)
     a = a[smap[hit[a]] >= 0]   # hit cells off the surface carry no surface charge
-    if a.size == 0:
-        return dep, ddep
     c = hit[a].astype(np.int64)
     cx, cy = c // ny, c % ny
     xf = face[a] == 1
-    e, de = endpoints[a], d_endpoints[a]
+    e = endpoints[a]
     sp = np.asarray(spc)[a].astype(np.int64)
     wt = np.ones(a.size) if weights is None else np.asarray(weights, dtype=float)[a]
     # coordinate along the face and the side of the face the particle came from
     along = np.where(xf, e[:, 1] / dx - cy, np.mod(e[:, 0] / dx - cx, nx))
     across = np.where(xf, np.mod(e[:, 0] / dx - cx, nx), e[:, 1] / dx - cy)
-    d_along = np.where(xf, de[:, 1], de[:, 0]) / dx
-    d_along = np.where((along > 0.0) & (along < 1.0), d_along, 0.0)   # a clipped crossing point stays put
+    moving = (along > 0.0) & (along < 1.0)   # a clipped crossing point stays put
     along = np.clip(along, 0.0, 1.0)
     side = np.where(np.rint(across) % nx == 1, 1, -1)   # +1: from the cell after, -1: from the cell before
     step = np.where(along >= 0.5, 1, -1)
@@ -159,13 +157,75 @@ def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, d
     nb = nbx_c * ny + nby_c
     valid = inside & ~vac[nbx_c, nby_c] & vac[fx_c, fy_c] & (smap[nb] >= 0)
     w_nb = np.where(valid, np.abs(along - 0.5), 0.0)
-    dw_nb = np.where(valid, np.sign(along - 0.5) * d_along, 0.0)
-    k_hit, k_nb = smap[c], smap[np.where(valid, nb, c)]
+    slope = np.where(valid & moving, np.sign(along - 0.5), 0.0) / dx
+    return a, sp, wt, smap[c], smap[np.where(valid, nb, c)], w_nb, slope, xf
+
+
+def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx, weights=None):
+    """Face-smoothed deposit per species on the surface cells and its
+    derivative: (dep, ddep), each [2, m] in particles (``surface`` = flat
+    indices of the surface cells, ascending).  ``weights`` (per particle)
+    scale each particle's contribution to both."""
+    m = surface.size
+    dep, ddep = np.zeros((2, m)), np.zeros((2, m))
+    a, sp, wt, k_hit, k_nb, w_nb, slope, xf = _face_geometry(vac, surface, spc, status, hit, face, endpoints, dx,
+                                                             weights)
+    if a.size == 0:
+        return dep, ddep
+    de = d_endpoints[a]
+    dw_nb = slope * np.where(xf, de[:, 1], de[:, 0])
     for s in (0, 1):
         sel = sp == s
         dep[s] = np.bincount(k_hit[sel], wt[sel] * (1.0 - w_nb[sel]), m) + np.bincount(k_nb[sel], wt[sel] * w_nb[sel], m)
         ddep[s] = np.bincount(k_hit[sel], -wt[sel] * dw_nb[sel], m) + np.bincount(k_nb[sel], wt[sel] * dw_nb[sel], m)
     return dep, ddep
+
+
+def face_deposit_vjp(vac, surface, spc, status, hit, face, endpoints, dx, gbar, weights=None):
+    """Transpose of face_deposit's derivative: for cotangents ``gbar`` [2, m]
+    of the deposit, the cotangent of every particle's recorded end point
+    [n, 2], so that sum(gbar * ddep) = sum(bar_end * d_endpoints) for any
+    d_endpoints."""
+    bar = np.zeros((len(status), 2))
+    a, sp, wt, k_hit, k_nb, _, slope, xf = _face_geometry(vac, surface, spc, status, hit, face, endpoints, dx,
+                                                          weights)
+    if a.size:
+        coef = wt * slope * (gbar[sp, k_nb] - gbar[sp, k_hit])
+        bar[a[xf], 1] = coef[xf]
+        bar[a[~xf], 0] = coef[~xf]
+    return bar
+
+
+def linear_field_transpose(bex, bey, vac, p):
+    """Transpose of ``linear_field`` (E = -grad(phi) by the field stencil with
+    zero ghost voltages): for cotangents (bex, bey) of the field on the grid,
+    the cotangent of phi, so that sum(bex Ex + bey Ey) = sum(bphi phi)."""
+    vac = np.asarray(vac, dtype=bool)
+    dx = p.dx
+    bex = np.asarray(bex, dtype=float)
+    bey = np.asarray(bey, dtype=float)
+    out = np.zeros(vac.shape)
+    # x: neighbours west (i-1) and east (i+1), periodic
+    v_lo, v_hi = np.roll(vac, 1, 0), np.roll(vac, -1, 0)
+    both, only_hi, only_lo = v_lo & v_hi, v_hi & ~v_lo, v_lo & ~v_hi
+    c_hi = -bex * (both / (2 * dx) + only_hi / dx)
+    c_lo = bex * (both / (2 * dx) + only_lo / dx)
+    out += bex * (only_hi / dx - only_lo / dx)
+    out += np.roll(c_hi, 1, 0)     # coefficient of phi[i+1] lands on cell i+1
+    out += np.roll(c_lo, -1, 0)    # coefficient of phi[i-1] lands on cell i-1
+    # y: neighbours below (j-1) and above (j+1); below the grid no vacuum, above
+    # it the ghost row (Dirichlet: -phi of the top cell, Neumann: +phi)
+    edge = np.zeros((vac.shape[0], 1), dtype=bool)
+    v_lo = np.concatenate([edge, vac[:, :-1]], axis=1)
+    v_hi = np.concatenate([vac[:, 1:], ~edge], axis=1)
+    both, only_hi, only_lo = v_lo & v_hi, v_hi & ~v_lo, v_lo & ~v_hi
+    c_hi = -bey * (both / (2 * dx) + only_hi / dx)
+    c_lo = bey * (both / (2 * dx) + only_lo / dx)
+    out += bey * (only_hi / dx - only_lo / dx)
+    out[:, 1:] += c_hi[:, :-1]
+    out[:, -1] += c_hi[:, -1] * (-1.0 if p.top_bc == "dirichlet" else 1.0)
+    out[:, :-1] += c_lo[:, 1:]
+    return out
 
 
 class BatchDerivative:
@@ -275,6 +335,34 @@ class PathwiseBatch:
         inc[self.cond_s] = 0.0
         grad[self.cond_s] = 0.0
         return inc, grad
+
+    def vjp(self, rho_s, g, sigma_deg=None, energy_shift=0.0):
+        """Reverse-mode derivative of <g, increment> from one primal trace and
+        one adjoint push (th_trace_adjoint): returns (increment [m], gradient
+        with respect to the surface charge rho_s [m], gradient with respect
+        to (ion angular sigma [deg], energy shift [eV]) [2]).  The transpose of
+        charge_jvp and ion_jacobian: the same pathwise derivative (events held
+        fixed), for all m + 2 directions at once."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else np.asarray(rho_s, dtype=float)
+        g = np.asarray(g, dtype=float)
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        parts, tan = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        barrier = (phi, self.fields.wface) if self.p.wall_model == "barrier" else None
+        r = BatchDerivative(self.tracer, self.p, ex, ey, barrier)(parts)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        inc[self.cond_s] = 0.0
+        gs = np.where(self.cond_s, 0.0, g) * self.scale
+        bar_end = face_deposit_vjp(self.geo.vac, self.surface, parts[5], r["status"], r["hit"], r["face"],
+                                   r["endpoints"], self.p.dx, np.stack([gs, -gs]))
+        bar_state, bar_field = self.tracer.adjoint(*parts, ex, ey, self.p, barrier, r["steps"], bar_end)
+        vac = self.geo.vac
+        bphi = linear_field_transpose(bar_field[..., 0] * vac, bar_field[..., 1] * vac, vac, self.p)
+        s = self.setup
+        per_col = s.green.T @ bphi.ravel()[s.rows]            # d<g, inc> / d(column charge) [per C/m]
+        grad_rho = np.where(s.col_of >= 0, per_col[np.maximum(s.col_of, 0)], 0.0) * self.p.dx ** 2
+        ion = np.array([bar_state[:, 2] @ tan[k][0] + bar_state[:, 3] @ tan[k][1] for k in ("sigma", "energy")])
+        return inc, grad_rho, ion
 
     def charge_jvp(self, rho_s, drho_s, sigma_deg=None, energy_shift=0.0):
         """(increment [m], d increment along the surface-charge direction drho_s [m])."""
