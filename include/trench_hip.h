@@ -48,6 +48,8 @@ typedef struct th_trace_params {
                              later passes continue the survivors with doubling caps; 0 = one pass */
     int32_t rf;           /* 1: RF full-cycle push (th_set_rf, launch times per particle), FP64 only */
     int32_t blocks_per_cu; /* schedule: resident workgroups per CU at most, 0 = as many as fit */
+    int32_t tile_pass;    /* schedule: passes from this one on (1-based) stage the field of the columns
+                             around the trench in LDS (FP64, non-RF, multi-pass only); 0 = never */
 } th_trace_params;
 
 /* Scalar tallies of one trace call (trench_cuda.py:119-122). */
@@ -310,6 +312,24 @@ int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_p
  * is paid once for all of them. */
 int th_charging_step_replicas(th_charging *c, int64_t offset, int64_t n, int32_t replicas,
                               const th_trace_params *p, double *rho_out, int64_t *scal_out, th_batch_diag *out);
+
+/* `batches` (at most 1024) consecutive batches of the device-resident loop
+ * without the host: each batch injects 2 * ip->n particles into
+ * pool[offset, offset + 2 ip->n) on the device (th_inject's distributions and
+ * stream, batch indices first_batch, first_batch + 1, ...), traces them and
+ * updates charge, potential and field exactly as th_inject + th_charging_step,
+ * and records a row: the monitors of th_charging_monitors (none if unset),
+ * then {escaped ions, escaped electrons, lost, steps, absorbed off the
+ * surface}, then {leak_dq, drift, dv, n_i_cond, n_e_cond} (th_batch_diag).
+ * rows[batches][th_charging_row_size].  The batch is captured once as a
+ * hipGraph (fill kernels, no memset nodes; re-captured when the offset,
+ * schedule, injection, precision or monitors change) and replayed; *ms
+ * (nullable) receives the GPU time of the replays.  No RF push.  No
+ * reference counterpart: the batch loop of trench_charging_2d.py:1225-1285
+ * with its host steps between batches removed. */
+int th_charging_row_size(const th_charging *c, int32_t *size);
+int th_charging_run(th_charging *c, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t first_batch,
+                    int32_t batches, const th_trace_params *p, double *rows, double *ms);
 
 /* Per-batch monitors on the device (trench_charging_2d.py:1272-1285): after
  * every th_charging_step the library gathers the potential at n_probes cells,

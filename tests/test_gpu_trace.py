@@ -163,3 +163,28 @@ def test_multi_pass_schedule_does_not_change_results(precision):
         np.testing.assert_array_equal(st, s0)
         np.testing.assert_array_equal(h, h0)
         assert nsteps == n0
+
+
+@pytest.mark.parametrize("name", ["ar8-marched", "ar5-marched", "taper-barrier"])
+def test_tile_push_does_not_change_results(name):
+    """The late passes with the trench's field staged in LDS (tile_pass) give
+    the plain multi-pass push's results bit for bit: per-particle status and
+    hit cell, tallies and total steps, whichever pass the tile starts at."""
+    _print_case(f"tile push {name}")
+    c = CASES[name]()
+    p = c["p"]
+    parts = c["particles"]
+    runs = []
+    for pass_steps, tile_pass in ((100, 0), (100, 1), (100, 3), (37, 2), (250, 4)):
+        t = HipTracer(c["vac"])
+        t.pass_steps = pass_steps
+        t.tile_pass = tile_pass
+        r = t(*parts, c["Ex"], c["Ey"], c["vac"], p, c["barrier"], keep_status=True)
+        runs.append((r, t.last_status.copy(), t.last_hit.copy(), t.last_tally["steps"]))
+    (r0, s0, h0, n0) = runs[0]
+    for r, st, h, nsteps in runs[1:]:
+        for a, b in zip(r, r0):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(st, s0)
+        np.testing.assert_array_equal(h, h0)
+        assert nsteps == n0

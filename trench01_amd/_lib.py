@@ -22,7 +22,7 @@ class TraceParams(ctypes.Structure):
                 ("tapered", ctypes.c_int32), ("barrier", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("refill", ctypes.c_int32), ("chunk", ctypes.c_int32), ("order", ctypes.c_int32),
                 ("prio_steps", ctypes.c_int32), ("check_steps", ctypes.c_int32), ("pass_steps", ctypes.c_int32),
-                ("rf", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32)]
+                ("rf", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("tile_pass", ctypes.c_int32)]
 
 
 class Tally(ctypes.Structure):
@@ -102,6 +102,9 @@ SIGNATURES = {
     "th_charging_destroy": (None, [vp]),
     "th_charging_set_rho": (ctypes.c_int, [vp, dp]),
     "th_charging_set_q_macro": (ctypes.c_int, [vp, ctypes.c_double]),
+    "th_charging_row_size": (ctypes.c_int, [vp, i32p]),
+    "th_charging_run": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(InjectParams), ctypes.c_uint64,
+                                       ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(TraceParams), dp, dp]),
     "th_charging_set_green": (ctypes.c_int, [vp, ctypes.c_int32]),
     "th_charging_step": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.POINTER(TraceParams), ctypes.POINTER(BatchDiag)]),

@@ -17,6 +17,7 @@ draws it on the GPU (Philox, same distributions, different stream).
 """
 import ctypes
 import time
+from types import SimpleNamespace
 from dataclasses import asdict
 
 import numpy as np
@@ -318,6 +319,49 @@ class DeviceCharging:
         _lib.check(_lib.load().th_inject(self.tracer._h, int(offset), ctypes.byref(spec), int(seed) & (2**64 - 1),
                                          int(batch)))
 
+    def run_batches(self, offset, n_per_species, seed, first_batch, batches):
+        """``batches`` device-injected batches replayed from one captured
+        hipGraph (th_charging_run).  Returns a list of per-batch records
+        (probe potentials dict, region charges dict incl. 'surface absolute',
+        floor ion count, floor electron count, tally dict, diag dict) and the
+        GPU time [ms]."""
+        p = self.setup.p
+        spec = ion_spec(p)
+        spec.n = int(n_per_species)
+        spec.lx = p.nx * p.dx
+        spec.y0 = self.setup.geo.ny * p.dx - 1e-3 * p.dx
+        L = _lib.load()
+        size = ctypes.c_int32()
+        _lib.check(L.th_charging_row_size(self._h, ctypes.byref(size)))
+        out = []
+        total_ms = 0.0
+        done = 0
+        while done < batches:
+            k = min(1024, batches - done)
+            rows = np.empty((k, size.value))
+            ms = ctypes.c_double()
+            _lib.check(L.th_charging_run(self._h, int(offset), ctypes.byref(spec), int(seed) & (2**64 - 1),
+                                         int(first_batch + done), k, ctypes.byref(self.prm), _lib.ptr(rows, _lib.dp),
+                                         ctypes.byref(ms)))
+            total_ms += ms.value
+            npb = len(getattr(self, "monitor_probes", []))
+            nr = len(getattr(self, "monitor_regions", []))
+            nm = npb + nr + 3 if npb or nr else 0
+            for r in rows:
+                probes = dict(zip(self.monitor_probes, r[:npb].tolist())) if nm else {}
+                charges = dict(zip(self.monitor_regions, r[npb:npb + nr].tolist())) if nm else {}
+                if nm:
+                    charges["surface absolute"] = float(r[npb + nr])
+                t = r[nm:nm + 5]
+                tally = dict(escaped=(int(t[0]), int(t[1])), lost=int(t[2]), steps=int(t[3]), off_surface=int(t[4]))
+                dg = r[nm + 5:nm + 10]
+                diag = dict(leak_dq=float(dg[0]), drift=float(dg[1]), dv=float(dg[2]), n_i_cond=float(dg[3]),
+                            n_e_cond=float(dg[4]))
+                out.append((probes, charges, float(r[npb + nr + 1]) if nm else 0.0,
+                            float(r[npb + nr + 2]) if nm else 0.0, tally, diag))
+            done += k
+        return out, total_ms
+
     def step(self, offset, n):
         d = _lib.BatchDiag()
         _lib.check(_lib.load().th_charging_step(self._h, int(offset), int(n), ctypes.byref(self.prm),
@@ -444,7 +488,7 @@ def resumed_steady_count(resume, p, start_batch, history_start, hist, probes, ch
 
 
 def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=None, ion_velocity=None,
-        device=0, resume=None, device_monitors=True, circuit=None):
+        device=0, resume=None, device_monitors=True, circuit=None, graph=True):
     """Charging run (trench_charging_2d.py:1045-1369) with the batch on the GPU.
     ``resume``: a result of ``run`` or ``results.load_results``; its charge,
     histories and random state are continued (fixed length: n_batches more;
@@ -452,7 +496,10 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     for field_mode "circuit" / "circuit_rf" (the reference's
     PC.solve_circuit_cached output: t, uw, um, up, V1 over one period); with
     "circuit_rf" every batch draws stratified launch phases and the push
-    follows the voltage wave in flight.  Returns the reference's result dict
+    follows the voltage wave in flight.  ``graph``: device-injected runs
+    without a progress callback replay a captured batch up to each steady
+    check (same results; ``stop`` is then polled between such blocks and
+    timing.push_ms holds the whole batches' GPU time).  Returns the reference's result dict
     (plus ``timing``), or None if stopped before the first batch."""
     log = log or (lambda s: None)
     start_batch = 0
@@ -559,25 +606,44 @@ def run(p, log=print, progress=None, stop=None, injection="host", ion_energy=Non
     push_ms = update_ms = 0.0
     phi = fields.external.ravel().copy()
     t0 = time.time()
+    # Device-injected runs without per-batch host work replay a captured batch
+    # (th_charging_run) up to the next steady check, then read the records.
+    replay = graph and injection == "device" and device_monitors and phase_rng is None and progress is None
+    pending = []
     for b in range(start_batch, n_max):
         if stop is not None and stop.is_set():
             stopped = True
             break
-        if injection == "host":
-            dev.tracer.upload(0, *model.inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity))
+        if replay:
+            if not pending:
+                k = n_max - b
+                if p.until_steady:
+                    k = min(k, p.steady_window - b % p.steady_window)
+                records, ms = dev.run_batches(0, p.n_per_batch, p.seed, b, k)
+                push_ms += ms
+                pending = records
+            values, charges, n_i_floor, n_e_floor, tally, dg = pending.pop(0)
+            d = SimpleNamespace(tally=SimpleNamespace(escaped=tally["escaped"], lost=tally["lost"],
+                                                      off_surface=tally["off_surface"]), leak_dq=dg["leak_dq"])
         else:
-            dev.inject_device(0, p.n_per_batch, p.seed, b)
-        if phase_rng is not None:
-            dev.tracer.upload_times(0, np.concatenate(
-                [model.launch_times(phase_rng, p.n_per_batch, fields.rf["period"]) for _ in range(2)]))
-        d = dev.step(0, n)
+            if injection == "host":
+                dev.tracer.upload(0, *model.inject(p, p.n_per_batch, ny, rng, ion_energy, ion_velocity))
+            else:
+                dev.inject_device(0, p.n_per_batch, p.seed, b)
+            if phase_rng is not None:
+                dev.tracer.upload_times(0, np.concatenate(
+                    [model.launch_times(phase_rng, p.n_per_batch, fields.rf["period"]) for _ in range(2)]))
+            d = dev.step(0, n)
+            push_ms += d.push_ms
+            update_ms += d.update_ms
         if d.tally.off_surface:
             raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
-        push_ms += d.push_ms
-        update_ms += d.update_ms
         hist["t"].append((b + 1) * p.dt_batch)
         norm = p.n_per_batch * frac_open
-        if device_monitors:
+        if replay:
+            for k in probes:
+                hist[k].append(values[k])
+        elif device_monitors:
             values, charges, n_i_floor, n_e_floor = dev.monitors()
             for k in probes:
                 hist[k].append(values[k])

@@ -15,6 +15,7 @@
 #include "th_inject.hpp"
 #include "th_lu.hpp"
 #include "th_push.hpp"
+#include "th_util.hpp"
 
 namespace {
 
@@ -98,7 +99,8 @@ struct th_tracer {
     int8_t *d_spc = nullptr, *d_status = nullptr;
     int32_t *d_hit = nullptr;
     double *d_t0 = nullptr;                          // RF launch times
-    th::Survivor *d_surv[2] = {nullptr, nullptr};   // multi-pass carry lists (ping-pong)
+    th::SurvivorRF *d_surv[2] = {nullptr, nullptr}; // multi-pass carry lists (ping-pong; the non-RF
+                                                    // push stores 40-byte Survivor records in them)
     unsigned *d_surv_count = nullptr;                // per-pass list lengths
     // tallies
     uint32_t *d_counts = nullptr, *d_overflow = nullptr;
@@ -106,6 +108,7 @@ struct th_tracer {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the push launches
     double push_ms = 0.0;
     int blocks_per_cu = 0;                     // launch cap of the current trace (0: occupancy limit)
+    int tile_c0 = 0, tile_w = 0;               // columns the tile push stages in LDS (0: no tile)
     char name[256] = "";
 };
 
@@ -199,6 +202,8 @@ th::Grid make_grid(const th_tracer *t, const th_trace_params *p) {
     g.dt_max = t->dt_max;
     g.bound0 = t->bound[0];
     g.bound1 = t->bound[1];
+    g.tile_c0 = t->tile_c0;
+    g.tile_w = t->tile_w;
     return g;
 }
 
@@ -226,6 +231,25 @@ int launch_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t w
     if (precision == 1) return launch_kernel(t, th::push32_kernel<B, T>, g, w, want, lds);
     if (record) return launch_kernel(t, th::push64_record_kernel<B, T>, g, w, want, lds);
     return launch_kernel(t, th::push64_kernel<B, T>, g, w, want, lds);
+}
+
+// The FP64 push with the tile columns in LDS: one workgroup per CU (the tile
+// fills most of the CU's LDS).
+template <bool B, bool T>
+int launch_tile_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want) {
+    const size_t lds = th::tile_offset(t->m) + th::tile_bytes(t->tile_w, t->ny);
+    auto kernel = th::push64_tile_kernel<B, T>;
+    TH_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return launch_kernel(t, kernel, g, w, want, lds);
+}
+
+int launch_tile(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered) {
+    int64_t n = w.end - w.begin;
+    if (n <= 0 && !w.in) return TH_OK;
+    int64_t want = w.in ? (int64_t)t->cus : (n + th::kBlock - 1) / th::kBlock;
+    if (barrier)
+        return tapered ? launch_tile_variant<true, true>(t, g, w, want) : launch_tile_variant<true, false>(t, g, w, want);
+    return tapered ? launch_tile_variant<false, true>(t, g, w, want) : launch_tile_variant<false, false>(t, g, w, want);
 }
 
 int launch_push(th_tracer *t, const th::Grid &g, const th::Work &w, bool barrier, bool tapered, int precision,
@@ -286,6 +310,26 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
     t->cus = prop.multiProcessorCount;
     if (2 * (size_t)t->m * sizeof(uint32_t) > 96 * 1024)
         return bad(fail(TH_EINVAL, "%d surface cells exceed the LDS tally capacity", t->m));
+    {
+        // LDS tile of the late passes: the widest band of whole columns whose
+        // field and vacuum bits fit the CU's LDS beside the tallies, centred on
+        // the vacuum cells of the grid's lower half (the trench, where the
+        // particles that live longest spend most of their steps).
+        const long lds = (long)prop.maxSharedMemoryPerMultiProcessor - (long)th::tile_offset(t->m) - 512;
+        int w = 0;
+        while (w < nx && lds > 0 && th::tile_bytes(w + 1, ny) <= (size_t)lds) ++w;
+        double sum = 0.0;
+        long count = 0;
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iy = 0; iy < ny / 2; ++iy)
+                if (t->vac[(size_t)ix * ny + iy]) {
+                    sum += ix + 0.5;
+                    ++count;
+                }
+        const int centre = count ? (int)(sum / count) : nx / 2;
+        t->tile_w = w >= 2 ? w : 0;
+        t->tile_c0 = t->tile_w ? std::max(0, std::min(nx - t->tile_w, centre - t->tile_w / 2)) : 0;
+    }
     if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess)
         return bad(fail(TH_EHIP, "hipStreamCreate failed"));
     if (hipEventCreate(&t->ev0) != hipSuccess || hipEventCreate(&t->ev1) != hipSuccess)
@@ -450,7 +494,11 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
                 w.out = last ? nullptr : t->d_surv[pass & 1];
                 w.out_count = last ? nullptr : t->d_surv_count + pass;
                 TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
-                if ((rc = launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false, p->rf != 0)))
+                const bool tile = p->tile_pass > 0 && pass + 1 >= p->tile_pass && t->tile_w > 0 && p->precision == 0 &&
+                                  !p->rf;
+                if ((rc = tile ? launch_tile(t, g, w, p->barrier != 0, p->tapered != 0)
+                               : launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false,
+                                             p->rf != 0)))
                     return rc;
                 if (last) break;
                 cap = std::min<int64_t>(2 * cap, p->max_steps);
@@ -751,6 +799,16 @@ struct th_charging {
     uint32_t *rep_counts = nullptr;
     unsigned long long *rep_scal = nullptr;
     double *rep_rho = nullptr, *rep_prev = nullptr, *rep_diag = nullptr, *rep_qcol = nullptr;
+    // one device-injected batch captured as a hipGraph (th_charging_run)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    th_trace_params graph_params{};
+    th::InjectDev graph_inject{};
+    th::ChargeDev graph_d{};
+    th::MonitorDev graph_mon{};
+    int64_t graph_offset = -1;
+    unsigned long long *d_run = nullptr;   // {batch index, first batch of the run}
+    double *d_hist = nullptr;              // [kRunRows][row] per-batch records
     // monitors
     th::MonitorDev mon{};
     int32_t *mon_buf = nullptr;     // probe cells, region ptr, region idx, bottom (one allocation)
@@ -811,6 +869,10 @@ void th_charging_destroy(th_charging *c) {
                     (void *)c->rep_rho, (void *)c->rep_prev, (void *)c->rep_diag, (void *)c->rep_qcol})
         if (p) (void)hipFree(p);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->d_run) (void)hipFree(c->d_run);
+    if (c->d_hist) (void)hipFree(c->d_hist);
     if (c->mon_buf) (void)hipFree(c->mon_buf);
     if (c->mon_out) (void)hipFree(c->mon_out);
     delete c;
@@ -1068,6 +1130,198 @@ int th_charging_step_replicas(th_charging *c, int64_t offset, int64_t n, int32_t
     out->push_ms = a;
     out->update_ms = b;
     t->push_ms = a;
+    return TH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+static_assert(th::kScalEsc0 == 1 && th::kScalEsc1 == 2 && th::kScalLost == 3 && th::kScalSteps == 4 &&
+                  th::kScalOff == 5,
+              "batch_record_kernel reads the push's scalar tallies by these indices");
+
+constexpr int kRunRows = 1024;   // batches per th_charging_run call at most
+
+int run_row(const th_charging *c) { return c->mon.n_probes + c->mon.n_regions + 3 + 5 + th::kDiagCount; }
+
+int zero_u32(th_tracer *t, uint32_t *p, int64_t n) {
+    hipLaunchKernelGGL(th::zero_u32_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024))),
+                       dim3(256), 0, t->stream, p, n);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+int zero_u64(th_tracer *t, unsigned long long *p, int64_t n) {
+    hipLaunchKernelGGL(th::zero_u64_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024))),
+                       dim3(256), 0, t->stream, p, n);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+// One batch as kernels only (capturable): injection at the device batch
+// index, tally clears by fills, the push passes, deposit / leak / conductor,
+// potential and field, monitors, and the batch's record.  The same kernels
+// in the same order as th_inject + th_charging_step, so the results are the
+// same.
+int charging_batch_kernels(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p,
+                           const th::InjectDev &inj) {
+    th_tracer *t = c->t;
+    int rc;
+    hipLaunchKernelGGL(th::inject_at_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, t->stream, inj,
+                       c->d_run);
+    TH_HIP(hipGetLastError());
+    if ((rc = zero_u32(t, t->d_counts, 2 * (int64_t)std::max(t->m, 1))) ||
+        (rc = zero_u32(t, t->d_overflow, 2 * (int64_t)t->N)) || (rc = zero_u64(t, t->d_scal, th::kScalCount)))
+        return rc;
+    t->blocks_per_cu = p->blocks_per_cu;
+    th::Grid g = make_grid(t, p);
+    th::Work w;
+    w.x = t->d_x; w.y = t->d_y; w.vx = t->d_vx; w.vy = t->d_vy; w.qm = t->d_qm; w.spc = t->d_spc;
+    w.t0 = t->d_t0;
+    w.status = nullptr;
+    w.hit = nullptr;
+    w.counts = t->d_counts;
+    w.overflow = t->d_overflow;
+    w.scal = t->d_scal;
+    w.max_steps = p->max_steps;
+    w.refill = p->refill > 0 ? std::min(p->refill, 64) : 16;
+    w.order = p->order != 0;
+    w.prio_steps = p->prio_steps > 0 ? p->prio_steps : 0;
+    w.check_steps = p->check_steps > 0 ? std::min(p->check_steps, 256) : th::kStepsPerCheck;
+    w.rec = th::Record{};
+    w.tan = th::TangentIO{};
+    w.begin = offset;
+    w.end = offset + n;
+    const bool passes = p->pass_steps > 0 && p->pass_steps < p->max_steps;
+    if ((rc = zero_u32(t, t->d_surv_count, kMaxPasses + 1))) return rc;
+    int64_t cap = passes ? p->pass_steps : p->max_steps;
+    for (int pass = 0;; ++pass) {
+        const bool last = cap >= p->max_steps || pass == kMaxPasses - 1;
+        w.step_cap = last ? p->max_steps : (int)cap;
+        w.in = pass ? t->d_surv[(pass - 1) & 1] : nullptr;
+        w.in_count = pass ? t->d_surv_count + pass - 1 : nullptr;
+        w.out = last ? nullptr : t->d_surv[pass & 1];
+        w.out_count = last ? nullptr : t->d_surv_count + pass;
+        if ((rc = zero_u64(t, t->d_scal, 1))) return rc;   // work counter
+        const bool tile = p->tile_pass > 0 && pass + 1 >= p->tile_pass && t->tile_w > 0 && p->precision == 0;
+        if ((rc = tile ? launch_tile(t, g, w, p->barrier != 0, p->tapered != 0)
+                       : launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false, false)))
+            return rc;
+        if (last) break;
+        cap = std::min<int64_t>(2 * cap, p->max_steps);
+    }
+    hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
+    TH_HIP(hipGetLastError());
+    if ((rc = launch_field(c))) return rc;
+    const int n_mon = c->mon_out ? c->mon.n_probes + c->mon.n_regions + 3 : 0;
+    if (c->mon_out) {
+        hipLaunchKernelGGL(th::monitor_kernel, dim3(1), dim3(256), 0, t->stream, c->d, c->mon);
+        TH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(th::batch_record_kernel, dim3(1), dim3(64), 0, t->stream, c->mon_out, n_mon, t->d_scal, c->diag,
+                       c->d_run, c->d_hist, run_row(c), kRunRows);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
+void free_charging_graph(th_charging *c) {
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    c->graph_offset = -1;
+}
+
+// Captures the batch once per (offset, schedule, injection, charging state
+// layout, monitors): everything the captured kernels take by value.
+int charging_capture(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, const th::InjectDev &inj) {
+    if (c->exec && c->graph_offset == offset && memcmp(&c->graph_params, p, sizeof *p) == 0 &&
+        memcmp(&c->graph_inject, &inj, sizeof inj) == 0 && memcmp(&c->graph_d, &c->d, sizeof c->d) == 0 &&
+        memcmp(&c->graph_mon, &c->mon, sizeof c->mon) == 0)
+        return TH_OK;
+    free_charging_graph(c);
+    th_tracer *t = c->t;
+    TH_HIP(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    int rc = charging_batch_kernels(c, offset, n, p, inj);
+    hipGraph_t graph = nullptr;
+    hipError_t end = hipStreamEndCapture(t->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (end != hipSuccess) return fail(TH_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(end));
+    c->graph = graph;
+    hipError_t inst = hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0);
+    if (inst != hipSuccess) {
+        free_charging_graph(c);
+        return fail(TH_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(inst));
+    }
+    c->graph_offset = offset;
+    c->graph_params = *p;
+    c->graph_inject = inj;
+    c->graph_d = c->d;
+    c->graph_mon = c->mon;
+    return TH_OK;
+}
+
+th::InjectDev inject_dev(th_tracer *t, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t batch) {
+    th::InjectDev d{};
+    d.n = ip->n; d.lx = ip->lx; d.y0 = ip->y0; d.qm_i = ip->qm_i; d.qm_e = ip->qm_e; d.ion_kind = ip->ion_kind;
+    d.e_ion = ip->e_ion; d.e_lo = ip->e_lo; d.e_hi = ip->e_hi; d.sigma_theta = ip->sigma_theta;
+    d.vth_i = ip->vth_i; d.v_per_sqrt_ev = ip->v_per_sqrt_ev; d.s_e = ip->s_e;
+    d.seed = seed;
+    d.batch = batch;
+    d.x = t->d_x + offset; d.y = t->d_y + offset; d.vx = t->d_vx + offset; d.vy = t->d_vy + offset;
+    d.qm = t->d_qm + offset; d.spc = t->d_spc + offset;
+    return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+int th_charging_row_size(const th_charging *c, int32_t *size) {
+    if (!c || !size) return fail(TH_EINVAL, "invalid argument");
+    *size = run_row(c);
+    return TH_OK;
+}
+
+int th_charging_run(th_charging *c, int64_t offset, const th_inject_params *ip, uint64_t seed, uint64_t first_batch,
+                    int32_t batches, const th_trace_params *p, double *rows, double *ms) {
+    if (!c || !ip || !p || offset < 0 || batches < 0 || (batches > 0 && !rows)) return fail(TH_EINVAL, "invalid argument");
+    if (batches > kRunRows) return fail(TH_EINVAL, "at most %d batches per call", kRunRows);
+    if (ip->ion_kind < 0 || ip->ion_kind > 2) return fail(TH_EINVAL, "unknown ion distribution %d", ip->ion_kind);
+    if (ip->n < 1) return fail(TH_EINVAL, "a run needs particles");
+    if (p->rf) return fail(TH_EINVAL, "the captured batch has no RF push (launch times come from the host)");
+    th_tracer *t = c->t;
+    const int64_t n = 2 * ip->n;
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    if (ms) *ms = 0.0;
+    if (batches == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    const int row = run_row(c);
+    if (row > 64) return fail(TH_EINVAL, "%d monitors exceed the run record", row);
+    if (!c->d_run && (rc = dev_alloc(&c->d_run, 2))) return rc;
+    if (!c->d_hist && (rc = dev_alloc(&c->d_hist, (size_t)kRunRows * 64))) return rc;
+    const th::InjectDev inj = inject_dev(t, offset, ip, seed, 0);
+    if ((rc = charging_capture(c, offset, n, p, inj))) return rc;
+    const unsigned long long run[2] = {first_batch, first_batch};
+    TH_HIP(hipMemcpyAsync(c->d_run, run, sizeof run, hipMemcpyHostToDevice, t->stream));
+    TH_HIP(hipEventRecord(t->ev0, t->stream));
+    for (int b = 0; b < batches; ++b) TH_HIP(hipGraphLaunch(c->exec, t->stream));
+    TH_HIP(hipEventRecord(t->ev1, t->stream));
+    TH_HIP(hipMemcpyAsync(rows, c->d_hist, sizeof(double) * (size_t)batches * row, hipMemcpyDeviceToHost, t->stream));
+    TH_HIP(hipStreamSynchronize(t->stream));
+    if (c->mon_out) {   // the last batch's monitors, as after th_charging_step
+        const size_t k = c->mon_host.size();
+        memcpy(c->mon_host.data(), rows + (size_t)(batches - 1) * row, sizeof(double) * k);
+    }
+    float el = 0.0f;
+    TH_HIP(hipEventElapsedTime(&el, t->ev0, t->ev1));
+    if (ms) *ms = el;
+    t->fields = true;
     return TH_OK;
 }
 

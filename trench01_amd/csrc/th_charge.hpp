@@ -399,6 +399,29 @@ struct MonitorDev {
     double *out;                  // [n_probes + n_regions + 3]
 };
 
+// End of a captured batch: its monitors, scalar tallies and update
+// diagnostics into row (batch - first) of the run's history, then batch + 1.
+// run = {batch, first}.  Row: monitors [n_mon], {escaped ions, escaped
+// electrons, lost, steps, absorbed off the surface}, {leak_dq, drift, dv,
+// n_i_cond, n_e_cond}.
+__global__ void batch_record_kernel(const double *__restrict__ mon, int n_mon, const unsigned long long *__restrict__ scal,
+                                    const double *__restrict__ diag, unsigned long long *run, double *hist, int row,
+                                    int cap) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long b = run[0], r = b - run[1];
+    if (r < (unsigned long long)cap) {
+        double *o = hist + r * (unsigned long long)row;
+        for (int k = 0; k < n_mon; ++k) o[k] = mon[k];
+        o[n_mon + 0] = (double)scal[1];   // kScalEsc0
+        o[n_mon + 1] = (double)scal[2];   // kScalEsc1
+        o[n_mon + 2] = (double)scal[3];   // kScalLost
+        o[n_mon + 3] = (double)scal[4];   // kScalSteps
+        o[n_mon + 4] = (double)scal[5];   // kScalOff
+        for (int k = 0; k < kDiagCount; ++k) o[n_mon + 5 + k] = diag[k];
+    }
+    run[0] = b + 1;
+}
+
 __global__ void __launch_bounds__(256) monitor_kernel(ChargeDev d, MonitorDev mo) {
     __shared__ double scratch[40];
     const double dx2 = d.dx * d.dx;

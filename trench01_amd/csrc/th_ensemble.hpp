@@ -27,6 +27,7 @@
 
 #include "th_inject.hpp"
 #include "th_push.hpp"
+#include "th_util.hpp"
 
 namespace th {
 
@@ -86,7 +87,9 @@ struct LaneView {
     __device__ __forceinline__ double4 cell(int c) const { return cells[cell0 + c]; }
     __device__ __forceinline__ const double2 *corner(int key) const { return corners + 4 * (int64_t)(key0 + key); }
     __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key0 + key]; }
+    static constexpr bool tiled = false;
     __device__ __forceinline__ bool is_vac(int c) const { return vac[cell0 + c] != 0; }
+    __device__ __forceinline__ bool is_vac_at(int ix, int iy) const { return vac[cell0 + ix * ny + iy] != 0; }
     __device__ __forceinline__ double face(int c) const { return wface[cell0 + c]; }
     // nx*dx and ny*dx: the products the single-instance push forms on the host
     // (recomputed per use rather than held in four more vector registers)
@@ -159,8 +162,6 @@ __device__ __forceinline__ void ensemble_body(const EnsembleGrid &g, const Ensem
                 if (pid <= -3) {
                     Survivor c;
                     c.x = s.x; c.y = s.y; c.vx = s.vx; c.vy = s.vy;
-                    c.clock = 0.0;
-                    c.elapsed = 0.0;
                     c.pid = (int32_t)(-3 - pid);
                     c.k = k;
                     w.out[o + (unsigned)__popcll(carry & below)] = c;
@@ -309,19 +310,6 @@ __global__ void ensemble_inject_kernel(const InjectDev *__restrict__ params, con
                       pvy, pqm, sp);
         x[k] = px; y[k] = d.y0; vx[k] = pvx; vy[k] = pvy; qm[k] = pqm; spc[k] = sp;
     }
-}
-
-// Zero fills used inside captured batches (no memset nodes in a graph).
-__global__ void zero_u64_kernel(unsigned long long *p, int64_t n) {
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        p[k] = 0ull;
-}
-__global__ void zero_u32_kernel(uint32_t *p, int64_t n) {
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        p[k] = 0u;
-}
-__global__ void bump_kernel(unsigned long long *p) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *p += 1ull;
 }
 
 }  // namespace th

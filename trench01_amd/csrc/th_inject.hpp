@@ -93,6 +93,18 @@ __device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint64_t batch) {
     return seed ^ (batch >> 32) * 0x9E3779B97F4A7C15ull;
 }
 
+// inject_kernel with the batch index read from device memory (captured
+// batches advance it themselves).
+__global__ void inject_at_kernel(InjectDev d, const unsigned long long *__restrict__ batch) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 2 * d.n) return;
+    d.batch = *batch;
+    double x, vx, vy, qm;
+    int8_t sp;
+    draw_particle(d, k, stream_key(d.seed, d.batch), x, vx, vy, qm, sp);
+    d.x[k] = x; d.y[k] = d.y0; d.vx[k] = vx; d.vy[k] = vy; d.qm[k] = qm; d.spc[k] = sp;
+}
+
 __global__ void inject_kernel(InjectDev d) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 2 * d.n) return;

@@ -67,16 +67,40 @@ struct Grid {
     const double2 *__restrict__ wave;     // [n_wave] {substrate, mask} deviation from the period mean
     int n_wave;
     double period, dt_max, bound0, bound1;   // bound = max |wave| per column
+    // Columns [tile_c0, tile_c0 + tile_w) that the tile push stages in LDS
+    // (tile_w = 0: none).
+    int tile_c0, tile_w;
 
     // Record access of the FP64 step (shared with the ensemble's lane view).
     static constexpr bool taper = true;   // the TAPERED template flag decides
+    static constexpr bool tiled = false;  // records from global memory only
     __device__ __forceinline__ double4 cell(int c) const { return cells[c]; }
     __device__ __forceinline__ const double2 *corner(int key) const { return corners + 4 * (int64_t)key; }
     __device__ __forceinline__ unsigned corner_flags(int key) const { return corner_vac[key]; }
     __device__ __forceinline__ bool is_vac(int c) const { return vac[c] != 0; }
+    __device__ __forceinline__ bool is_vac_at(int ix, int iy) const { return vac[ix * ny + iy] != 0; }
     __device__ __forceinline__ double face(int c) const { return wface[c]; }
     __device__ __forceinline__ double len_x() const { return lx; }
     __device__ __forceinline__ double len_y() const { return ly; }
+};
+
+// The single-instance grid with the columns [tile_c0, tile_c0 + tile_w)
+// staged in LDS: per cell {Ex*vac, Ey*vac} (the doubles of the corner
+// records) and one vacuum bit.  A step whose four interpolation cells lie in
+// those columns reads them from LDS instead of its global corner record; the
+// vacuum test of a cell in them reads the bit.  The values are the same, so
+// the step's results are.
+struct TileView : Grid {
+    static constexpr bool tiled = true;
+    const double2 *te;     // [tile_w * ny], column-major like the grid (cell (c0 + u, j) at u * ny + j)
+    const uint32_t *tv;    // vacuum bits of the same cells, 32 per word
+    __device__ __forceinline__ bool holds(int i) const { return i >= tile_c0 && i + 1 < tile_c0 + tile_w; }
+    __device__ __forceinline__ bool tile_bit(int k) const { return (tv[k >> 5] >> (k & 31)) & 1u; }
+    __device__ __forceinline__ bool is_vac_at(int ix, int iy) const {
+        const unsigned u = (unsigned)(ix - tile_c0);
+        if (u < (unsigned)tile_w) return tile_bit((int)u * ny + iy);
+        return vac[ix * ny + iy] != 0;
+    }
 };
 
 // Record-mode outputs (trench_trace.cu:149-183, :189-192): the end point of
@@ -95,11 +119,14 @@ struct Record {
 
 // A particle carried from one pass of the push to the next (multi-pass
 // schedule): its full state and step count, so the next pass continues it
-// bit for bit.
+// bit for bit (40 bytes); the RF push also carries its phase clock and
+// flight time (56 bytes).
 struct Survivor {
     double x, y, vx, vy;
-    double clock, elapsed;   // RF phase clock and flight time
     int32_t pid, k;
+};
+struct SurvivorRF : Survivor {
+    double clock, elapsed;   // RF phase clock and flight time
 };
 
 // Tangent-mode inputs and outputs (push64_tangent_kernel): the direction's
@@ -131,9 +158,9 @@ struct Work {
     // appended to `out` instead of being counted; a later pass reads them from
     // `in` (its length in *in_count).  step_cap >= max_steps: single pass.
     int step_cap;
-    const Survivor *in;
+    const void *in;                      // Survivor, or SurvivorRF for the RF push
     const unsigned *in_count;
-    Survivor *out;
+    void *out;
     unsigned *out_count;
     TangentIO tan;                       // tangent mode only (single pass)
 };
@@ -331,22 +358,38 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     double2 e00, e10, e01, e11;
     double v00, v10, v01, v11;              // vacuum weights 1.0 / 0.0
     int i0, i1, j0, j1;
+    unsigned fl;                            // vacuum flags of (i0,j0), (i1,j0), (i0,j1), (i1,j1): bits 0-3
     if (__builtin_expect((unsigned)(i + 1) <= (unsigned)nx && (unsigned)(j + 1) <= (unsigned)ny, 1)) {
-        // one 64-byte corner record and one flag byte per step
-        const int key = (i + 1) * (ny + 1) + (j + 1);
-        // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
-        typedef double dvec4 __attribute__((ext_vector_type(4)));
-        const dvec4 *c = (const dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
-        const dvec4 a = c[0], b = c[1];
-        e00 = make_double2(a.x, a.y); e10 = make_double2(a.z, a.w);
-        e01 = make_double2(b.x, b.y); e11 = make_double2(b.z, b.w);
-        const unsigned f = g.corner_flags(key);
-        v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
-        v01 = (f & 4u) ? 1.0 : 0.0; v11 = (f & 8u) ? 1.0 : 0.0;
         i0 = i < 0 ? nx - 1 : i;
         i1 = i + 1 == nx ? 0 : i + 1;
         j0 = j < 0 ? 0 : j;
         j1 = j + 1 >= ny ? ny - 1 : j + 1;
+        unsigned f = 0u;
+        bool staged = false;
+        if constexpr (V::tiled) {
+            if (g.holds(i)) {   // four cells in the LDS tile (i0 = i, i1 = i + 1, no wrap)
+                const int b0 = (i - g.tile_c0) * ny;
+                e00 = g.te[b0 + j0]; e10 = g.te[b0 + ny + j0];
+                e01 = g.te[b0 + j1]; e11 = g.te[b0 + ny + j1];
+                f = (g.tile_bit(b0 + j0) ? 1u : 0u) | (g.tile_bit(b0 + ny + j0) ? 2u : 0u) |
+                    (g.tile_bit(b0 + j1) ? 4u : 0u) | (g.tile_bit(b0 + ny + j1) ? 8u : 0u);
+                staged = true;
+            }
+        }
+        if (!staged) {
+            // one 64-byte corner record and one flag byte per step
+            const int key = (i + 1) * (ny + 1) + (j + 1);
+            // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
+            typedef double dvec4 __attribute__((ext_vector_type(4)));
+            const dvec4 *c = (const dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
+            const dvec4 a = c[0], b = c[1];
+            e00 = make_double2(a.x, a.y); e10 = make_double2(a.z, a.w);
+            e01 = make_double2(b.x, b.y); e11 = make_double2(b.z, b.w);
+            f = g.corner_flags(key);
+        }
+        fl = f;
+        v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
+        v01 = (f & 4u) ? 1.0 : 0.0; v11 = (f & 8u) ? 1.0 : 0.0;
     } else {                                // positions outside the period or grid
         wrap_pair(i, nx, i0, i1);
         j0 = clip_cell(j, ny);
@@ -356,6 +399,7 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         e00 = make_double2(c00.x, c00.y); e10 = make_double2(c10.x, c10.y);
         e01 = make_double2(c01.x, c01.y); e11 = make_double2(c11.x, c11.y);
         v00 = c00.z; v10 = c10.z; v01 = c01.z; v11 = c11.z;
+        fl = (v00 != 0.0 ? 1u : 0u) | (v10 != 0.0 ? 2u : 0u) | (v01 != 0.0 ? 4u : 0u) | (v11 != 0.0 ? 8u : 0u);
     }
     double ex = 0.0, ey = 0.0, sw = 0.0;
     ex += w00 * e00.x; ey += w00 * e00.y; sw += w00 * v00;
@@ -412,10 +456,24 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     // Classification of the end point.
     bool escaped = s.y >= g.len_y();
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
-    bool hit = !escaped && !g.is_vac(ix * ny + iy);
+    bool hit = !escaped && !g.is_vac_at(ix, iy);
     bool corner = false;
-    if (TAPERED && g.taper && !hit && !escaped)
-        corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
+    if (TAPERED && g.taper && !hit && !escaped) {
+        const int ix0c = clip_cell((int)q0, nx), iy0c = clip_cell((int)r0, ny);
+        corner = ix != ix0c && iy != iy0c;
+        // A diagonal move whose two side cells (ix, iy0) and (ix0, iy) are both
+        // vacuum crosses no solid cell: the wall-entry search below would
+        // return the end cell, not hit.  Without a periodic wrap of the step,
+        // and with x/dx inside the grid, its dix / diy are ix - ix0 and
+        // iy - iy0, so it is skipped when the interpolation's four cells hold
+        // both side cells and flag them vacuum.
+        if (corner && xu >= 0.0 && xu < g.len_x() && s.q < (double)nx) {
+            const int xe = ix == i0 ? 0 : (ix == i1 ? 1 : -1), xs = ix0c == i0 ? 0 : (ix0c == i1 ? 1 : -1);
+            const int ye = iy == j0 ? 0 : (iy == j1 ? 2 : -1), ys = iy0c == j0 ? 0 : (iy0c == j1 ? 2 : -1);
+            if (xe >= 0 && xs >= 0 && ye >= 0 && ys >= 0 && ((fl >> (xe + ys)) & 1u) && ((fl >> (xs + ye)) & 1u))
+                corner = false;
+        }
+    }
     int cx = ix, cy = iy;
     bool face_x = false;
     if ((hit && (BARRIER || TAPERED || RECORD)) || corner) {
@@ -428,12 +486,12 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         bool x_first = tfx < tfy;
         int ax_ = x_first ? wrap_cell(ix0 + dix, nx) : ix0;
         int ay_ = x_first ? iy0 : iy0 + diy;
-        bool last = dix != 0 && diy != 0 && g.is_vac(ax_ * ny + ay_);
+        bool last = dix != 0 && diy != 0 && g.is_vac_at(ax_, ay_);
         cx = last ? wrap_cell(ix0 + dix, nx) : ax_;
         cy = last ? iy0 + diy : ay_;
         int vcx = last ? ax_ : ix0, vcy = last ? ay_ : iy0;
         face_x = last ? !x_first : x_first;
-        hit = !g.is_vac(cx * ny + cy);
+        hit = !g.is_vac_at(cx, cy);
         if (BARRIER && hit) {
             double vn = face_x ? s.vx : s.vy;
             int c = cx * ny + cy, v = vcx * ny + vcy;
@@ -601,18 +659,49 @@ __device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *l
 
 enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4, kModeF64Tangent = 5 };
 
-// Persistent refill loop shared by every push variant.
-template <int MODE, bool BARRIER, bool TAPERED>
+// LDS bytes of the tile push beyond the tallies: the tallies [2][m] (u32)
+// rounded up to 16 bytes, then tile_w * ny double2 and their vacuum bits.
+__host__ __device__ __forceinline__ size_t tile_offset(int m) { return ((size_t)2 * m * 4 + 15) & ~(size_t)15; }
+__host__ __device__ __forceinline__ size_t tile_bytes(int w, int ny) {
+    return (size_t)w * ny * 16 + (((size_t)w * ny + 31) / 32) * 4;
+}
+
+// Persistent refill loop shared by every push variant.  TILE: the FP64 push
+// with the grid's tile columns staged in LDS (TileView).
+template <int MODE, bool BARRIER, bool TAPERED, bool TILE = false>
 __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     using Real = typename std::conditional<MODE == kModeF32, float, double>::type;
     constexpr bool TANGENT = MODE == kModeF64Tangent;
     constexpr bool RECORD = MODE == kModeF64Record || MODE == kModeF64RfRecord || TANGENT;
     constexpr bool RF = MODE == kModeF64Rf || MODE == kModeF64RfRecord;
+    static_assert(!TILE || MODE == kModeF64, "the tile push is the plain FP64 push");
     using Tan = typename std::conditional<TANGENT, Tangent64, NoTangent>::type;
-    extern __shared__ uint32_t lds_counts[];   // [2][m]
+    using Surv = typename std::conditional<RF, SurvivorRF, Survivor>::type;
+    extern __shared__ uint32_t lds_counts[];   // [2][m] (then the tile, TILE only)
     __shared__ unsigned long long lds_scal[kScalCount];
     for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
     if (threadIdx.x < kScalCount) lds_scal[threadIdx.x] = 0ull;
+    TileView tview;
+    if constexpr (TILE) {
+        static_cast<Grid &>(tview) = g;
+        double2 *te = reinterpret_cast<double2 *>(reinterpret_cast<char *>(lds_counts) + tile_offset(g.m));
+        uint32_t *tv = reinterpret_cast<uint32_t *>(te + (size_t)g.tile_w * g.ny);
+        const int cells = g.tile_w * g.ny, base = g.tile_c0 * g.ny;   // the tile's columns are contiguous
+        for (int k = threadIdx.x; k < cells; k += kBlock) {
+            const double4 r = g.cells[base + k];
+            te[k] = make_double2(r.x, r.y);
+        }
+        for (int wd = threadIdx.x; wd < (cells + 31) / 32; wd += kBlock) {
+            uint32_t bits = 0u;
+            for (int b = 0; b < 32; ++b) {
+                const int k = wd * 32 + b;
+                if (k < cells && g.vac[base + k]) bits |= 1u << b;
+            }
+            tv[wd] = bits;
+        }
+        tview.te = te;
+        tview.tv = tv;
+    }
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -641,13 +730,15 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 q = w.order ? q_end - 1 - q : q_begin + q;
                 if (q >= q_begin && q < q_end) {
                     if (w.in) {
-                        const Survivor &v = w.in[q];
+                        const Surv &v = static_cast<const Surv *>(w.in)[q];
                         pid = v.pid;
                         s.x = (Real)v.x; s.y = (Real)v.y; s.vx = (Real)v.vx; s.vy = (Real)v.vy;
                         k = v.k;
                         k0 = k;
-                        rc.clock = v.clock;
-                        rc.elapsed = v.elapsed;
+                        if constexpr (RF) {
+                            rc.clock = v.clock;
+                            rc.elapsed = v.elapsed;
+                        }
                     } else {
                         pid = q;
                         s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
@@ -707,6 +798,8 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 int st;
                 if constexpr (MODE == kModeF32) {
                     st = step32<BARRIER, TAPERED>(g, s, cell);
+                } else if constexpr (TILE) {
+                    st = step64t<BARRIER, TAPERED, false, false>(tview, s, cell, xp, yp, rc, tn);
                 } else {
                     st = step64t<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc, tn);
                     if constexpr (RECORD) {
@@ -746,13 +839,15 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 }
                 if (k >= w.step_cap) {   // carry to the next pass
                     unsigned slot_out = atomicAdd(w.out_count, 1u);
-                    Survivor v;
+                    Surv v;
                     v.x = (double)s.x; v.y = (double)s.y; v.vx = (double)s.vx; v.vy = (double)s.vy;
-                    v.clock = rc.clock;
-                    v.elapsed = rc.elapsed;
+                    if constexpr (RF) {
+                        v.clock = rc.clock;
+                        v.elapsed = rc.elapsed;
+                    }
                     v.pid = (int32_t)pid;
                     v.k = k;
-                    w.out[slot_out] = v;
+                    static_cast<Surv *>(w.out)[slot_out] = v;
                     my_steps += (unsigned long long)(k - k0);
                     pid = -1;
                     break;
@@ -772,6 +867,13 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) push64_kernel(Grid g, Work w) {
     push_body<kModeF64, BARRIER, TAPERED>(g, w);
+}
+
+// FP64 push with the tile columns of the field staged in LDS (for passes
+// whose few, long-lived particles wait on the latency of each step's loads).
+template <bool BARRIER, bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_tile_kernel(Grid g, Work w) {
+    push_body<kModeF64, BARRIER, TAPERED, true>(g, w);
 }
 
 // FP64 push with record mode (diagnostics; its extra registers stay unspilled).

@@ -1,0 +1,21 @@
+// Small kernels shared by the captured batches: fills instead of memset nodes
+// (a graph replays them reliably) and a device-side counter increment.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace th {
+
+__global__ void zero_u64_kernel(unsigned long long *p, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0ull;
+}
+__global__ void zero_u32_kernel(uint32_t *p, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0u;
+}
+__global__ void bump_kernel(unsigned long long *p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p += 1ull;
+}
+
+}  // namespace th

@@ -222,14 +222,16 @@ class _Session:
         W = p.steady_window
         n_ok, done, b2 = 0, False, 0
         first = 1 << 20                              # batch indices away from the solver's
+        pending = []
         while self.traced < p.max_batches:
-            dev.inject_device(0, p.n_per_batch, p.seed, first + b2)
-            d = dev.step(0, self.n)
-            if d.tally.off_surface:
-                raise RuntimeError(f"{d.tally.off_surface} particles were absorbed off the surface cells")
+            if not pending:   # replay the captured batch up to the next check
+                k = int(min(W - b2 % W, np.ceil(p.max_batches - self.traced)))
+                pending, _ = dev.run_batches(0, p.n_per_batch, p.seed, first + b2, max(1, k))
+            values, charges, _, _, tally, _ = pending.pop(0)
+            if tally["off_surface"]:
+                raise RuntimeError(f"{tally['off_surface']} particles were absorbed off the surface cells")
             b2 += 1
             self.traced += 1
-            values, charges, _, _ = dev.monitors()
             for k in probes:
                 hist[k].append(values[k])
             for k, q in charges.items():

@@ -40,6 +40,7 @@ class HipTracer:
     check_steps = 8
     pass_steps = 250
     blocks_per_cu = 0
+    tile_pass = 0
 
     def __init__(self, vac, device=0):
         L = _lib.load()
@@ -233,7 +234,7 @@ class HipTracer:
                                 precision=0 if rf else int(self.precision), refill=int(self.refill), chunk=0,
                                 order=int(self.order), prio_steps=int(self.prio_steps),
                                 check_steps=int(self.check_steps), pass_steps=int(self.pass_steps), rf=int(bool(rf)),
-                                blocks_per_cu=int(self.blocks_per_cu))
+                                blocks_per_cu=int(self.blocks_per_cu), tile_pass=int(self.tile_pass))
 
     def upload_times(self, offset, initial_time):
         """Launch times of pool[offset, offset+n) for resident RF traces."""
