@@ -68,8 +68,9 @@ def parse():
                          "three quarters is the long-run time-marched value (0: skip)")
     ap.add_argument("--no-host-array", action="store_true", help="skip the host-array (PCIe-inclusive) tracer rate")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1), help="0: FP64 push, 1: FP32 push")
-    ap.add_argument("--green", type=int, default=0, choices=(0, 1, 2),
-                    help="potential update: 0 FP64 Green's matrix, 1 BF16 hi/lo split, 2 FP16 hi/lo split")
+    ap.add_argument("--green", type=int, default=0, choices=(0, 1, 2, 3, 4),
+                    help="potential update: 0 FP64 Green's matrix, 1 BF16 hi/lo split, 2 FP16 hi/lo split, "
+                         "3 / 4 the same splits on MFMA")
     ap.add_argument("--pass-steps", type=int, default=None,
                     help="multi-pass schedule's first cap (0: one pass); default: the library's tuned value")
     ap.add_argument("--tile-pass", type=int, default=None,
@@ -362,7 +363,9 @@ def ensemble_parity(ens, specs, fields, prm, log, sample=4, max_particles=400_00
             "oracle_sample": checked, "bit_exact": accounting_ok and all(c["bit_exact"] for c in checked)}
 
 
-GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums"}
+GREEN_NAMES = {0: "FP64", 1: "BF16 hi+lo, FP32 sums", 2: "FP16 hi+lo (scaled), FP32 sums",
+               3: "BF16 hi+lo on MFMA (charges split too), FP32 accumulation",
+               4: "FP16 hi+lo (scaled) on MFMA (charges split too), FP32 accumulation"}
 
 
 def precision_errors(dev, setup, green):
@@ -546,9 +549,9 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
             e["chosen_minus_long_run_V"] = e["chosen"]["bottom_center_V"] - long["value_V"]
             e["plain_minus_long_run_V"] = e["plain"]["bottom_center_V"] - long["value_V"]
         out["max_abs_diff_vs_long_run_V"] = max(abs(e["chosen_minus_long_run_V"]) for e in per_seed)
-    # the precision pattern in the steady solve: FP32 push + BF16 split Green's unless asked otherwise
+    # the precision pattern in the steady solve: FP32 push + BF16 split Green's on MFMA unless asked otherwise
     if not (push_precision or green_precision):
-        push_precision, green_precision = 1, 1
+        push_precision, green_precision = 1, 3
     ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
     rv = steady.anderson(ps, push_precision=push_precision, green_precision=green_precision)
     plain0, chosen0 = per_seed[0]["plain"], per_seed[0]["chosen"]

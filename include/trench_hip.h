@@ -280,8 +280,19 @@ void th_charging_destroy(th_charging *c);
  * Green's matrix; 1 the matrix split into BF16 hi + lo halves; 2 split into
  * FP16 hi + lo halves of the matrix scaled into FP16 range.  The split modes
  * form products and sums in FP32 and add the external potential in FP64.
- * Recomputes the potential and field of the current charge. */
+ * 3 and 4: the BF16 / FP16 split on matrix cores (MFMA, FP32 accumulation),
+ * with each charge split into two halves of the same type (FP16: after
+ * scaling by a power of two into its range).  Recomputes the potential and
+ * field of the current charge. */
 int th_charging_set_green(th_charging *c, int32_t mode);
+
+/* G Q for R (1..16) charge vectors at once on matrix cores, in the split
+ * precision of mode 3 or 4 (th_charging_set_green): qcols[R][n_cols] column
+ * charges [C/m], out[R][n_rows] potentials on the kept rows without the
+ * external potential [V].  Reads G once for all R.  *ms (nullable): kernel
+ * time.  No reference counterpart (the reference solves one right-hand side
+ * per LU solve, trench_charging_2d.py:1268). */
+int th_charging_potentials(th_charging *c, int32_t R, const double *qcols, double *out, double *ms);
 
 /* Charge per macro-particle of the following batches [C/m] (the set-up's
  * q_macro, trench_charging_2d.py:1093-1094): a batch of f * n_per_batch

@@ -102,3 +102,65 @@ def test_batch_runs_in_split_precision(charged):
     # same field before the push (up to the split error) -> tallies differ by a handful of particles at most
     assert np.abs(c0.astype(np.int64) - c1).sum() <= 0.01 * n
     assert np.abs(f1[s.rows] - f0[s.rows]).max() < 0.05
+
+
+@pytest.mark.parametrize("mode, scalar, tol_v, tol_e", [(3, 1, 5e-3, 1e-3), (4, 2, 1e-3, 2e-4)])
+def test_split_green_on_matrix_cores(charged, mode, scalar, tol_v, tol_e):
+    """The split update on MFMA (modes 3 / 4: the charges split into two
+    halves of the matrix's 16-bit type too) within the same bounds against
+    FP64 as the scalar split, and within 2e-5 of the potential scale of it."""
+    s, dev, _, _, _ = charged
+    rows = s.rows
+    dev.set_green_precision(0)          # the FP64 reference of the fixture's current charge
+    _, phi64, _ = dev.read(counts=False)
+    e64 = dev.tracer.read_field()
+    scale = np.abs(phi64[rows]).max()
+    try:
+        dev.set_green_precision(scalar)
+        _, phis, _ = dev.read(counts=False)
+        dev.set_green_precision(mode)
+        _, phi, _ = dev.read(counts=False)
+        err = np.abs(phi[rows] - phi64[rows])
+        print(f"mode {mode}: max |dphi| {err.max():.3g} V of {scale:.1f} V; "
+              f"vs scalar split {np.abs(phi[rows] - phis[rows]).max():.3g} V", flush=True)
+        assert err.max() < tol_v
+        assert np.abs(phi[rows] - phis[rows]).max() < 2e-5 * scale
+        ex, ey = dev.tracer.read_field()
+        emax = max(np.abs(e64[0]).max(), np.abs(e64[1]).max())
+        assert max(np.abs(ex - e64[0]).max(), np.abs(ey - e64[1]).max()) / emax < tol_e
+    finally:
+        dev.set_green_precision(0)
+    _, back, _ = dev.read(counts=False)
+    np.testing.assert_array_equal(back, phi64)
+
+
+@pytest.mark.parametrize("mode, tol", [(3, 5e-3), (4, 1e-3)])
+def test_potentials_of_many_charge_vectors(charged, mode, tol):
+    """th_charging_potentials: G Q for 16 charge vectors in one pass over G,
+    against the FP64 product on the host; one vector at a time gives each
+    column within rounding."""
+    import ctypes
+    from trench01_amd import _lib
+    s, dev, q, phi64, e64 = charged
+    rng = np.random.default_rng(5)
+    R = 16
+    Q = q[None, :] * rng.uniform(0.5, 1.5, (R, q.size))
+    ref = Q @ s.green.T                                       # [R][n_rows]
+    dev.set_green_precision(mode)
+    try:
+        out = np.empty((R, s.green.shape[0]))
+        ms = ctypes.c_double()
+        Qc = np.ascontiguousarray(Q)
+        _lib.check(_lib.load().th_charging_potentials(dev._h, R, _lib.ptr(Qc, _lib.dp), _lib.ptr(out, _lib.dp),
+                                                      ctypes.byref(ms)))
+        err = np.abs(out - ref).max()
+        print(f"mode {mode}: 16 right-hand sides, max error {err:.3g} V, kernel {ms.value * 1e3:.1f} us", flush=True)
+        assert err < tol
+        one = np.empty((1, s.green.shape[0]))
+        q1 = np.ascontiguousarray(Q[3:4])
+        _lib.check(_lib.load().th_charging_potentials(dev._h, 1, _lib.ptr(q1, _lib.dp), _lib.ptr(one, _lib.dp),
+                                                      ctypes.byref(ms)))
+        print(f"mode {mode}: 1 right-hand side, kernel {ms.value * 1e3:.1f} us", flush=True)
+        np.testing.assert_allclose(one[0], out[3], rtol=0, atol=1e-6 * np.abs(ref).max())
+    finally:
+        dev.set_green_precision(0)

@@ -103,6 +103,7 @@ SIGNATURES = {
     "th_charging_set_rho": (ctypes.c_int, [vp, dp]),
     "th_charging_set_q_macro": (ctypes.c_int, [vp, ctypes.c_double]),
     "th_charging_row_size": (ctypes.c_int, [vp, i32p]),
+    "th_charging_potentials": (ctypes.c_int, [vp, ctypes.c_int32, dp, dp, dp]),
     "th_charging_run": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.POINTER(InjectParams), ctypes.c_uint64,
                                        ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(TraceParams), dp, dp]),
     "th_charging_set_green": (ctypes.c_int, [vp, ctypes.c_int32]),

@@ -286,8 +286,8 @@ class DeviceCharging:
 
     def set_green_precision(self, mode):
         """Potential update in FP64 (0) or with the Green's matrix split into
-        BF16 (1) or scaled FP16 (2) halves, FP32 products and sums
-        (th_charging_set_green)."""
+        BF16 (1) or scaled FP16 (2) halves, FP32 products and sums; 3 / 4:
+        the same splits on matrix cores (th_charging_set_green)."""
         _lib.check(_lib.load().th_charging_set_green(self._h, int(mode)))
         self.green_precision = int(mode)
 

@@ -827,6 +827,25 @@ int upload_new(T **dst, const T *src, size_t count, hipStream_t s) {
     return TH_OK;
 }
 
+// G Q on matrix cores for R <= 16 charge vectors qcols [R][n_cols] (device):
+// into phi (batch) or out [R][n_rows].  One wave per 16 rows, four per
+// workgroup, the charge halves in LDS.
+int launch_green_mfma(th_charging *c, const double *qcols, int R, double *out, bool batch) {
+    th_tracer *t = c->t;
+    const int tiles = (c->d.n_rows + 15) / 16;
+    const int blocks = std::max(1, std::min((tiles + 3) / 4, t->cus * 8));
+    const size_t lds = sizeof(uint16_t) * 2 * 16 * (size_t)c->d.n_pad;
+    const bool bf16 = c->d.green_mode == th::kGreenBF16Mfma || c->d.green_mode == th::kGreenBF16;
+    if (bf16)
+        hipLaunchKernelGGL(th::green_mfma_kernel<th::kGreenBF16>, dim3(blocks), dim3(256), lds, t->stream, c->d, qcols,
+                           R, out, batch ? 1 : 0);
+    else
+        hipLaunchKernelGGL(th::green_mfma_kernel<th::kGreenF16>, dim3(blocks), dim3(256), lds, t->stream, c->d, qcols,
+                           R, out, batch ? 1 : 0);
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 // Potential from the column charges, then the push's field records.
 int launch_field(th_charging *c) {
     th_tracer *t = c->t;
@@ -836,6 +855,9 @@ int launch_field(th_charging *c) {
         if (blocks < 1) blocks = 1;
         hipLaunchKernelGGL(th::green_gemv_kernel, dim3(blocks), dim3(64 * waves), sizeof(double) * c->d.n_cols,
                            t->stream, c->d);
+    } else if (c->d.green_mode == th::kGreenBF16Mfma || c->d.green_mode == th::kGreenF16Mfma) {
+        int rc = launch_green_mfma(c, c->d.qcol, 1, nullptr, true);
+        if (rc) return rc;
     } else {
         int rows_per_block = 256 / 16;
         int blocks = (int)std::min<int64_t>((c->d.n_rows + rows_per_block - 1) / rows_per_block, (int64_t)t->cus * 8);
@@ -926,7 +948,7 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
     d.cells = t->d_cells; d.cells32 = t->d_cells32; d.counts = t->d_counts;
     d.q_macro = s->q_macro; d.dx = s->dx; d.C_cond = s->C_cond; d.Te = s->Te; d.dt_batch = s->dt_batch;
     d.green_mode = th::kGreenF64;
-    d.n_pad = (s->n_cols + 7) / 8 * 8;
+    d.n_pad = (s->n_cols + 31) / 32 * 32;   // whole K blocks of the MFMA update
     d.lo_scale = 1.0f;
     d.inv_scale = 1.0;
     for (size_t k = 0; k < (size_t)s->n_rows * s->n_cols; ++k)
@@ -942,19 +964,22 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
 
 int th_charging_set_green(th_charging *c, int32_t mode) {
     if (!c) return fail(TH_EINVAL, "invalid argument");
-    if (mode < th::kGreenF64 || mode > th::kGreenF16) return fail(TH_EINVAL, "Green's precision %d is not available", mode);
+    if (mode < th::kGreenF64 || mode > th::kGreenF16Mfma)
+        return fail(TH_EINVAL, "Green's precision %d is not available", mode);
     th_tracer *t = c->t;
     TH_HIP(hipSetDevice(t->device));
     th::ChargeDev &d = c->d;
     double scale = 1.0;
     float lo_scale = 1.0f;
-    if (mode == th::kGreenF16) {
+    // the MFMA modes use the same split matrix as the scalar ones
+    const int half = mode == th::kGreenBF16Mfma ? th::kGreenBF16 : mode == th::kGreenF16Mfma ? th::kGreenF16 : mode;
+    if (half == th::kGreenF16) {
         // G * scale spans at most 2^14 (FP16 overflows at 65504); lo carries 11 more bits
         int e = c->green_absmax > 0.0 ? (int)std::ceil(std::log2(c->green_absmax)) : 0;
         scale = std::ldexp(1.0, 14 - e);
         lo_scale = std::ldexp(1.0f, -11);
     }
-    if (mode != th::kGreenF64 && c->green_built != mode) {
+    if (half != th::kGreenF64 && c->green_built != half) {
         size_t count = (size_t)d.n_rows * d.n_pad;
         int rc;
         if (!c->g_hi || !c->g_lo) {   // both halves or neither: a failed allocation leaves no half behind
@@ -966,14 +991,14 @@ int th_charging_set_green(th_charging *c, int32_t mode) {
             }
         }
         int blocks = (int)((count + 255) / 256);
-        if (mode == th::kGreenBF16)
+        if (half == th::kGreenBF16)
             hipLaunchKernelGGL(th::green_split_kernel<th::kGreenBF16>, dim3(blocks), dim3(256), 0, t->stream, c->green,
                                d.n_rows, d.n_cols, d.n_pad, scale, lo_scale, c->g_hi, c->g_lo);
         else
             hipLaunchKernelGGL(th::green_split_kernel<th::kGreenF16>, dim3(blocks), dim3(256), 0, t->stream, c->green,
                                d.n_rows, d.n_cols, d.n_pad, scale, lo_scale, c->g_hi, c->g_lo);
         TH_HIP(hipGetLastError());
-        c->green_built = mode;
+        c->green_built = half;
     }
     d.green_mode = mode;
     d.g_hi = c->g_hi;
@@ -983,6 +1008,38 @@ int th_charging_set_green(th_charging *c, int32_t mode) {
     int rc = launch_field(c);       // potential and field of the current charge in the new precision
     if (rc) return rc;
     TH_HIP(hipStreamSynchronize(t->stream));
+    return TH_OK;
+}
+
+int th_charging_potentials(th_charging *c, int32_t R, const double *qcols, double *out, double *ms) {
+    if (!c || R < 1 || R > 16 || !qcols || !out) return fail(TH_EINVAL, "invalid argument (1 <= R <= 16)");
+    const int mode = c->d.green_mode;
+    if (mode != th::kGreenBF16Mfma && mode != th::kGreenF16Mfma)
+        return fail(TH_ESTATE, "th_charging_potentials needs a split Green's matrix on matrix cores (mode 3 or 4)");
+    th_tracer *t = c->t;
+    TH_HIP(hipSetDevice(t->device));
+    double *dq = nullptr, *dout = nullptr;
+    int rc = dev_alloc(&dq, (size_t)R * c->d.n_cols);
+    if (rc) return rc;
+    if ((rc = dev_alloc(&dout, (size_t)R * c->d.n_rows))) {
+        (void)hipFree(dq);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dq, qcols, sizeof(double) * R * c->d.n_cols, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipEventRecord(t->ev0, t->stream);
+    if (e == hipSuccess) rc = launch_green_mfma(c, dq, R, dout, false);
+    if (e == hipSuccess && !rc) e = hipEventRecord(t->ev1, t->stream);
+    if (e == hipSuccess && !rc)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * R * c->d.n_rows, hipMemcpyDeviceToHost, t->stream);
+    hipError_t sync = hipStreamSynchronize(t->stream);
+    float el = 0.0f;
+    if (e == hipSuccess && !rc && sync == hipSuccess && ms && hipEventElapsedTime(&el, t->ev0, t->ev1) == hipSuccess)
+        *ms = el;
+    (void)hipFree(dq);
+    (void)hipFree(dout);
+    if (rc) return rc;
+    if (e != hipSuccess || sync != hipSuccess)
+        return fail(TH_EHIP, "potentials: %s", hipGetErrorString(e != hipSuccess ? e : sync));
     return TH_OK;
 }
 

@@ -274,7 +274,7 @@ __global__ void green_adjoint_sum_kernel(const double *__restrict__ partial, int
     out[c] = acc;
 }
 
-enum : int { kGreenF64 = 0, kGreenBF16 = 1, kGreenF16 = 2 };
+enum : int { kGreenF64 = 0, kGreenBF16 = 1, kGreenF16 = 2, kGreenBF16Mfma = 3, kGreenF16Mfma = 4 };
 
 __device__ __forceinline__ float bf16_to_float(uint32_t h) { return __uint_as_float(h << 16); }
 __device__ __forceinline__ uint32_t float_to_bf16(float f) {   // round to nearest even (finite f)
@@ -346,6 +346,104 @@ __global__ void __launch_bounds__(256) green_gemv_split_kernel(ChargeDev d) {
         if (sub == 0) {
             int cell = d.rows[r];
             d.phi[cell] = (double)acc * d.inv_scale + d.external[cell];
+        }
+    }
+}
+
+// The split Green's matrix on matrix cores: Phi = G Q for 1 to 16 charge
+// vectors Q (columns of qcols [R][n_cols], C/m), with G = (hi + lo lo_scale)
+// inv_scale as in green_gemv_split_kernel and each charge split into two
+// halves of the same 16-bit type (BF16, or FP16 after scaling the charges by a
+// power of two into its range): three MFMA products per K block
+// (hi*hi + hi*lo + lo*hi, FP32 accumulation; lo*lo is below FP32's last bit).
+// v_mfma_f32_16x16x32_{bf16,f16}: lane l holds A[l%16][8(l/16)+e],
+// B[8(l/16)+e][l%16], C[4(l/16)+v][l%16].  A wave computes 16 rows x 16
+// right-hand sides; G streams from HBM in 16-byte loads per lane (n_pad is a
+// multiple of 32), the charge halves sit transposed in LDS.  batch: the batch
+// update (R = 1), phi[rows[r]] = G q + external; else out[j][r] = (G Q)[r][j].
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__device__ __forceinline__ void split_half(float v, uint16_t &hi, uint16_t &lo) {
+    if constexpr (MODE == kGreenBF16) {
+        hi = (uint16_t)float_to_bf16(v);
+        lo = (uint16_t)float_to_bf16(v - bf16_to_float(hi));
+    } else {
+        _Float16 h = (_Float16)v;
+        _Float16 l = (_Float16)(v - (float)h);
+        __builtin_memcpy(&hi, &h, 2);
+        __builtin_memcpy(&lo, &l, 2);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ f32x4 mfma16(s16x8 a, s16x8 b, f32x4 c) {
+    if constexpr (MODE == kGreenBF16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+                                                      0, 0, 0);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) green_mfma_kernel(ChargeDev d, const double *__restrict__ qcols, int R,
+                                                         double *__restrict__ out, int batch) {
+    extern __shared__ __align__(16) uint16_t bsh[];   // [2][16][K]: charge halves hi, lo, transposed (row j = right-hand side j)
+    __shared__ float red[4];
+    const int K = d.n_pad, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nq = R * d.n_cols;
+    // FP16: charges scaled by 2^(14 - ceil(log2 max|q|)) (the same in every workgroup)
+    float qs = 1.0f;
+    if constexpr (MODE == kGreenF16) {
+        float mx = 0.0f;
+        for (int i = tid; i < nq; i += blockDim.x) mx = fmaxf(mx, fabsf((float)qcols[i]));
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0) red[wave] = mx;
+        __syncthreads();
+        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        qs = mx > 0.0f ? ldexpf(1.0f, 14 - (int)ceilf(log2f(mx))) : 1.0f;
+    }
+    for (int idx = tid; idx < 16 * K; idx += blockDim.x) {
+        const int j = idx / K, k = idx - j * K;
+        const float v = (j < R && k < d.n_cols) ? (float)qcols[(int64_t)j * d.n_cols + k] * qs : 0.0f;
+        uint16_t hi, lo;
+        split_half<MODE>(v, hi, lo);
+        bsh[idx] = hi;
+        bsh[16 * K + idx] = lo;
+    }
+    __syncthreads();
+    const int n_tiles = (d.n_rows + 15) >> 4;
+    const double scale = d.inv_scale / (double)qs;
+    for (int tile = blockIdx.x * 4 + wave; tile < n_tiles; tile += gridDim.x * 4) {
+        const int r = min(tile * 16 + (lane & 15), d.n_rows - 1);   // rows past the end repeat the last (not stored)
+        const s16x8 *gh = reinterpret_cast<const s16x8 *>(d.g_hi + (int64_t)r * K + 8 * (lane >> 4));
+        const s16x8 *gl = reinterpret_cast<const s16x8 *>(d.g_lo + (int64_t)r * K + 8 * (lane >> 4));
+        const s16x8 *bh = reinterpret_cast<const s16x8 *>(bsh + (lane & 15) * K + 8 * (lane >> 4));
+        const s16x8 *bl = reinterpret_cast<const s16x8 *>(bsh + 16 * K + (lane & 15) * K + 8 * (lane >> 4));
+        f32x4 acc1 = {0.0f, 0.0f, 0.0f, 0.0f}, acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kb = 0; kb < (K >> 5); ++kb) {   // 32 columns per block: 4 vectors of 8 halves
+            const s16x8 ah = gh[4 * kb], al = gl[4 * kb], b_hi = bh[4 * kb], b_lo = bl[4 * kb];
+            acc1 = mfma16<MODE>(ah, b_hi, acc1);
+            acc1 = mfma16<MODE>(ah, b_lo, acc1);
+            acc2 = mfma16<MODE>(al, b_hi, acc2);
+        }
+        const int j = lane & 15;
+        if (j < R) {
+            for (int v = 0; v < 4; ++v) {
+                const int row = tile * 16 + 4 * (lane >> 4) + v;
+                if (row >= d.n_rows) break;
+                const double val = (double)(acc1[v] + d.lo_scale * acc2[v]) * scale;
+                if (batch) {
+                    const int cell = d.rows[row];
+                    d.phi[cell] = val + d.external[cell];
+                } else {
+                    out[(int64_t)j * d.n_rows + row] = val;
+                }
+            }
         }
     }
 }
