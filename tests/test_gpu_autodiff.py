@@ -71,20 +71,28 @@ def test_charge_backward_is_the_transpose_of_the_charge_jvp(batch):
     assert lhs == pytest.approx(rhs, rel=1e-9)
 
 
-def test_sigma_score_gradient_follows_the_expected_increment():
+@pytest.fixture(scope="module")
+def charged_ar8():
+    """A charged AR 8 trench: 300 batches of the charging loop (same charge per
+    batch at 2.5e4 particles per species), and a 2e5-per-species batch on it."""
+    from trench01_amd import charging, configs
+    p, _ = configs.product_params("AR8", n_per_batch=200_000)
+    pre = charging.run(replace(p, n_per_batch=25_000, n_batches=300, until_steady=False), log=None,
+                       injection="device")
+    b = tangent.PathwiseBatch(p, seed=7)
+    rho = np.asarray(pre["rho_raw"]).ravel()[b.surface]
+    yield p, b, rho
+    b.close()
+
+
+def test_sigma_score_gradient_follows_the_expected_increment(charged_ar8):
     """The likelihood-ratio gradient with respect to the ion angular sigma
     estimates the derivative of the expected increment: on a charged AR 8
     trench it points along the central difference of the increment at a
     20 % step with common random numbers (reproducible between streams), and
     SigmaIncrement's backward is that gradient."""
     _case("autodiff sigma score gradient")
-    from trench01_amd import charging, configs
-    p, _ = configs.product_params("AR8", n_per_batch=200_000)
-    # a charged trench: 300 batches of the charging loop (same charge per batch at fewer particles)
-    pre = charging.run(replace(p, n_per_batch=25_000, n_batches=300, until_steady=False), log=None,
-                       injection="device")
-    b = tangent.PathwiseBatch(p, seed=7)
-    rho = np.asarray(pre["rho_raw"]).ravel()[b.surface]
+    p, b, rho = charged_ar8
     sig = p.ion_angle_deg
     inc, grad = b.sigma_score_gradient(rho, sig, 0.0)
     np.testing.assert_array_equal(inc, b.increment(rho, sig))
@@ -98,4 +106,34 @@ def test_sigma_score_gradient_follows_the_expected_increment():
     w = torch.tensor(np.random.default_rng(2).standard_normal(inc.size))
     (out @ w).backward()
     assert float(s.grad) == pytest.approx(float(grad @ w.numpy()), rel=1e-12)
-    b.close()
+
+
+def test_energy_score_gradient_follows_the_expected_increment(charged_ar8):
+    """The likelihood-ratio gradient with respect to the mean ion energy along
+    the exponentially tilted family (arcsine 40-260 eV) points along the
+    central difference of the increment over that family (tilts moving the
+    mean by +-10 %, common random numbers), and EnergyIncrement's backward is
+    that gradient."""
+    _case("autodiff energy score gradient")
+    p, b, rho = charged_ar8
+    sampler = tangent.ion_sampler(p)
+    mean, var = sampler.energy_moments()
+    inc, grad = b.energy_score_gradient(rho)
+    np.testing.assert_array_equal(inc, b.increment(rho))
+    t = 0.1 * mean / var
+    a = np.linspace(0.0, 0.5 * np.pi, 200_001)
+    e = p.ion_energy_lo_eV + (p.ion_energy_hi_eV - p.ion_energy_lo_eV) * np.sin(a) ** 2
+
+    def tilted_mean(tt):
+        w = np.exp(tt * (e - mean))
+        return float(np.trapz(w * e, a) / np.trapz(w, a))
+    fd = (b.increment_tilted(rho, t) - b.increment_tilted(rho, -t)) / (tilted_mean(t) - tilted_mean(-t))
+    cos = float(grad @ fd / (np.linalg.norm(grad) * np.linalg.norm(fd)))
+    ratio = float(np.linalg.norm(grad) / np.linalg.norm(fd))
+    print(f"  cos(energy score gradient, difference) {cos:.3f}, norm ratio {ratio:.3f}", flush=True)
+    assert cos > 0.8 and 0.6 < ratio < 1.6
+    m = torch.tensor(mean, dtype=torch.float64, requires_grad=True)
+    out = autodiff.EnergyIncrement.apply(m, b, rho)
+    w = torch.tensor(np.random.default_rng(3).standard_normal(inc.size))
+    (out @ w).backward()
+    assert float(m.grad) == pytest.approx(float(grad @ w.numpy()), rel=1e-12)

@@ -247,3 +247,46 @@ def test_linear_field_transpose_is_the_adjoint_of_the_field_stencil(top_bc):
     col = tangent.linear_field_transpose(e, np.zeros_like(e), geo.vac, p)
     phi2 = rng.standard_normal(geo.solid.shape)
     assert float((col * phi2).sum()) == pytest.approx(tangent.linear_field(phi2, geo.vac, p)[0][ix, iy], rel=1e-12)
+
+
+def test_tilted_energy_family_and_its_score():
+    """The exponentially tilted arcsine family: at tilt 0 the inverse
+    distribution function reproduces the arcsine draws; the tilted mean moves
+    by tilt * Var(E) to first order (central difference of the exact tilted
+    mean by quadrature); the score (E - mean) / Var(E) has mean 0 and
+    E[score * E] = 1; and for f(E) = E^2 the likelihood-ratio derivative along
+    the family agrees with common-random-number differences of the draws."""
+    p, _ = configs.product_params("AR8")
+    base = tangent.ion_sampler(p)
+    mean, var = base.energy_moments()
+    assert mean == pytest.approx(150.0) and var == pytest.approx(220.0 ** 2 / 8)
+    u = np.random.default_rng(0).uniform(size=200_000)
+    s0 = tangent.ion_sampler(p, tilt=1e-12)
+    e_tab = s0._tilted_energy(u)
+    e_exact = p.ion_energy_lo_eV + (p.ion_energy_hi_eV - p.ion_energy_lo_eV) * np.sin(0.5 * np.pi * u) ** 2
+    assert np.abs(e_tab - e_exact).max() < 1e-3
+    # exact tilted mean by quadrature in the arcsine angle
+    a = np.linspace(0.0, 0.5 * np.pi, 200_001)
+    e = p.ion_energy_lo_eV + 220.0 * np.sin(a) ** 2
+
+    def tilted_mean(t):
+        w = np.exp(t * (e - mean))
+        return float(np.trapz(w * e, a) / np.trapz(w, a))
+    h = 1e-5
+    assert (tilted_mean(h) - tilted_mean(-h)) / (2 * h) == pytest.approx(var, rel=1e-6)
+    # the score of a large sample
+    s = tangent.ion_sampler(p)
+    s(np.random.default_rng(4), 400_000)
+    score = s.energy_score()
+    assert abs(score.mean()) < 5e-3 / np.sqrt(var) * 10
+    assert float(np.mean(score * s.energy)) == pytest.approx(1.0, abs=0.01)
+    # d E[E^2] / d mean along the family: Cov(E^2, E) / Var(E), by the score and by CRN differences
+    lr = float(np.mean(s.energy ** 2 * score))
+    t = 0.05 * mean / var
+    plus = tangent.ion_sampler(p, tilt=t)
+    minus = tangent.ion_sampler(p, tilt=-t)
+    plus(np.random.default_rng(4), 400_000)
+    minus(np.random.default_rng(4), 400_000)
+    fd = (np.mean(plus.energy ** 2) - np.mean(minus.energy ** 2)) / (tilted_mean(t) - tilted_mean(-t))
+    assert lr == pytest.approx(fd, rel=0.02)
+    assert lr == pytest.approx(2 * mean, rel=0.02)   # Cov(E^2, E) / Var(E) = 2 mean for a symmetric law

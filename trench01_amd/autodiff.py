@@ -12,6 +12,10 @@ carries the result through autograd:
   surface charge; forward-mode JVPs by one tangent push along the tangent's
   charge direction; backward (vector-Jacobian product) by one adjoint push
   (batch.vjp: the transpose of the tangent, for all surface cells at once).
+* ``EnergyIncrement.apply(mean_energy, batch, rho_s)``: the same for the mean
+  ion energy, along the exponentially tilted family of the energy
+  distribution (batch.energy_score_gradient, a likelihood ratio with a
+  bounded score).
 * ``SigmaIncrement.apply(sigma, batch, rho_s)``: the increment as a function
   of the ion angular sigma [deg], differentiated by the likelihood ratio
   (batch.sigma_score_gradient): an unbiased estimate of the derivative of
@@ -62,6 +66,33 @@ class ChargeIncrement(torch.autograd.Function):
         g = grad.detach().cpu().numpy().astype(float)
         out = ctx.batch.vjp(ctx.rho, g)[1]
         return torch.as_tensor(out, dtype=grad.dtype, device=grad.device), None
+
+
+class EnergyIncrement(torch.autograd.Function):
+    """The increment as a function of the mean ion energy [eV] (the tilted
+    family, batch.energy_score_gradient); the value is the batch's own at the
+    untilted distribution, and the input only selects where the gradient is
+    reported."""
+
+    @staticmethod
+    def forward(ctx, mean_energy, batch, rho_s=None):
+        rho = None if rho_s is None else np.asarray(rho_s, dtype=float)
+        inc, grad = batch.energy_score_gradient(rho)
+        g = torch.as_tensor(grad, dtype=mean_energy.dtype, device=mean_energy.device)
+        ctx.save_for_backward(g)
+        ctx.save_for_forward(g)
+        ctx.shape = mean_energy.shape
+        return torch.as_tensor(inc, dtype=mean_energy.dtype, device=mean_energy.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        g, = ctx.saved_tensors
+        return (g @ grad).reshape(ctx.shape), None, None
+
+    @staticmethod
+    def jvp(ctx, dmean, *_):
+        g, = ctx.saved_tensors
+        return g * dmean.reshape(-1)[0]
 
 
 class SigmaIncrement(torch.autograd.Function):

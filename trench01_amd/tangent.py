@@ -46,19 +46,46 @@ class IonSampler:
     last draw with respect to sigma [per degree] and to the shift [per eV] in
     ``tangents`` = {"sigma": (dvx, dvy), "energy": (dvx, dvy)}."""
 
-    def __init__(self, ion_mass_amu, kind, sigma_deg, energy_eV=None, e_lo=None, e_hi=None, energy_shift=0.0):
+    def __init__(self, ion_mass_amu, kind, sigma_deg, energy_eV=None, e_lo=None, e_hi=None, energy_shift=0.0,
+                 tilt=0.0):
         if kind not in ("mono", "arcsine"):
             raise ValueError(f"unknown ion energy distribution {kind!r}")
+        if tilt and kind != "arcsine":
+            raise ValueError("only the arcsine energy distribution can be tilted")
         self.m_i = ion_mass_amu * AMU
         self.kind, self.sigma_deg = kind, float(sigma_deg)
         self.energy_eV, self.e_lo, self.e_hi = energy_eV, e_lo, e_hi
         self.energy_shift = float(energy_shift)
+        self.tilt = float(tilt)
         self.tangents = None
+
+    def energy_moments(self):
+        """Mean and variance of the (untilted, unshifted) energy distribution
+        [eV, eV^2]: arcsine on [lo, hi]: (lo + hi) / 2, (hi - lo)^2 / 8."""
+        if self.kind == "mono":
+            return float(self.energy_eV), 0.0
+        return 0.5 * (self.e_lo + self.e_hi), (self.e_hi - self.e_lo) ** 2 / 8.0
+
+    def _tilted_energy(self, u):
+        """Energies of the exponentially tilted arcsine distribution
+        p(E) exp(tilt (E - mean)) / Z from the uniforms u, by the inverse of
+        its distribution function.  The arcsine law is uniform in the angle
+        a = pi u / 2 of E = lo + (hi - lo) sin^2(a); the tilted one has density
+        proportional to exp(tilt (E(a) - mean)) in a, tabulated on 2^16
+        intervals and inverted by linear interpolation."""
+        mean, _ = self.energy_moments()
+        a = np.linspace(0.0, 0.5 * np.pi, (1 << 16) + 1)
+        e = self.e_lo + (self.e_hi - self.e_lo) * np.sin(a) ** 2
+        w = np.exp(self.tilt * (e - mean))
+        cdf = np.concatenate([[0.0], np.cumsum(0.5 * (w[1:] + w[:-1]) * np.diff(a))])
+        return np.interp(u, cdf / cdf[-1], e)
 
     def __call__(self, rng, n):
         sig = np.radians(self.sigma_deg)
         if self.kind == "mono":
             E = np.full(n, float(self.energy_eV))
+        elif self.tilt:
+            E = self._tilted_energy(rng.uniform(size=n))
         else:
             E = self.e_lo + (self.e_hi - self.e_lo) * np.sin(0.5 * np.pi * rng.uniform(size=n)) ** 2
         if self.energy_shift:
@@ -70,7 +97,19 @@ class IonSampler:
         dv = E_CHARGE / (self.m_i * v)                                      # dv/dE
         self.tangents = {"sigma": (v * c * dtheta, v * s * dtheta), "energy": (s * dv, -c * dv)}
         self.theta = theta
+        self.energy = E
         return v * s, -v * c
+
+    def energy_score(self):
+        """Score of the last draw's energies with respect to the mean energy
+        along the exponentially tilted family p(E) exp(t (E - mean)) / Z at
+        t = 0: d log p_t / dt = E - mean, and d mean / dt = Var(E), so the
+        score per eV of mean energy is (E - mean) / Var(E) (mean and variance
+        of the untilted distribution, the shift included in the mean)."""
+        mean, var = self.energy_moments()
+        if var <= 0.0:
+            raise ValueError("a monoenergetic distribution has no tilted family")
+        return (self.energy - (mean + self.energy_shift)) / var
 
     def sigma_score(self):
         """Score of the last draw's polar angles with respect to sigma [per
@@ -80,13 +119,13 @@ class IonSampler:
         return (self.theta ** 2 / s ** 2 - 1.0) / s * np.radians(1.0)
 
 
-def ion_sampler(p, energy_shift=0.0, sigma_deg=None):
+def ion_sampler(p, energy_shift=0.0, sigma_deg=None, tilt=0.0):
     """IonSampler of Params ``p``'s ion distribution fields."""
     sig = p.ion_angle_deg if sigma_deg is None else sigma_deg
     if p.ion_distribution == "mono":
-        return IonSampler(p.ion_mass_amu, "mono", sig, energy_eV=p.ion_energy_eV, energy_shift=energy_shift)
+        return IonSampler(p.ion_mass_amu, "mono", sig, energy_eV=p.ion_energy_eV, energy_shift=energy_shift, tilt=tilt)
     return IonSampler(p.ion_mass_amu, "arcsine", sig, e_lo=p.ion_energy_lo_eV, e_hi=p.ion_energy_hi_eV,
-                      energy_shift=energy_shift)
+                      energy_shift=energy_shift, tilt=tilt)
 
 
 def draw_batch(p, ny, seed, sampler, n=None):
@@ -312,6 +351,40 @@ class PathwiseBatch:
         _, d_en, _ = self._trace(rho_s, sigma_deg, energy_shift, "energy")
         return inc, np.column_stack((d_sig, d_en))
 
+    def increment_tilted(self, rho_s, tilt):
+        """The increment with the ion energies drawn from the exponentially
+        tilted distribution (same random stream)."""
+        return self._trace_sampler(rho_s, ion_sampler(self.p, tilt=tilt))[0]
+
+    def _trace_sampler(self, rho_s, sampler, weights_from=None):
+        parts, _ = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        barrier = (phi, self.fields.wface) if self.p.wall_model == "barrier" else None
+        r = BatchDerivative(self.tracer, self.p, ex, ey, barrier)(parts)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        inc[self.cond_s] = 0.0
+        grad = None
+        if weights_from is not None:
+            score = np.zeros(parts[0].size)
+            score[:self.n] = weights_from(sampler)   # ions first; electrons do not depend on it
+            dep, _ = face_deposit(self.tracer.vac, self.surface, parts[5], r["status"], r["hit"], r["face"],
+                                  r["endpoints"], r["d_endpoints"], self.p.dx, weights=score)
+            grad = (dep[0] - dep[1]) * self.scale
+            grad[self.cond_s] = 0.0
+        return inc, grad
+
+    def energy_score_gradient(self, rho_s=None, sigma_deg=None, energy_shift=0.0):
+        """Likelihood-ratio estimate of the derivative of the expected
+        increment with respect to the mean ion energy [per eV], along the
+        exponentially tilted family of the energy distribution (IonSampler.
+        energy_score): each ion's face deposit weighted by (E - mean) /
+        Var(E), from one trace.  Unbiased for that family, with a bounded
+        score (a shift of the arcsine law has a score of infinite variance at
+        its edges).  Returns (increment [m], d E[increment] / d mean [m])."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else rho_s
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        return self._trace_sampler(rho_s, sampler, weights_from=lambda s: s.energy_score())
+
     def sigma_score_gradient(self, rho_s=None, sigma_deg=None, energy_shift=0.0):
         """Likelihood-ratio (score-function) estimate of the derivative of the
         expected increment with respect to the ion angular sigma [per degree],
@@ -392,13 +465,22 @@ def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None
     sig = p.ion_angle_deg
     e_mean = p.ion_energy_eV if p.ion_distribution == "mono" else 0.5 * (p.ion_energy_lo_eV + p.ion_energy_hi_eV)
     out = {}
-    for name in ("sigma", "sigma_score", "energy", "charge"):
+    names = ["sigma", "sigma_score", "energy", "charge"]
+    if p.ion_distribution != "mono":
+        names.insert(3, "energy_score")
+    for name in names:
         pw, fds = [], []
         for b in batches:
             if name == "sigma_score":
                 pw.append(b.sigma_score_gradient(rho_s, sig, 0.0)[1])
                 f = lambda e: b.increment(rho_s, sig * (1.0 + e))
                 scale = sig
+            elif name == "energy_score":
+                # along the tilted family: mean energy changes by tilt * Var(E) to first order
+                mean, var = ion_sampler(p).energy_moments()
+                pw.append(b.energy_score_gradient(rho_s, sig, 0.0)[1])
+                f = lambda e: b.increment_tilted(rho_s, e * mean / var)
+                scale = mean
             elif name == "charge":
                 pw.append(b.charge_jvp(rho_s, rho_s)[1])
                 f = lambda e: b.increment(rho_s * (1.0 + e))
