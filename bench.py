@@ -257,9 +257,28 @@ def differentiation(p, dev, rho, push_ms):
         _lib.check(_lib.load().th_last_push_ms(b.tracer._h, ctypes.byref(ms)))
         times.append(ms.value)
     b.close()
+    # reverse mode: one primal trace + one adjoint push give <g, increment>'s
+    # gradient for every surface cell and both ion parameters
+    vb = tangent.PathwiseBatch(p, seed=7)
+    g = np.random.default_rng(1).standard_normal(vb.surface.size)
+    vb.vjp(rho, g)                                   # warm-up
+    t0 = time.perf_counter()
+    vb.vjp(rho, g)
+    vjp_s = time.perf_counter() - t0
+    adjoint_ms = vb.tracer.last_adjoint_ms
+    t0 = time.perf_counter()
+    vb.charge_jvp(rho, rho)
+    jvp_s = time.perf_counter() - t0
+    m = int(vb.surface.size)
+    vb.close()
     agreement = tangent.derivative_agreement(replace(p, n_per_batch=250_000), rho)
     return {"tangent_push_ms": min(times), "plain_push_ms": push_ms, "particles": n,
-            "kernel": "push64_tangent_kernel", "agreement": agreement}
+            "kernel": "push64_tangent_kernel", "agreement": agreement,
+            "reverse_mode": {"vjp_s": vjp_s, "adjoint_kernel_ms": adjoint_ms, "jvp_s": jvp_s, "surface_cells": m,
+                             "m_jvp_loop_s_estimate": m * jvp_s,
+                             "note": "PathwiseBatch.vjp (primal trace, face-deposit VJP, th_trace_adjoint, field "
+                                     "stencil and G^T on the host) against one forward-mode JVP; the backward that "
+                                     "the forward mode alone gives takes one JVP per surface cell"}}
 
 
 def batching(count, batches, log, schedule=None):
@@ -549,9 +568,18 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
             e["chosen_minus_long_run_V"] = e["chosen"]["bottom_center_V"] - long["value_V"]
             e["plain_minus_long_run_V"] = e["plain"]["bottom_center_V"] - long["value_V"]
         out["max_abs_diff_vs_long_run_V"] = max(abs(e["chosen_minus_long_run_V"]) for e in per_seed)
-    # the precision pattern in the steady solve: FP32 push + BF16 split Green's on MFMA unless asked otherwise
+    # Newton-Krylov on the batch map (difference JVPs over merged replicas) on the first seed
+    ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
+    nk = steady.newton_krylov(ps)
+    out["newton_krylov"] = {"seed": seeds[0], "method": nk["method"], "converged": bool(nk["converged"]),
+                            "batches": nk["batches"], "wall_s": nk["wall_s"], "phases": nk["phases"],
+                            "bottom_center_V": nk["steady"]["bottom center"],
+                            "speedup_vs_plain": per_seed[0]["plain"]["wall_s"] / nk["wall_s"]}
+    log(f"time-to-steady, newton-krylov: {nk['wall_s']:.1f}s, bottom centre {nk['steady']['bottom center']:.3f} V")
+    # the precision pattern in the steady solve: FP32 push + BF16 split Green's unless asked otherwise (the
+    # one-column update is faster as the scalar split GEMV than on MFMA, 7.0 vs 10.8 us)
     if not (push_precision or green_precision):
-        push_precision, green_precision = 1, 3
+        push_precision, green_precision = 1, 1
     ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
     rv = steady.anderson(ps, push_precision=push_precision, green_precision=green_precision)
     plain0, chosen0 = per_seed[0]["plain"], per_seed[0]["chosen"]

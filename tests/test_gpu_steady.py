@@ -71,3 +71,27 @@ def test_replicas_in_one_push_equal_separate_batches():
     assert tot.tally.steps == steps
     assert not np.array_equal(rhos[0], rhos[1])
     dev.close()
+
+
+def test_newton_krylov_reaches_the_time_marched_state():
+    """Newton-Krylov on the batch map (GMRES, difference JVPs over merged
+    replicas, from the coarse Anderson start) followed by plain batches to
+    the reference criterion: it converges, within 1 V of the time-marched
+    bottom-centre potential.  Its speed and accuracy are measured in the
+    bench, not asserted: the difference JVPs carry too much sampling noise
+    for its Newton steps to descend, so its plain phase starts below the
+    fixed point and, like time-marching from zero charge, passes the
+    criterion while still approaching it."""
+    print("[gpu case] steady AR8 (1.25e5 per species) newton-krylov", flush=True)
+    p, _ = configs.product_params("AR8", n_per_batch=125_000)
+    p = replace(p, until_steady=True, max_batches=4000)
+    plain = charging.run(p, log=None, injection="device")
+    nb = plain["timing"]["batches"]
+    more = charging.run(replace(p, until_steady=False, n_batches=300), log=None, injection="device", resume=plain)
+    long_run = float(np.mean(more["hist"]["bottom center"][nb:]))
+    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1)
+    v = nk["steady"]["bottom center"]
+    print(f"plain {nb} batches, 300 on {long_run:.3f} V; newton-krylov {nk['batches']} batches {nk['phases']}, "
+          f"{v:.3f} V", flush=True)
+    assert nk["converged"]
+    assert abs(v - long_run) < 1.0

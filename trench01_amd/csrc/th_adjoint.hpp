@@ -20,8 +20,9 @@
 //           s = (v + v') dt / 2, x' = x + s;
 //   reflect x' = x0, one velocity component flipped;
 //   crossing the recorded point x0 + f s, f = (face - x0) / s.
-// The field cotangents are summed with FP64 atomics (their order, and so the
-// last bits of the sums, may vary between runs).
+// The field cotangents are summed in registers while a lane's consecutive
+// steps share their four interpolation cells, and then with FP64 atomics
+// (their order, and so the last bits of the sums, may vary between runs).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -83,13 +84,45 @@ struct Bar4 {
     double x, y, vx, vy;
 };
 
+// Field cotangent of the four interpolation cells a lane's consecutive steps
+// share, summed in registers and added to the grid's with FP64 atomics only
+// when the cells change (about every third step) and at the path's end.
+struct FieldBar {
+    int k00 = -1, k10 = -1, k01 = -1, k11 = -1;
+    double a00x = 0.0, a00y = 0.0, a10x = 0.0, a10y = 0.0, a01x = 0.0, a01y = 0.0, a11x = 0.0, a11y = 0.0;
+    __device__ __forceinline__ void flush(double *fbar) {
+        if (k00 < 0 || !fbar) return;
+        unsafeAtomicAdd(&fbar[2 * k00], a00x);
+        unsafeAtomicAdd(&fbar[2 * k00 + 1], a00y);
+        unsafeAtomicAdd(&fbar[2 * k10], a10x);
+        unsafeAtomicAdd(&fbar[2 * k10 + 1], a10y);
+        unsafeAtomicAdd(&fbar[2 * k01], a01x);
+        unsafeAtomicAdd(&fbar[2 * k01 + 1], a01y);
+        unsafeAtomicAdd(&fbar[2 * k11], a11x);
+        unsafeAtomicAdd(&fbar[2 * k11 + 1], a11y);
+        k00 = -1;
+    }
+    __device__ __forceinline__ void add(int c00, int c10, int c01, int c11, double w00, double w10, double w01,
+                                        double w11, double bx, double by, double *fbar) {
+        if (c00 != k00 || c10 != k10 || c01 != k01 || c11 != k11) {
+            flush(fbar);
+            k00 = c00; k10 = c10; k01 = c01; k11 = c11;
+            a00x = a00y = a10x = a10y = a01x = a01y = a11x = a11y = 0.0;
+        }
+        a00x += w00 * bx; a00y += w00 * by;
+        a10x += w10 * bx; a10y += w10 * by;
+        a01x += w01 * bx; a01y += w01 * by;
+        a11x += w11 * bx; a11y += w11 * by;
+    }
+};
+
 // Transpose of one step.  `out` is the cotangent of the state after the step;
 // on the recorded (last) step, `pend` is the cotangent of the recorded point
 // (else zero).  Returns the cotangent of the state before the step and adds
-// the field cotangent of the step's four corners to fbar ([N] {Ex, Ey}, may
-// be null).
+// the field cotangent of the step's four corners to fb (then fbar, [N] {Ex,
+// Ey}, which may be null).
 __device__ __forceinline__ Bar4 step_transpose(const StepCapture &c, Bar4 out, double pend_x, double pend_y,
-                                               bool last, int ny, double inv_dx, double *fbar) {
+                                               bool last, int ny, double inv_dx, double *fbar, FieldBar &fb) {
     double bsx = 0.0, bsy = 0.0, bx0 = 0.0, by0 = 0.0;
     double bvxn = out.vx, bvyn = out.vy;
     double bxo = out.x, byo = out.y;
@@ -154,17 +187,9 @@ __device__ __forceinline__ Bar4 step_transpose(const StepCapture &c, Bar4 out, d
     // field: E = n / sw with n = sum w e, sw = sum w v (clamped: constant)
     const double bnx = bex / c.sw, bny = bey / c.sw;
     const double bsw = c.clamped ? 0.0 : -(c.ex * bex + c.ey * bey) / c.sw;
-    if (fbar) {
-        const int k00 = c.i0 * ny + c.j0, k10 = c.i1 * ny + c.j0, k01 = c.i0 * ny + c.j1, k11 = c.i1 * ny + c.j1;
-        unsafeAtomicAdd(&fbar[2 * k00], c.w00 * bnx);
-        unsafeAtomicAdd(&fbar[2 * k00 + 1], c.w00 * bny);
-        unsafeAtomicAdd(&fbar[2 * k10], c.w10 * bnx);
-        unsafeAtomicAdd(&fbar[2 * k10 + 1], c.w10 * bny);
-        unsafeAtomicAdd(&fbar[2 * k01], c.w01 * bnx);
-        unsafeAtomicAdd(&fbar[2 * k01 + 1], c.w01 * bny);
-        unsafeAtomicAdd(&fbar[2 * k11], c.w11 * bnx);
-        unsafeAtomicAdd(&fbar[2 * k11 + 1], c.w11 * bny);
-    }
+    if (fbar)
+        fb.add(c.i0 * ny + c.j0, c.i1 * ny + c.j0, c.i0 * ny + c.j1, c.i1 * ny + c.j1, c.w00, c.w10, c.w01, c.w11,
+               bnx, bny, fbar);
     const double b00 = c.e00.x * bnx + c.e00.y * bny + c.v00 * bsw;
     const double b10 = c.e10.x * bnx + c.e10.y * bny + c.v10 * bsw;
     const double b01 = c.e01.x * bnx + c.e01.y * bny + c.v01 * bsw;
@@ -228,6 +253,7 @@ __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a
                a.bar_vel ? a.bar_vel[2 * (int64_t)pid + 1] : 0.0};
         const double pe_x = a.bar_end ? a.bar_end[2 * (int64_t)pid] : 0.0;
         const double pe_y = a.bar_end ? a.bar_end[2 * (int64_t)pid + 1] : 0.0;
+        FieldBar fb;
         for (int j = n_steps - 1; j >= 0; --j) {
             const double4 t0 = tape[j];
             State64 r;
@@ -238,8 +264,9 @@ __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a
             int cj = -1;
             (void)step64t<BARRIER, TAPERED, true, false>(g, r, cj, xp, yp, rc, cap);
             const bool last = j == n_steps - 1;
-            b = step_transpose(cap, b, last ? pe_x : 0.0, last ? pe_y : 0.0, last, g.ny, g.inv_dx, a.bar_field);
+            b = step_transpose(cap, b, last ? pe_x : 0.0, last ? pe_y : 0.0, last, g.ny, g.inv_dx, a.bar_field, fb);
         }
+        fb.flush(a.bar_field);
         if (a.bar_state) {
             double *o4 = a.bar_state + 4 * (int64_t)pid;
             o4[0] = b.x; o4[1] = b.y; o4[2] = b.vx; o4[3] = b.vy;

@@ -254,6 +254,36 @@ class _Session:
         self.dev.close()
 
 
+def _secant_stages(ss, stages, memory, max_evals, patience, log):
+    """The Anderson stages of ``anderson`` on session ``ss`` from zero charge:
+    returns (charge, surface potentials, evaluations per stage, next free
+    batch index)."""
+    pairs = []
+    rho = np.zeros(ss.dev.m)
+    x = ss.to_u(rho)
+    evals_per_stage = []
+    first = 0
+    for stage in stages:
+        K, R, tol = stage[:3]
+        ss.set_fraction(stage[3] if len(stage) > 3 else 1.0)
+        box = {"like": rho}
+
+        def g(x, first=first, K=K, R=R):
+            rho_in = ss.to_rho(x, box["like"])
+            rho_g, gx = ss.evaluate(rho_in, first, K, R)
+            box["like"] = rho_g
+            return rho_g, gx
+
+        rho_g, x, e = anderson_iterate(
+            g, x, pairs, tol, memory, max_evals, patience,
+            log=lambda k, res, K=K, R=R: log(f"  [anderson K={K} R={R}] eval {k}: max |g(x) - x| = {res:.4g} V"))
+        rho = ss.to_rho(x, rho_g)
+        evals_per_stage.append(e)
+        first += K * R
+    ss.set_fraction(1.0)
+    return rho, x, evals_per_stage, first
+
+
 def anderson(p, stages=None, memory=10, max_evals=40, patience=10, log=None, device=0, setup=None,
              push_precision=0, green_precision=0, merge=True, secant_push_precision=None):
     """Steady state by multilevel Anderson acceleration of the batch map with
@@ -274,30 +304,9 @@ def anderson(p, stages=None, memory=10, max_evals=40, patience=10, log=None, dev
     log = log or (lambda s: None)
     stages = default_stages(p) if stages is None else stages
     ss = _Session(p, device, setup, push_precision, green_precision, max(st[1] for st in stages), merge)
-    pairs = []
-    rho = np.zeros(ss.dev.m)
-    x = ss.to_u(rho)
-    evals_per_stage = []
-    first = 0
     if secant_push_precision is not None:
         ss.dev.set_schedule(precision=secant_push_precision)
-    for stage in stages:
-        K, R, tol = stage[:3]
-        ss.set_fraction(stage[3] if len(stage) > 3 else 1.0)
-        box = {"like": rho}
-
-        def g(x, first=first, K=K, R=R):
-            rho_in = ss.to_rho(x, box["like"])
-            rho_g, gx = ss.evaluate(rho_in, first, K, R)
-            box["like"] = rho_g
-            return rho_g, gx
-
-        rho_g, x, e = anderson_iterate(
-            g, x, pairs, tol, memory, max_evals, patience,
-            log=lambda k, res, K=K, R=R: log(f"  [anderson K={K} R={R}] eval {k}: max |g(x) - x| = {res:.4g} V"))
-        rho = ss.to_rho(x, rho_g)
-        evals_per_stage.append(e)
-        first += K * R
+    rho, x, evals_per_stage, _ = _secant_stages(ss, stages, memory, max_evals, patience, log)
     secant_batches = ss.traced
     if secant_push_precision is not None:
         ss.dev.set_schedule(precision=push_precision)
@@ -391,25 +400,32 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
     return state, gx, evals
 
 
-def newton_krylov(p, K=1, R=4, h_volts=0.5, krylov=30, max_newton=3, tol=0.003, log=None, device=0, setup=None):
+def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
+                  setup=None):
     """Steady state by Jacobian-free Newton-Krylov on the batch map (GMRES,
     ``krylov`` steps per Newton step, JVPs by forward differences of step
     ``h_volts`` on the surface potentials, R replicas of K batches per map
-    evaluation with common random numbers), then plain batches to the
-    reference criterion.  Same return dict as ``anderson``."""
+    evaluation with common random numbers, traced in one push for K = 1),
+    started from the Anderson stages ``start`` (default: the coarse first
+    default stage), then plain batches to the reference criterion.  Same
+    return dict as ``anderson``."""
     log = log or (lambda s: None)
-    ss = _Session(p, device, setup, 0, 0, R)
-    rho = np.zeros(ss.dev.m)
+    start = default_stages(p)[:1] if start is None else start
+    noise = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
+    tol = 0.002 * noise if tol is None else tol
+    ss = _Session(p, device, setup, 0, 0, max([R] + [st[1] for st in start]))
+    rho, x, evals_start, first = _secant_stages(ss, start, 10, 40, 10, log)
+    start_batches = ss.traced
     box = {"like": rho}
 
     def g(x):
         rho_in = ss.to_rho(x, box["like"])
-        rho_g, gx = ss.evaluate(rho_in, 0, K, R)
+        rho_g, gx = ss.evaluate(rho_in, first, K, R)
         box["like"] = rho_g
         return rho_g, gx
 
     rho_g, x, evals = newton_krylov_iterate(
-        g, ss.to_u(rho), tol, h_volts, krylov, max_newton,
+        g, x, tol, h_volts, krylov, max_newton,
         log=lambda k, res, e: log(f"  [newton-krylov] step {k}: max |g(x) - x| = {res:.4g} V after {e} evaluations"))
     rho = ss.to_rho(x, rho_g)
     solver_batches = ss.traced
@@ -417,9 +433,12 @@ def newton_krylov(p, K=1, R=4, h_volts=0.5, krylov=30, max_newton=3, tol=0.003, 
     wall = time.perf_counter() - ss.t_start
     ss.close()
     return dict(method="Jacobian-free Newton-Krylov (GMRES) on the batch map (common random numbers, forward-"
-                       "difference JVPs), then plain batches to the reference criterion",
+                       "difference JVPs over merged replicas) from a coarse Anderson start, then plain batches to the "
+                       "reference criterion",
                 converged=done, batches=ss.traced,
-                phases=[dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov,
-                             evaluations=evals, batches=solver_batches),
+                phases=[dict(kind="secant", stages=[list(st) for st in start], evaluations_per_stage=evals_start,
+                             batches=start_batches),
+                        dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov,
+                             evaluations=evals, batches=solver_batches - start_batches),
                         dict(kind="plain", batches=b2, converged=done)],
                 steady=steady, hist=hist, rho_s=rho_s, wall_s=wall, setup_s=ss.setup_s)
