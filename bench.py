@@ -47,7 +47,8 @@ CPU_SAMPLE_S = 10.0         # minimum CPU work behind the cpu_baseline figure
 # (FETCH_SIZE / WRITE_SIZE per dispatch, summed over the passes of one batch,
 # scaled by the copy-kernel calibration); the command that made it is stored
 # with it.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "push64_ar8_1e6_traffic.json")
+TRAFFIC_FILES = {0: os.path.join(ROOT, "profiles", "push64_ar8_1e6_traffic.json"),
+                 1: os.path.join(ROOT, "profiles", "push32_ar8_1e6_traffic.json")}
 ENSEMBLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ensemble_push_256_traffic.json")
 
 
@@ -186,8 +187,9 @@ def main():
         "config": {"workload": f"AR8 trench 120x320 (dx 5 nm), trench 20x160 with 2/5 deg tapers, conductive "
                                f"floating mask 40, barrier wall, sigma_s 1e-15 S; ions arcsine IED 40-260 eV with "
                                f"4 deg polar sigma, electrons isotropic Te 4 eV; 1e6 particles per batch "
-                               f"(5e5 per species), seed {p.seed}; full batch on device (inject, FP64 push, "
-                               f"deposit, leak, conductor, Green's potential, field); multi-pass push schedule",
+                               f"(5e5 per species), seed {p.seed}; full batch on device (inject, "
+                               f"{'FP64' if a.precision == 0 else 'FP32'} push, deposit, leak, conductor, "
+                               f"{GREEN_NAMES[a.green]} Green's potential, field); multi-pass push schedule",
                    "particles_per_batch": n, "grid": [setup.geo.nx, setup.geo.ny], "surface_cells": m,
                    "green_rows_cols": list(setup.green.shape), "precharge_batches": a.precharge,
                    "max_steps": p.max_steps, "mean_steps_per_particle": steps_total / (a.steps * n),
@@ -415,13 +417,14 @@ def pass_caps(pass_steps, max_steps):
 
 
 def measured_traffic(precision):
-    """HBM bytes per batch of the FP64 push from the committed rocprofv3
-    counter summary (None for the FP32 push, which has no such summary)."""
-    if precision != 0 or not os.path.exists(TRAFFIC_FILE):
+    """HBM bytes per batch of the push (FP64 or FP32) from the committed
+    rocprofv3 counter summary, with its path (None, None without one)."""
+    path = TRAFFIC_FILES.get(precision)
+    if path is None or not os.path.exists(path):
         return None, None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
-    return t["fetch_bytes_per_batch"] + t["write_bytes_per_batch"], os.path.relpath(TRAFFIC_FILE, ROOT)
+    return t["fetch_bytes_per_batch"] + t["write_bytes_per_batch"], os.path.relpath(path, ROOT)
 
 
 def host_array_rate(p, setup, parts, field, phi, calls=5):

@@ -184,10 +184,10 @@ def test_ar5_run_on_device_against_the_reference_run():
     particles per species, seed 0) run on the device with host injection,
     against the reference's own run() of it (tests/golden/run_ar5.npz): every
     count-derived history is identical over all 50 batches, the probe
-    potentials agree to 1e-9 of their range and the final charge to 1e-9 of
-    its maximum (the device potential is G Q, the reference's an LU solve:
-    the same numbers up to rounding, and no trajectory lands on a different
-    cell because of it)."""
+    potentials agree to 1e-10 of their range and the final charge to 1e-10 of
+    its maximum (measured: 1e-15; the device potential is G Q, the
+    reference's an LU solve: the same numbers up to rounding, and no
+    trajectory lands on a different cell because of it)."""
     _case("AR5 run vs reference")
     g = load("run_ar5")
     rp = params_of(g)
@@ -198,5 +198,8 @@ def test_ar5_run_on_device_against_the_reference_run():
         np.testing.assert_array_equal(res["hist"][k], g[f"hist_{k}"], err_msg=k)
     for k in res["probes"]:
         a, b = res["hist"][k], g[f"hist_{k}"]
-        np.testing.assert_allclose(a, b, rtol=0, atol=1e-9 * max(1.0, np.abs(b).max()), err_msg=k)
-    np.testing.assert_allclose(res["rho_raw"], g["rho_raw"], rtol=0, atol=1e-9 * np.abs(g["rho_raw"]).max())
+        print(f"  {k}: max deviation {np.abs(a - b).max() / max(1.0, np.abs(b).max()):.2e} of its range", flush=True)
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-10 * max(1.0, np.abs(b).max()), err_msg=k)
+    print(f"  charge: max deviation {np.abs(res['rho_raw'] - g['rho_raw']).max() / np.abs(g['rho_raw']).max():.2e} "
+          f"of its maximum", flush=True)
+    np.testing.assert_allclose(res["rho_raw"], g["rho_raw"], rtol=0, atol=1e-10 * np.abs(g["rho_raw"]).max())
