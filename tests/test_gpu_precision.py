@@ -136,11 +136,10 @@ def test_split_green_on_matrix_cores(charged, mode, scalar, tol_v, tol_e):
 
 @pytest.mark.parametrize("mode, tol", [(3, 5e-3), (4, 1e-3)])
 def test_potentials_of_many_charge_vectors(charged, mode, tol):
-    """th_charging_potentials: G Q for 16 charge vectors in one pass over G,
-    against the FP64 product on the host; one vector at a time gives each
-    column within rounding."""
-    import ctypes
-    from trench01_amd import _lib
+    """th_charging_potentials (DeviceCharging.potentials): G Q for 16 charge
+    vectors in one pass over G, against the FP64 product on the host; one
+    vector at a time gives each column within rounding; 17 vectors or a
+    wrong column count are refused."""
     s, dev, q, phi64, e64 = charged
     rng = np.random.default_rng(5)
     R = 16
@@ -148,19 +147,15 @@ def test_potentials_of_many_charge_vectors(charged, mode, tol):
     ref = Q @ s.green.T                                       # [R][n_rows]
     dev.set_green_precision(mode)
     try:
-        out = np.empty((R, s.green.shape[0]))
-        ms = ctypes.c_double()
-        Qc = np.ascontiguousarray(Q)
-        _lib.check(_lib.load().th_charging_potentials(dev._h, R, _lib.ptr(Qc, _lib.dp), _lib.ptr(out, _lib.dp),
-                                                      ctypes.byref(ms)))
+        out, ms = dev.potentials(Q)
         err = np.abs(out - ref).max()
-        print(f"mode {mode}: 16 right-hand sides, max error {err:.3g} V, kernel {ms.value * 1e3:.1f} us", flush=True)
+        print(f"mode {mode}: 16 right-hand sides, max error {err:.3g} V, kernel {ms * 1e3:.1f} us", flush=True)
         assert err < tol
-        one = np.empty((1, s.green.shape[0]))
-        q1 = np.ascontiguousarray(Q[3:4])
-        _lib.check(_lib.load().th_charging_potentials(dev._h, 1, _lib.ptr(q1, _lib.dp), _lib.ptr(one, _lib.dp),
-                                                      ctypes.byref(ms)))
-        print(f"mode {mode}: 1 right-hand side, kernel {ms.value * 1e3:.1f} us", flush=True)
+        one, ms = dev.potentials(Q[3])
+        print(f"mode {mode}: 1 right-hand side, kernel {ms * 1e3:.1f} us", flush=True)
         np.testing.assert_allclose(one[0], out[3], rtol=0, atol=1e-6 * np.abs(ref).max())
+        for bad in (np.vstack([Q, Q[:1]]), Q[:, :-1]):
+            with pytest.raises(ValueError):
+                dev.potentials(bad)
     finally:
         dev.set_green_precision(0)

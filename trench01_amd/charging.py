@@ -291,6 +291,21 @@ class DeviceCharging:
         _lib.check(_lib.load().th_charging_set_green(self._h, int(mode)))
         self.green_precision = int(mode)
 
+    def potentials(self, Q):
+        """G Q for 1 to 16 charge vectors Q [R][n_cols] (C/m per Green's
+        column) in one pass over G, on matrix cores; needs split precision 3 or
+        4 (th_charging_potentials).  Returns ([R][n_rows] potentials without the
+        external part, kernel ms)."""
+        Q = np.ascontiguousarray(np.atleast_2d(Q), dtype=np.float64)
+        rows, cols = self.setup.green.shape
+        if Q.ndim != 2 or Q.shape[1] != cols or not 1 <= Q.shape[0] <= 16:
+            raise ValueError(f"charge vectors must be [1..16][{cols}]")
+        out = np.empty((Q.shape[0], rows))
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().th_charging_potentials(self._h, Q.shape[0], _lib.ptr(Q, _lib.dp),
+                                                      _lib.ptr(out, _lib.dp), ctypes.byref(ms)))
+        return out, ms.value
+
     def set_rho(self, rho_s):
         rho_s = np.ascontiguousarray(rho_s, dtype=np.float64)
         _lib.check(_lib.load().th_charging_set_rho(self._h, _lib.ptr(rho_s, _lib.dp)))

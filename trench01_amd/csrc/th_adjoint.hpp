@@ -10,7 +10,8 @@
 //
 // One lane owns one particle.  A forward sweep re-traces the path with the
 // push's own step (step64t, record mode) and stores the state at the start of
-// every step on a tape in HBM (32 bytes per step, at the particle's offset);
+// every step on a tape in HBM (32 bytes per step; a wave's 64 particles share
+// a region laid out step-major, so one step's stores and loads are contiguous);
 // the backward sweep re-runs each step from its tape entry with a policy that
 // captures the step's intermediate values, then applies the transposed
 // hooks:
@@ -208,8 +209,8 @@ struct AdjointIO {
     const double *x, *y, *vx, *vy, *qm;   // initial states [n]
     const int32_t *order;                  // lanes take particles in this order [n]
     const int32_t *steps;                  // path length of each particle [n]
-    const int64_t *tape_off;               // tape offset of each particle [n]
-    double4 *tape;                         // per step: start state {x, y, vx, vy}
+    const int64_t *group_off;              // tape offset of each group of 64 positions of `order`
+    double4 *tape;                         // per group, step-major: [step][64 lanes] start state {x, y, vx, vy}
     const double *bar_end, *bar_vel;       // [n][2] each, nullable
     double *bar_state;                     // [n][4] out
     double *bar_field;                     // [N][2] accumulated, nullable
@@ -221,10 +222,12 @@ struct AdjointIO {
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // begin and stride are multiples of 64, so a wave's lanes hold the 64
+    // positions of one group
     for (int64_t o = a.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < a.end; o += stride) {
         const int pid = a.order[o];
         const int n_steps = a.steps[pid];
-        double4 *tape = a.tape + a.tape_off[pid];
+        double4 *tape = a.tape + a.group_off[o >> 6] + (o & 63);
         State64 s;
         s.x = a.x[pid]; s.y = a.y[pid]; s.vx = a.vx[pid]; s.vy = a.vy[pid]; s.qm = a.qm[pid];
         s.q = div_dx(s.x, g.dx, g.inv_dx);
@@ -234,9 +237,9 @@ __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a
         int cell = -1, k = 0, st = 0;
         NoTangent none;
         // forward sweep: the tape holds the start state of every step (never
-        // more than n_steps entries: the particle's own tape region)
+        // more than n_steps entries, at most the group's longest path)
         while (k < n_steps) {
-            tape[k] = make_double4(s.x, s.y, s.vx, s.vy);
+            tape[64 * (int64_t)k] = make_double4(s.x, s.y, s.vx, s.vy);
             st = step64t<BARRIER, TAPERED, true, false>(g, s, cell, xp, yp, rc, none);
             ++k;
             if (st != 0 || k >= a.max_steps) break;
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a
         const double pe_y = a.bar_end ? a.bar_end[2 * (int64_t)pid + 1] : 0.0;
         FieldBar fb;
         for (int j = n_steps - 1; j >= 0; --j) {
-            const double4 t0 = tape[j];
+            const double4 t0 = tape[64 * (int64_t)j];
             State64 r;
             r.x = t0.x; r.y = t0.y; r.vx = t0.z; r.vy = t0.w; r.qm = s.qm;
             r.q = div_dx(r.x, g.dx, g.inv_dx);

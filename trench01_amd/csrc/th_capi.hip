@@ -1943,25 +1943,30 @@ extern "C" int th_trace_adjoint(th_tracer *t, const double *x, const double *y, 
     TH_HIP(hipSetDevice(t->device));
     if ((rc = reserve_particles(t, n))) return rc;
     if ((rc = upload(t, 0, x, y, vx, vy, qm, spc, n))) return rc;
-    // Lanes take particles longest first (waves of similar path lengths); the
-    // tape is filled chunk by chunk, at most kTapeSteps steps (32 bytes each) at once.
+    // Lanes take particles longest first (waves of similar path lengths).  The
+    // 64 consecutive positions of `order` that one wave traces share a tape
+    // region of 64 x (their longest path) entries, step-major, so that the
+    // wave's tape stores and loads of one step are contiguous; the tape is
+    // filled chunk by chunk (whole groups), at most kTapeSteps entries (32
+    // bytes each) at once.
     const size_t nn = (size_t)n;
     std::vector<int32_t> order(nn);
     for (size_t k = 0; k < nn; ++k) order[k] = (int32_t)k;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
     constexpr int64_t kTapeSteps = (int64_t)1 << 27;   // 4 GiB of tape per chunk
-    std::vector<int64_t> off(nn);
+    const size_t ng = (nn + 63) / 64;
+    std::vector<int64_t> goff(ng);
     std::vector<std::pair<int64_t, int64_t>> chunks;
     int64_t run = 0, begin = 0, tape_steps = 0;
-    for (size_t o = 0; o < nn; ++o) {
-        const int64_t len = steps[order[o]];
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const int64_t len = 64 * (int64_t)steps[order[64 * gi]];   // the group's longest path (sorted)
         if (run + len > kTapeSteps && run > 0) {
-            chunks.emplace_back(begin, (int64_t)o);
+            chunks.emplace_back(begin, (int64_t)(64 * gi));
             tape_steps = std::max(tape_steps, run);
-            begin = (int64_t)o;
+            begin = (int64_t)(64 * gi);
             run = 0;
         }
-        off[order[o]] = run;
+        goff[gi] = run;
         run += len;
     }
     chunks.emplace_back(begin, (int64_t)nn);
@@ -1977,7 +1982,7 @@ extern "C" int th_trace_adjoint(th_tracer *t, const double *x, const double *y, 
                         (void *)d_tape, (void *)d_mis})
             if (q) (void)hipFree(q);
     };
-    if ((rc = dev_alloc(&d_order, nn)) || (rc = dev_alloc(&d_steps, nn)) || (rc = dev_alloc(&d_off, nn)) ||
+    if ((rc = dev_alloc(&d_order, nn)) || (rc = dev_alloc(&d_steps, nn)) || (rc = dev_alloc(&d_off, ng)) ||
         (rc = dev_alloc(&d_bar, 8 * nn)) || (bar_field && (rc = dev_alloc(&d_field, 2 * (size_t)t->N))) ||
         (rc = dev_alloc(&d_tape, (size_t)tape_steps)) || (rc = dev_alloc(&d_mis, 1))) {
         release();
@@ -1986,7 +1991,7 @@ extern "C" int th_trace_adjoint(th_tracer *t, const double *x, const double *y, 
     double *d_end = d_bar, *d_vel = d_bar + 2 * nn, *d_state = d_bar + 4 * nn;
     hipError_t e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_steps, steps, sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, goff.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess && bar_end)
         e = hipMemcpyAsync(d_end, bar_end, sizeof(double) * 2 * nn, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess && bar_vel)

@@ -453,10 +453,16 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     s.vy = vyn;
     s.q = div_dx(s.x, g.dx, g.inv_dx);
     s.r = div_dx(s.y, g.dx, g.inv_dx);
-    // Classification of the end point.
+    // Classification of the end point.  The end cell is nearly always one of
+    // the four interpolation cells (a step moves less than half a cell at
+    // cfl <= 0.5): then its vacuum flag is one of fl's bits, the same byte of
+    // the vacuum map, and the map is read only otherwise.
     bool escaped = s.y >= g.len_y();
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
-    bool hit = !escaped && !g.is_vac_at(ix, iy);
+    const int xe = ix == i0 ? 0 : (ix == i1 ? 1 : -1);
+    const int ye = iy == j0 ? 0 : (iy == j1 ? 2 : -1);
+    bool hit = false;
+    if (!escaped) hit = (xe >= 0 && ye >= 0) ? !((fl >> (xe + ye)) & 1u) : !g.is_vac_at(ix, iy);
     bool corner = false;
     if (TAPERED && g.taper && !hit && !escaped) {
         const int ix0c = clip_cell((int)q0, nx), iy0c = clip_cell((int)r0, ny);
@@ -468,8 +474,8 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         // iy - iy0, so it is skipped when the interpolation's four cells hold
         // both side cells and flag them vacuum.
         if (corner && xu >= 0.0 && xu < g.len_x() && s.q < (double)nx) {
-            const int xe = ix == i0 ? 0 : (ix == i1 ? 1 : -1), xs = ix0c == i0 ? 0 : (ix0c == i1 ? 1 : -1);
-            const int ye = iy == j0 ? 0 : (iy == j1 ? 2 : -1), ys = iy0c == j0 ? 0 : (iy0c == j1 ? 2 : -1);
+            const int xs = ix0c == i0 ? 0 : (ix0c == i1 ? 1 : -1);
+            const int ys = iy0c == j0 ? 0 : (iy0c == j1 ? 2 : -1);
             if (xe >= 0 && xs >= 0 && ye >= 0 && ys >= 0 && ((fl >> (xe + ys)) & 1u) && ((fl >> (xs + ye)) & 1u))
                 corner = false;
         }
@@ -598,12 +604,27 @@ __device__ __forceinline__ int step32(const Grid &g, State32 &s, int &cell) {
     s.vy = vyn;
     s.q = s.x * inv_dx;
     s.r = s.y * inv_dx;
+    // End cell and diagonal moves as in step64t: vacuum flags from the four
+    // interpolation cells' records where they hold the cell, else the map.
     bool escaped = s.y >= ly;
     int ix = clip_cell((int)s.q, nx), iy = clip_cell((int)s.r, ny);
-    bool hit = !escaped && !g.vac[ix * ny + iy];
+    const unsigned fl = (c00.z != 0.0f ? 1u : 0u) | (c10.z != 0.0f ? 2u : 0u) | (c01.z != 0.0f ? 4u : 0u) |
+                        (c11.z != 0.0f ? 8u : 0u);
+    const int xe = ix == i0 ? 0 : (ix == i1 ? 1 : -1);
+    const int ye = iy == j0 ? 0 : (iy == j1 ? 2 : -1);
+    bool hit = false;
+    if (!escaped) hit = (xe >= 0 && ye >= 0) ? !((fl >> (xe + ye)) & 1u) : !g.vac[ix * ny + iy];
     bool corner = false;
-    if (TAPERED && !hit && !escaped)
-        corner = ix != clip_cell((int)q0, nx) && iy != clip_cell((int)r0, ny);
+    if (TAPERED && !hit && !escaped) {
+        const int ix0c = clip_cell((int)q0, nx), iy0c = clip_cell((int)r0, ny);
+        corner = ix != ix0c && iy != iy0c;
+        if (corner && xu >= 0.0f && xu < lx && s.q < (float)nx) {
+            const int xs = ix0c == i0 ? 0 : (ix0c == i1 ? 1 : -1);
+            const int ys = iy0c == j0 ? 0 : (iy0c == j1 ? 2 : -1);
+            if (xe >= 0 && xs >= 0 && ye >= 0 && ys >= 0 && ((fl >> (xe + ys)) & 1u) && ((fl >> (xs + ye)) & 1u))
+                corner = false;
+        }
+    }
     if ((hit && (BARRIER || TAPERED)) || corner) {
         int ix0 = clip_cell((int)q0, nx), iy0 = clip_cell((int)r0, ny);
         int dix = (int)floorf(xu * inv_dx) - ix0;
