@@ -884,9 +884,11 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
         atomicAdd(&w.scal[threadIdx.x], lds_scal[threadIdx.x]);
 }
 
-// FP64 push.  94 VGPRs without spills fits 5 waves per SIMD (6 would spill).
+// FP64 push.  At 4 waves per SIMD it keeps its 107 VGPRs unspilled; 5 waves
+// (95 VGPRs, two spilled) measured 10.85 against 9.7-9.85 ms per AR 8 batch of
+// 1e6 particles, 6 waves (80 VGPRs, 16 spilled) 11.9 ms.
 template <bool BARRIER, bool TAPERED>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) push64_kernel(Grid g, Work w) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) push64_kernel(Grid g, Work w) {
     push_body<kModeF64, BARRIER, TAPERED>(g, w);
 }
 
