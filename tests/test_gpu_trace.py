@@ -28,6 +28,19 @@ def test_device_arithmetic_matches_host_c_library():
         parts.append((rng.standard_normal(n // 8) * 2.0 ** ea, rng.standard_normal(n // 8) * 2.0 ** eb))
     e = rng.integers(-1070, 1020, n // 8)
     parts.append((np.ldexp(rng.random(n // 8) + .5, e), np.ldexp(rng.random(n // 8) + .5, e + rng.integers(-40, 40, n // 8))))
+    # hypot operands at the edges of its range-free core: widely separated pairs around the 2^-54 ratio and
+    # the smaller operand around 2^-383, over the exponent range
+    x = np.ldexp(rng.random(n // 8) + .5, rng.integers(-383, 511, n // 8))
+    parts.append((x, x * 2.0 ** -54 * (1 + rng.uniform(-1e-3, 1e-3, x.size))))
+    y = np.ldexp(rng.random(n // 8) + .5, rng.integers(-386, -380, n // 8))
+    parts.append((y * 2.0 ** rng.uniform(0, 60, y.size), y))
+    # quotients near the ranges where the division's scaling steps act
+    parts.append((np.ldexp(rng.random(n // 16) + .5, rng.integers(-975, -960, n // 16)),
+                  np.ldexp(rng.random(n // 16) + .5, rng.integers(-10, 10, n // 16))))
+    parts.append((np.ldexp(rng.random(n // 16) + .5, rng.integers(-60, 60, n // 16)),
+                  np.ldexp(rng.random(n // 16) + .5, rng.integers(1015, 1024, n // 16))))
+    parts.append((np.ldexp(rng.random(n // 16) + .5, rng.integers(700, 1023, n // 16)),
+                  np.ldexp(rng.random(n // 16) + .5, rng.integers(-80, 10, n // 16))))
     a = np.concatenate([p[0] for p in parts])
     b = np.concatenate([p[1] for p in parts])
     t = HipTracer(np.ones((4, 4), bool))
@@ -37,6 +50,18 @@ def test_device_arithmetic_matches_host_c_library():
         np.testing.assert_array_equal(out[:, 1], np.sqrt(a))
         np.testing.assert_array_equal(out[:, 2], a / b)
         np.testing.assert_array_equal(out[:, 3], np.fmod(a, b))
+        # the range-free forms equal the library operations wherever the IEEE sequences do not rescale
+        ok = (np.abs(a) >= 2.0 ** -767) & np.isfinite(a)
+        assert ok.sum() > n // 2
+        np.testing.assert_array_equal(out[ok, 4], np.sqrt(np.abs(a[ok])))
+        q = a / b
+        ea, eb = np.frexp(a)[1], np.frexp(b)[1]
+        ok = (np.abs(b) >= 2.0 ** -1022) & (np.abs(b) < 2.0 ** 1021) & np.isfinite(a) & \
+             ((a == 0) | (np.abs(a) >= 2.0 ** -968)) & ((a == 0) | (np.abs(q) >= 2.0 ** -1021)) & \
+             np.isfinite(q) & (ea - eb < 767)
+        assert ok.sum() > n // 2
+        np.testing.assert_array_equal(out[ok, 5], q[ok])
+        np.testing.assert_array_equal(out[:, 6], 1.75e-9 / (np.abs(a) + np.sqrt(1e-8 * np.abs(b)) + 1.0))
 
 
 CASES = {

@@ -10,6 +10,71 @@
 #define TH_HD static inline
 #endif
 
+// Square root and quotient without the range steps of the IEEE sequences.
+//
+// On gfx950 the compiler evaluates a correctly rounded FP64 sqrt(x) as: scale
+// x below 2^-767 by 2^256; r = rsq(x); g = x r, h = r / 2; one refinement
+// e = 1/2 - h g, g += g e, h += h e; two corrections d = x - g g, g += d h;
+// scale g back by 2^-128 if x was scaled; return x itself for +-0 and +inf.
+// For x in [2^-767, DBL_MAX] both scalings are by 2^0 and the final select
+// keeps g, so th_sqrt_n -- the same instructions without those steps --
+// returns the same double, i.e. the correctly rounded square root.
+//
+// A correctly rounded a / b is evaluated as: d = div_scale(b), n =
+// div_scale(a), each the operand itself or the operand scaled by 2^+-128;
+// r = rcp(d) refined twice by r += r (1 - d r); q = n r; q = fma(n - d q, r,
+// q), scaled back when n was scaled; then div_fixup, which gives the result
+// its sign and handles zero, infinite and NaN operands.  div_scale scales
+// only for a = 0 or b = 0 (then div_fixup alone decides the result), a
+// denormal b, |b| >= 2^1022 (1/b denormal), |a| < 2^-969, a quotient in the
+// denormal range, or an exponent of a exceeding that of b by 768 or more.
+// Outside those cases th_div_n -- the same instructions without the two
+// div_scale steps -- returns the same double; it is used only where the
+// operands are bounded away from them (stated at each use).  The host
+// versions are the library operations, which are correctly rounded too.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double th_sqrt_n(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+__device__ __forceinline__ double th_div_n(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    double q = a * r;
+    const double rem = fma(-b, q, a);
+    q = fma(rem, r, q);
+    return __builtin_amdgcn_div_fixup(q, b, a);
+}
+#else
+TH_HD double th_sqrt_n(double x) { return sqrt(x); }
+TH_HD double th_div_n(double a, double b) { return a / b; }
+#endif
+
+// The push's CFL step cfl_dx / (hv + sqrt(two_dx ha) + 1) with the square
+// root and the division in their range-free forms where the operands allow
+// (velocities and accelerations of any physical particle), else the library
+// operations: the same double either way.
+TH_HD double th_cfl_dt(double cfl_dx, double two_dx, double hv, double ha) {
+    const double s = two_dx * ha;
+    const double root = (s >= 0x1p-767 && s <= 1.7976931348623157e308) ? th_sqrt_n(s) : sqrt(s);
+    const double den = hv + root + 1.0;
+    // den >= 1 unless NaN: with den <= 2^800 and cfl_dx in [2^-100, 2^100] the
+    // quotient lies in [2^-900, 2^100] and no operand is near a scaling case.
+    if (den <= 0x1p+800 && cfl_dx >= 0x1p-100 && cfl_dx <= 0x1p+100) return th_div_n(cfl_dx, den);
+    return cfl_dx / den;
+}
+
 // Hypot core from C. F. Borges, "An Improved Algorithm for hypot(a,b)",
 // arXiv:1904.09481 (2019): h = sqrt(a^2+b^2) followed by one correction step
 // computed without FMA (Borges' non-fused variant).  This is the algorithm of
@@ -59,6 +124,28 @@ TH_HD double th_hypot_full(double x, double y) {
     return th_hypot_core(ax * down, ay * down) * up;
 }
 
+// th_hypot_core with the range-free square root and quotient, for 2^-383 <=
+// ay <= ax <= 2^511 and ay > ax 2^-54: the square's argument lies in
+// [2^-766, 2^1023]; the correction's divisor 2h in [2^-382, 2^513]; its
+// dividend t1 + t2 is at most h^2 2^-50 (the correction is below one ulp of
+// h) and is 0 or at least h^2 2^-160 (a sum of rounded products of operands
+// no smaller than ay >= h 2^-54 is a multiple of one ulp of ay^2), so the
+// quotient lies in [h 2^-161, h 2^-51] and no operand is near a scaling case.
+TH_HD double th_hypot_core_n(double ax, double ay) {
+    double h = th_sqrt_n(ax * ax + ay * ay);
+    double t1, t2;
+    if (h <= 2.0 * ay) {
+        double d = h - ay;
+        t1 = ax * (2.0 * d - ax);
+        t2 = (d - 2.0 * (ax - ay)) * d;
+    } else {
+        double d = h - ax;
+        t1 = 2.0 * d * (ax - 2.0 * ay);
+        t2 = (4.0 * d - ay) * ay + d * d;
+    }
+    return h - th_div_n(t1 + t2, 2.0 * h);
+}
+
 // The same function with the common case first: operands in the core's range
 // (no rescaling, not widely separated, finite, non-zero) go straight to the
 // core, which is what th_hypot_full computes for them (its scalings are then
@@ -66,8 +153,8 @@ TH_HD double th_hypot_full(double x, double y) {
 TH_HD double th_hypot(double x, double y) {
     double fx = fabs(x), fy = fabs(y);
     double ax = fx < fy ? fy : fx, ay = fx < fy ? fx : fy;
-    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-459 && ay > ax * 0x1p-54, 1))
-        return th_hypot_core(ax, ay);
+    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-383 && ay > ax * 0x1p-54, 1))
+        return th_hypot_core_n(ax, ay);
     return th_hypot_full(x, y);
 }
 

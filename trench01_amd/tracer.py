@@ -362,10 +362,11 @@ class HipTracer:
         return bar_state, bar_field
 
     def selftest_math(self, a, b):
-        """Device hypot / sqrt / divide / fmod on arrays (for arithmetic pinning)."""
+        """Device arithmetic on arrays (for arithmetic pinning): columns hypot(a, b), sqrt(a), a / b,
+        fmod(a, b), the range-free sqrt(|a|) and a / b, and the CFL step 1.75e-9 / (|a| + sqrt(1e-8 |b|) + 1)."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
-        out = np.empty((a.size, 4), dtype=np.float64)
+        out = np.empty((a.size, 7), dtype=np.float64)
         _lib.check(_lib.load().th_selftest_math(self._h, _lib.ptr(a, _lib.dp), _lib.ptr(b, _lib.dp), a.size,
                                                 _lib.ptr(out, _lib.dp)))
         return out
