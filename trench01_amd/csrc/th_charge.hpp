@@ -116,12 +116,18 @@ __device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, do
             q_new[k] = d.leak_has_c ? q + d.leak_c[k] : q;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        // one wave per row of M: lanes read the row coalesced, partial sums
+        // combined by a fixed shuffle tree
+        const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+        for (int k = threadIdx.x >> 6; k < m; k += waves) {
             const double *row = d.leak_M + (int64_t)k * m;
             double acc = 0.0;
-            for (int j = 0; j < m; ++j) acc += row[j] * q_new[j];
-            d.rho[k] = acc / dx2;
-            q_old[k] = d.region[k] ? q_old[k] - acc : 0.0;   // region outflow terms
+            for (int j = lane; j < m; j += 64) acc += row[j] * q_new[j];
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+            if (lane == 0) {
+                d.rho[k] = acc / dx2;
+                q_old[k] = d.region[k] ? q_old[k] - acc : 0.0;   // region outflow terms
+            }
         }
         __syncthreads();
         double part = 0.0;
@@ -154,12 +160,20 @@ __device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, do
         __syncthreads();
     }
     if (threadIdx.x == 0) d.diag[kDiagLeakDQ] = leak_dq;
-    // charge per Green's column
-    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) d.qcol[c] = 0.0;
+    // charge per Green's column: thread c sums its surface cells in index order
+    // from LDS copies of rho and the column map
+    int *col_s = reinterpret_cast<int *>(q_new);
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        q_old[k] = d.rho[k];
+        col_s[k] = d.col_of[k];
+    }
     __syncthreads();
-    if (threadIdx.x == 0)
+    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) {
+        double acc = 0.0;
         for (int k = 0; k < m; ++k)
-            if (d.col_of[k] >= 0) d.qcol[d.col_of[k]] += d.rho[k] * dx2;
+            if (col_s[k] == c) acc += q_old[k] * dx2;
+        d.qcol[c] = acc;
+    }
 }
 
 __global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {

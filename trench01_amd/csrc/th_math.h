@@ -67,12 +67,12 @@ TH_HD double th_div_n(double a, double b) { return a / b; }
 // operations: the same double either way.
 TH_HD double th_cfl_dt(double cfl_dx, double two_dx, double hv, double ha) {
     const double s = two_dx * ha;
-    const double root = (s >= 0x1p-767 && s <= 1.7976931348623157e308) ? th_sqrt_n(s) : sqrt(s);
-    const double den = hv + root + 1.0;
-    // den >= 1 unless NaN: with den <= 2^800 and cfl_dx in [2^-100, 2^100] the
-    // quotient lies in [2^-900, 2^100] and no operand is near a scaling case.
-    if (den <= 0x1p+800 && cfl_dx >= 0x1p-100 && cfl_dx <= 0x1p+100) return th_div_n(cfl_dx, den);
-    return cfl_dx / den;
+    // With s in [2^-767, 2^1000] the root lies in [2^-384, 2^500]; with 0 <= hv
+    // <= 2^799 the divisor lies in [1, 2^800), and with cfl_dx in [2^-100,
+    // 2^100] the quotient in [2^-900, 2^100]: no operand is near a scaling case.
+    if (s >= 0x1p-767 && s <= 0x1p+1000 && hv <= 0x1p+799 && cfl_dx >= 0x1p-100 && cfl_dx <= 0x1p+100)
+        return th_div_n(cfl_dx, hv + th_sqrt_n(s) + 1.0);
+    return cfl_dx / (hv + sqrt(s) + 1.0);
 }
 
 // Hypot core from C. F. Borges, "An Improved Algorithm for hypot(a,b)",
@@ -131,18 +131,18 @@ TH_HD double th_hypot_full(double x, double y) {
 // h) and is 0 or at least h^2 2^-160 (a sum of rounded products of operands
 // no smaller than ay >= h 2^-54 is a multiple of one ulp of ay^2), so the
 // quotient lies in [h 2^-161, h 2^-51] and no operand is near a scaling case.
+// Both correction forms are evaluated and Borges' test selects one (no branch:
+// the push's waves hold operands of both kinds).
 TH_HD double th_hypot_core_n(double ax, double ay) {
-    double h = th_sqrt_n(ax * ax + ay * ay);
-    double t1, t2;
-    if (h <= 2.0 * ay) {
-        double d = h - ay;
-        t1 = ax * (2.0 * d - ax);
-        t2 = (d - 2.0 * (ax - ay)) * d;
-    } else {
-        double d = h - ax;
-        t1 = 2.0 * d * (ax - 2.0 * ay);
-        t2 = (4.0 * d - ay) * ay + d * d;
-    }
+    const double h = th_sqrt_n(ax * ax + ay * ay);
+    const double d1 = h - ay;
+    const double t11 = ax * (2.0 * d1 - ax);
+    const double t21 = (d1 - 2.0 * (ax - ay)) * d1;
+    const double d2 = h - ax;
+    const double t12 = 2.0 * d2 * (ax - 2.0 * ay);
+    const double t22 = (4.0 * d2 - ay) * ay + d2 * d2;
+    const bool first = h <= 2.0 * ay;
+    const double t1 = first ? t11 : t12, t2 = first ? t21 : t22;
     return h - th_div_n(t1 + t2, 2.0 * h);
 }
 

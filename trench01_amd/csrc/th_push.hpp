@@ -241,6 +241,21 @@ struct StateT {
 using State64 = StateT<double>;
 using State32 = StateT<float>;
 
+typedef double th_dvec4 __attribute__((ext_vector_type(4)));
+
+// The interpolation record a lane holds for its next step (single-instance
+// FP64 push): the lattice key it belongs to (-1: none) and the record itself.
+// A step that ends on a new lattice key issues the next record's loads before
+// its classification, so their latency overlaps that work instead of opening
+// the next step; a step that stays on its key reads nothing.  The record
+// depends only on the key within one launch (one field), so it stays valid
+// when the lane takes a new particle or a reflection restores the start point.
+struct CornerCache {
+    int key = -1;
+    unsigned flags = 0u;
+    th_dvec4 a, b;
+};
+
 // Step hooks of the FP64 push that do nothing: the push kernels' policy.
 struct NoTangent {
     template <class V>
@@ -344,9 +359,9 @@ struct Tangent64 {
 // records: Grid itself, or an ensemble lane's view (th_ensemble.hpp); cell
 // indices are the instance's own.  T receives the step's intermediate values
 // (NoTangent, or Tangent64 for the derivative of the path).
-template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V, class T>
+template <bool BARRIER, bool TAPERED, bool RECORD, bool RF, class V, class T, bool CACHE = false>
 __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double &xp, double &yp, RfClock &rc,
-                                       T &tan) {
+                                       T &tan, CornerCache *cc = nullptr) {
     static_assert(std::is_same<T, NoTangent>::value || (RECORD && !RF), "the tangent follows the FP64 record step");
     const int nx = g.nx, ny = g.ny;
     // Vacuum-weighted bilinear field at the cell-centred lattice.
@@ -379,13 +394,26 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         if (!staged) {
             // one 64-byte corner record and one flag byte per step
             const int key = (i + 1) * (ny + 1) + (j + 1);
-            // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
-            typedef double dvec4 __attribute__((ext_vector_type(4)));
-            const dvec4 *c = (const dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
-            const dvec4 a = c[0], b = c[1];
+            th_dvec4 a, b;
+            if (CACHE && cc->key == key) {
+                a = cc->a;
+                b = cc->b;
+                f = cc->flags;
+            } else {
+                // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
+                const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
+                a = c[0];
+                b = c[1];
+                f = g.corner_flags(key);
+                if constexpr (CACHE) {
+                    cc->key = key;
+                    cc->a = a;
+                    cc->b = b;
+                    cc->flags = f;
+                }
+            }
             e00 = make_double2(a.x, a.y); e10 = make_double2(a.z, a.w);
             e01 = make_double2(b.x, b.y); e11 = make_double2(b.z, b.w);
-            f = g.corner_flags(key);
         }
         fl = f;
         v00 = (f & 1u) ? 1.0 : 0.0; v10 = (f & 2u) ? 1.0 : 0.0;
@@ -453,6 +481,17 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     s.vy = vyn;
     s.q = div_dx(s.x, g.dx, g.inv_dx);
     s.r = div_dx(s.y, g.dx, g.inv_dx);
+    if constexpr (CACHE) {   // the next step's record, if its key is new and inside the period and grid
+        const int ni = (int)floor(s.q - 0.5), nj = (int)floor(s.r - 0.5);
+        const int nkey = (ni + 1) * (ny + 1) + (nj + 1);
+        if ((unsigned)(ni + 1) <= (unsigned)nx && (unsigned)(nj + 1) <= (unsigned)ny && nkey != cc->key) {
+            const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(nkey), 64);
+            cc->a = c[0];
+            cc->b = c[1];
+            cc->flags = g.corner_flags(nkey);
+            cc->key = nkey;
+        }
+    }
     // Classification of the end point.  The end cell is nearly always one of
     // the four interpolation cells (a step moves less than half a cell at
     // cfl <= 0.5): then its vacuum flag is one of fl's bits, the same byte of
@@ -729,15 +768,19 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     const unsigned long long below = (1ull << lane) - 1ull;
     const int64_t q_begin = w.in ? 0 : w.begin;
     const int64_t q_end = w.in ? (int64_t)*w.in_count : w.end;
-    int64_t pid = -1;   // >= 0 particle, -1 idle, -2 no work left
+    int pid = -1;       // >= 0 particle (pools hold < 2^31), -1 idle, -2 no work left
     StateT<Real> s;
-    int sp = 0, k = 0, k0 = 0;    // step count, and its value when this pass took the particle
+    int sp = 0, k = 0;            // species, step count
     int slot = -1, plen = 0;      // record mode: probe slot and path length
     double xp = 0.0, yp = 0.0;    // record mode: last recorded point
     RfClock rc{0.0, 0.0, 0.0, 0.0};
     Tan tn;
     if constexpr (TANGENT) tn.dfield = w.tan.dfield;
-    unsigned long long my_steps = 0;
+    CornerCache cache;
+    // Steps this lane traced: each particle adds its step count when it ends
+    // here and subtracts the count it arrived with, modulo 2^32 (a lane traces
+    // far fewer than 2^32 steps in one launch).
+    unsigned my_steps = 0u;
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
         unsigned long long busy = __ballot(pid >= 0);
@@ -755,16 +798,15 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         pid = v.pid;
                         s.x = (Real)v.x; s.y = (Real)v.y; s.vx = (Real)v.vx; s.vy = (Real)v.vy;
                         k = v.k;
-                        k0 = k;
+                        my_steps -= (unsigned)k;
                         if constexpr (RF) {
                             rc.clock = v.clock;
                             rc.elapsed = v.elapsed;
                         }
                     } else {
-                        pid = q;
+                        pid = (int)q;
                         s.x = (Real)w.x[q]; s.y = (Real)w.y[q]; s.vx = (Real)w.vx[q]; s.vy = (Real)w.vy[q];
                         k = 0;
-                        k0 = 0;
                         if constexpr (RF) {
                             rc.clock = py_mod(w.t0[q], g.period);
                             rc.elapsed = 0.0;
@@ -821,6 +863,8 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                     st = step32<BARRIER, TAPERED>(g, s, cell);
                 } else if constexpr (TILE) {
                     st = step64t<BARRIER, TAPERED, false, false>(tview, s, cell, xp, yp, rc, tn);
+                } else if constexpr (MODE == kModeF64) {
+                    st = step64t<BARRIER, TAPERED, false, false, Grid, Tan, true>(g, s, cell, xp, yp, rc, tn, &cache);
                 } else {
                     st = step64t<BARRIER, TAPERED, RECORD, RF>(g, s, cell, xp, yp, rc, tn);
                     if constexpr (RECORD) {
@@ -854,7 +898,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         w.tan.face[pid] = (int8_t)tn.face;
                         w.tan.steps[pid] = k;
                     }
-                    my_steps += (unsigned long long)(k - k0);
+                    my_steps += (unsigned)k;
                     pid = -1;
                     break;
                 }
@@ -866,17 +910,17 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         v.clock = rc.clock;
                         v.elapsed = rc.elapsed;
                     }
-                    v.pid = (int32_t)pid;
+                    v.pid = pid;
                     v.k = k;
                     static_cast<Surv *>(w.out)[slot_out] = v;
-                    my_steps += (unsigned long long)(k - k0);
+                    my_steps += (unsigned)k;
                     pid = -1;
                     break;
                 }
             }
         }
     }
-    atomicAdd(&lds_scal[kScalSteps], my_steps);
+    atomicAdd(&lds_scal[kScalSteps], (unsigned long long)my_steps);
     __syncthreads();
     for (int c = threadIdx.x; c < 2 * g.m; c += kBlock)
         if (lds_counts[c]) atomicAdd(&w.counts[c], lds_counts[c]);
