@@ -245,11 +245,11 @@ typedef double th_dvec4 __attribute__((ext_vector_type(4)));
 
 // The interpolation record a lane holds for its next step (single-instance
 // FP64 push): the lattice key it belongs to (-1: none) and the record itself.
-// A step that ends on a new lattice key issues the next record's loads before
-// its classification, so their latency overlaps that work instead of opening
-// the next step; a step that stays on its key reads nothing.  The record
-// depends only on the key within one launch (one field), so it stays valid
-// when the lane takes a new particle or a reflection restores the start point.
+// Each step issues the next record's loads as soon as the new position is
+// known, before its classification, so their latency overlaps that work
+// instead of opening the next step.  The record depends only on the key within
+// one launch (one field): a lane that takes a new particle, or whose particle
+// a reflection sends back, uses it when the key matches and loads otherwise.
 struct CornerCache {
     int key = -1;
     unsigned flags = 0u;
@@ -395,22 +395,21 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
             // one 64-byte corner record and one flag byte per step
             const int key = (i + 1) * (ny + 1) + (j + 1);
             th_dvec4 a, b;
-            if (CACHE && cc->key == key) {
+            // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
+            const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
+            if constexpr (CACHE) {
+                if (cc->key != key) {   // no record prefetched for this key (new particle, reflection)
+                    cc->a = c[0];
+                    cc->b = c[1];
+                    cc->flags = g.corner_flags(key);
+                }
                 a = cc->a;
                 b = cc->b;
                 f = cc->flags;
             } else {
-                // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
-                const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
                 a = c[0];
                 b = c[1];
                 f = g.corner_flags(key);
-                if constexpr (CACHE) {
-                    cc->key = key;
-                    cc->a = a;
-                    cc->b = b;
-                    cc->flags = f;
-                }
             }
             e00 = make_double2(a.x, a.y); e10 = make_double2(a.z, a.w);
             e01 = make_double2(b.x, b.y); e11 = make_double2(b.z, b.w);
@@ -481,16 +480,17 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     s.vy = vyn;
     s.q = div_dx(s.x, g.dx, g.inv_dx);
     s.r = div_dx(s.y, g.dx, g.inv_dx);
-    if constexpr (CACHE) {   // the next step's record, if its key is new and inside the period and grid
+    if constexpr (CACHE) {
+        // The next step's record (a position outside the period or grid loads
+        // key 0's record and marks it unusable).
         const int ni = (int)floor(s.q - 0.5), nj = (int)floor(s.r - 0.5);
-        const int nkey = (ni + 1) * (ny + 1) + (nj + 1);
-        if ((unsigned)(ni + 1) <= (unsigned)nx && (unsigned)(nj + 1) <= (unsigned)ny && nkey != cc->key) {
-            const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(nkey), 64);
-            cc->a = c[0];
-            cc->b = c[1];
-            cc->flags = g.corner_flags(nkey);
-            cc->key = nkey;
-        }
+        const bool inside = (unsigned)(ni + 1) <= (unsigned)nx && (unsigned)(nj + 1) <= (unsigned)ny;
+        const int nkey = inside ? (ni + 1) * (ny + 1) + (nj + 1) : 0;
+        const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(nkey), 64);
+        cc->a = c[0];
+        cc->b = c[1];
+        cc->flags = g.corner_flags(nkey);
+        cc->key = inside ? nkey : -1;
     }
     // Classification of the end point.  The end cell is nearly always one of
     // the four interpolation cells (a step moves less than half a cell at

@@ -34,7 +34,7 @@ class HipTracer:
     precision = 0
     #: push schedule (see trench01_amd.autotune); the defaults are the fastest
     #: of the schedule sweep on the AR 8 workload (MI355X)
-    refill = 32
+    refill = 16
     order = 1
     prio_steps = 0
     check_steps = 8
