@@ -135,6 +135,8 @@ void surface_cells(const std::vector<uint8_t> &vac, int nx, int ny, std::vector<
 }
 
 constexpr int kMaxPasses = 32;
+// d_scal: the push's kScalCount counters, then one work counter per pass.
+constexpr int kScalAll = th::kScalCount + kMaxPasses + 1;
 
 void free_particles(th_tracer *t) {
     for (void *p : {(void *)t->d_x, (void *)t->d_y, (void *)t->d_vx, (void *)t->d_vy, (void *)t->d_qm,
@@ -340,7 +342,7 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
         (rc = dev_alloc(&t->d_corner_vac, (size_t)(nx + 1) * (ny + 1))) ||
         (rc = dev_alloc(&t->d_wface, t->N)) || (rc = dev_alloc(&t->d_tmp, 3 * (size_t)t->N)) ||
         (rc = dev_alloc(&t->d_counts, 2 * (size_t)std::max(t->m, 1))) ||
-        (rc = dev_alloc(&t->d_overflow, 2 * (size_t)t->N)) || (rc = dev_alloc(&t->d_scal, th::kScalCount)))
+        (rc = dev_alloc(&t->d_overflow, 2 * (size_t)t->N)) || (rc = dev_alloc(&t->d_scal, kScalAll)))
         return bad(rc);
     if (hipMemcpy(t->d_vac, t->vac.data(), t->N, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(t->d_smap, smap.data(), sizeof(int32_t) * t->N, hipMemcpyHostToDevice) != hipSuccess)
@@ -449,7 +451,7 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
     t->blocks_per_cu = p->blocks_per_cu;
     TH_HIP(hipMemsetAsync(t->d_counts, 0, sizeof(uint32_t) * 2 * (size_t)std::max(t->m, 1), t->stream));
     TH_HIP(hipMemsetAsync(t->d_overflow, 0, sizeof(uint32_t) * 2 * (size_t)t->N, t->stream));
-    TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * th::kScalCount, t->stream));
+    TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long) * kScalAll, t->stream));   // incl. pass queues
     *traced = off;
     TH_HIP(hipEventRecord(t->ev0, t->stream));
     if (n > 0) {
@@ -476,6 +478,7 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
         w.in_count = nullptr;
         w.out = nullptr;
         w.out_count = nullptr;
+        w.queue = t->d_scal + th::kScalQueue;
         int rc;
         const bool passes = !stop && !rec && !tan && p->pass_steps > 0 && p->pass_steps < p->max_steps;
         if (passes) {
@@ -493,7 +496,7 @@ int launch_trace(th_tracer *t, int64_t off, int64_t n, const th_trace_params *p,
                 w.in_count = pass ? t->d_surv_count + pass - 1 : nullptr;
                 w.out = last ? nullptr : t->d_surv[pass & 1];
                 w.out_count = last ? nullptr : t->d_surv_count + pass;
-                TH_HIP(hipMemsetAsync(t->d_scal, 0, sizeof(unsigned long long), t->stream));   // work counter
+                w.queue = t->d_scal + th::kScalCount + pass;   // cleared above
                 const bool tile = p->tile_pass > 0 && pass + 1 >= p->tile_pass && t->tile_w > 0 && p->precision == 0 &&
                                   !p->rf;
                 if ((rc = tile ? launch_tile(t, g, w, p->barrier != 0, p->tapered != 0)
@@ -786,6 +789,7 @@ struct th_charging {
     th::ChargeDev d{};
     int32_t *col_of = nullptr, *rows = nullptr, *surf = nullptr;
     double *green = nullptr, *external = nullptr, *leak_M = nullptr, *leak_c = nullptr, *g_cond_s = nullptr;
+    double *leak_q = nullptr;   // [2m] the leak's input and outflow terms between the update's launches
     uint8_t *region = nullptr, *cond_s = nullptr;
     double *rho = nullptr, *rho_prev = nullptr, *qcol = nullptr, *phi = nullptr, *diag = nullptr;
     hipEvent_t ev2 = nullptr;
@@ -888,6 +892,7 @@ void th_charging_destroy(th_charging *c) {
                     (void *)c->leak_M, (void *)c->leak_c, (void *)c->g_cond_s, (void *)c->region,
                     (void *)c->cond_s, (void *)c->rho, (void *)c->rho_prev, (void *)c->qcol, (void *)c->phi,
                     (void *)c->diag, (void *)c->g_hi, (void *)c->g_lo, (void *)c->rep_counts, (void *)c->rep_scal,
+                    (void *)c->leak_q,
                     (void *)c->rep_rho, (void *)c->rep_prev, (void *)c->rep_diag, (void *)c->rep_qcol})
         if (p) (void)hipFree(p);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -935,6 +940,7 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
         (rc = upload_new(&c->diag, (const double *)nullptr, th::kDiagCount, st)))
         return bad(rc);
     if (s->leak_M && (rc = upload_new(&c->leak_M, s->leak_M, (size_t)m * m, st))) return bad(rc);
+    if (s->leak_M && (rc = upload_new(&c->leak_q, (const double *)nullptr, 2 * (size_t)m, st))) return bad(rc);
     if (s->leak_c && (rc = upload_new(&c->leak_c, s->leak_c, m, st))) return bad(rc);
     if (hipMemcpyAsync(t->d_wface, s->wface, sizeof(double) * t->N, hipMemcpyHostToDevice, st) != hipSuccess)
         return bad(fail(TH_EHIP, "wface upload failed"));
@@ -1071,6 +1077,25 @@ int th_charging_set_rho(th_charging *c, const double *rho_s) {
     return TH_OK;
 }
 
+// The batch's charge update on the tracer's stream: one workgroup, or with a
+// leak operator three launches so that the leak's GEMV streams M over many
+// workgroups (same arithmetic, same order).
+static int launch_update(th_charging *c) {
+    th_tracer *t = c->t;
+    if (c->leak_q) {
+        const int m = c->d.m;
+        hipLaunchKernelGGL(th::charge_deposit_kernel, dim3(1), dim3(th::kUpdBlock), 0, t->stream, c->d, c->leak_q);
+        hipLaunchKernelGGL(th::leak_rows_kernel, dim3((m + 3) / 4), dim3(256), sizeof(double) * (size_t)m, t->stream,
+                           c->d, c->leak_q);
+        hipLaunchKernelGGL(th::charge_finish_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d,
+                           (const double *)c->leak_q);
+    } else {
+        hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
+    }
+    TH_HIP(hipGetLastError());
+    return TH_OK;
+}
+
 int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_params *p, th_batch_diag *out) {
     if (!c || !p || !out || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
     th_tracer *t = c->t;
@@ -1081,8 +1106,7 @@ int th_charging_step(th_charging *c, int64_t offset, int64_t n, const th_trace_p
     memset(out, 0, sizeof *out);
     int64_t traced = offset;
     if ((rc = launch_trace(t, offset, n, p, nullptr, false, false, &traced, &out->tally))) return rc;
-    hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
-    TH_HIP(hipGetLastError());
+    if ((rc = launch_update(c))) return rc;
     if ((rc = launch_field(c))) return rc;
     if (c->mon_out) {
         hipLaunchKernelGGL(th::monitor_kernel, dim3(1), dim3(256), 0, t->stream, c->d, c->mon);
@@ -1202,19 +1226,6 @@ constexpr int kRunRows = 1024;   // batches per th_charging_run call at most
 
 int run_row(const th_charging *c) { return c->mon.n_probes + c->mon.n_regions + 3 + 5 + th::kDiagCount; }
 
-int zero_u32(th_tracer *t, uint32_t *p, int64_t n) {
-    hipLaunchKernelGGL(th::zero_u32_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024))),
-                       dim3(256), 0, t->stream, p, n);
-    TH_HIP(hipGetLastError());
-    return TH_OK;
-}
-int zero_u64(th_tracer *t, unsigned long long *p, int64_t n) {
-    hipLaunchKernelGGL(th::zero_u64_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024))),
-                       dim3(256), 0, t->stream, p, n);
-    TH_HIP(hipGetLastError());
-    return TH_OK;
-}
-
 // One batch as kernels only (capturable): injection at the device batch
 // index, tally clears by fills, the push passes, deposit / leak / conductor,
 // potential and field, monitors, and the batch's record.  The same kernels
@@ -1227,9 +1238,11 @@ int charging_batch_kernels(th_charging *c, int64_t offset, int64_t n, const th_t
     hipLaunchKernelGGL(th::inject_at_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, t->stream, inj,
                        c->d_run);
     TH_HIP(hipGetLastError());
-    if ((rc = zero_u32(t, t->d_counts, 2 * (int64_t)std::max(t->m, 1))) ||
-        (rc = zero_u32(t, t->d_overflow, 2 * (int64_t)t->N)) || (rc = zero_u64(t, t->d_scal, th::kScalCount)))
-        return rc;
+    // tallies, counters, the passes' work counters and survivor counts: one fill
+    hipLaunchKernelGGL(th::clear_batch_kernel, dim3(64), dim3(256), 0, t->stream, t->d_counts,
+                       2 * (int64_t)std::max(t->m, 1), t->d_overflow, 2 * (int64_t)t->N, t->d_scal, (int64_t)kScalAll,
+                       t->d_surv_count, (int64_t)(kMaxPasses + 1));
+    TH_HIP(hipGetLastError());
     t->blocks_per_cu = p->blocks_per_cu;
     th::Grid g = make_grid(t, p);
     th::Work w;
@@ -1250,7 +1263,6 @@ int charging_batch_kernels(th_charging *c, int64_t offset, int64_t n, const th_t
     w.begin = offset;
     w.end = offset + n;
     const bool passes = p->pass_steps > 0 && p->pass_steps < p->max_steps;
-    if ((rc = zero_u32(t, t->d_surv_count, kMaxPasses + 1))) return rc;
     int64_t cap = passes ? p->pass_steps : p->max_steps;
     for (int pass = 0;; ++pass) {
         const bool last = cap >= p->max_steps || pass == kMaxPasses - 1;
@@ -1259,7 +1271,7 @@ int charging_batch_kernels(th_charging *c, int64_t offset, int64_t n, const th_t
         w.in_count = pass ? t->d_surv_count + pass - 1 : nullptr;
         w.out = last ? nullptr : t->d_surv[pass & 1];
         w.out_count = last ? nullptr : t->d_surv_count + pass;
-        if ((rc = zero_u64(t, t->d_scal, 1))) return rc;   // work counter
+        w.queue = t->d_scal + th::kScalCount + pass;   // cleared above
         const bool tile = p->tile_pass > 0 && pass + 1 >= p->tile_pass && t->tile_w > 0 && p->precision == 0;
         if ((rc = tile ? launch_tile(t, g, w, p->barrier != 0, p->tapered != 0)
                        : launch_push(t, g, w, p->barrier != 0, p->tapered != 0, p->precision, false, false)))
@@ -1267,8 +1279,7 @@ int charging_batch_kernels(th_charging *c, int64_t offset, int64_t n, const th_t
         if (last) break;
         cap = std::min<int64_t>(2 * cap, p->max_steps);
     }
-    hipLaunchKernelGGL(th::charge_update_kernel, dim3(1), dim3(th::kUpdBlock), c->update_lds, t->stream, c->d);
-    TH_HIP(hipGetLastError());
+    if ((rc = launch_update(c))) return rc;
     if ((rc = launch_field(c))) return rc;
     const int n_mon = c->mon_out ? c->mon.n_probes + c->mon.n_regions + 3 : 0;
     if (c->mon_out) {

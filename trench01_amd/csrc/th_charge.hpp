@@ -91,13 +91,12 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
     return scratch[32];
 }
 
-// Deposit, leak, conductor and column charges of one batch over the m-vector
-// (one workgroup).
-__device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, double *scratch) {
-    double *q_old = sh, *q_new = sh + d.m;
+// Deposit of one batch's tallies; with a leak operator also the leak's input
+// q_new = rho dx^2 (+ dt G phi_ext) and q_old = rho dx^2 (the region outflow
+// terms start from it).  q_old / q_new: LDS or global, m each.
+__device__ __forceinline__ void charge_deposit(const ChargeDev &d, double *q_old, double *q_new) {
     const double dx2 = d.dx * d.dx;
     const int m = d.m;
-    // deposit
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
         double r = d.rho[k];
         d.rho_prev[k] = r;
@@ -105,33 +104,45 @@ __device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, do
         if (d.cond_s[k]) drho = 0.0;
         r += drho;
         d.rho[k] = r;
-    }
-    __syncthreads();
-    // leak (backward Euler over the surface conductance network)
-    double leak_dq = 0.0;
-    if (d.has_leak) {
-        for (int k = threadIdx.x; k < m; k += blockDim.x) {
-            double q = d.rho[k] * dx2;
+        if (d.has_leak) {
+            const double q = r * dx2;
             q_old[k] = q;
             q_new[k] = d.leak_has_c ? q + d.leak_c[k] : q;
         }
-        __syncthreads();
-        // one wave per row of M: lanes read the row coalesced, partial sums
-        // combined by a fixed shuffle tree
-        const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-        for (int k = threadIdx.x >> 6; k < m; k += waves) {
-            const double *row = d.leak_M + (int64_t)k * m;
-            double acc = 0.0;
-            for (int j = lane; j < m; j += 64) acc += row[j] * q_new[j];
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
-            if (lane == 0) {
-                d.rho[k] = acc / dx2;
-                q_old[k] = d.region[k] ? q_old[k] - acc : 0.0;   // region outflow terms
-            }
+    }
+}
+
+// Leak (backward Euler over the surface conductance network) for rows k =
+// first, first + step, ...: one wave per row of M, lanes read the row
+// coalesced, partial sums combined by a fixed shuffle tree.  Row k's region
+// outflow term replaces q_old[k].  q_new: all m entries, read only.
+__device__ __forceinline__ void leak_rows(const ChargeDev &d, const double *q_new, double *q_old, int first,
+                                          int step) {
+    const double dx2 = d.dx * d.dx;
+    const int m = d.m, lane = threadIdx.x & 63;
+    for (int k = first; k < m; k += step) {
+        const double *row = d.leak_M + (int64_t)k * m;
+        double acc = 0.0;
+        for (int j = lane; j < m; j += 64) acc += row[j] * q_new[j];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+        if (lane == 0) {
+            d.rho[k] = acc / dx2;
+            q_old[k] = d.region[k] ? q_old[k] - acc : 0.0;   // region outflow terms
         }
-        __syncthreads();
+    }
+}
+
+// The rest of the update after the leak (one workgroup): the leak monitor
+// (sum of the region outflow terms in q_old, global or LDS), a driven
+// conductor, the floating conductor's step, and the charge per Green's
+// column.  sh: LDS for 2m doubles; scratch: 40 doubles of LDS.
+__device__ __forceinline__ void charge_finish(const ChargeDev &d, const double *outflow, double *sh, double *scratch) {
+    const double dx2 = d.dx * d.dx;
+    const int m = d.m;
+    double leak_dq = 0.0;
+    if (d.has_leak) {
         double part = 0.0;
-        for (int k = threadIdx.x; k < m; k += blockDim.x) part += q_old[k];
+        for (int k = threadIdx.x; k < m; k += blockDim.x) part += outflow[k];
         leak_dq = block_sum(part, scratch);
     }
     if (d.driven)
@@ -162,18 +173,51 @@ __device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, do
     if (threadIdx.x == 0) d.diag[kDiagLeakDQ] = leak_dq;
     // charge per Green's column: thread c sums its surface cells in index order
     // from LDS copies of rho and the column map
-    int *col_s = reinterpret_cast<int *>(q_new);
+    double *rho_s = sh;
+    int *col_s = reinterpret_cast<int *>(sh + m);
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        q_old[k] = d.rho[k];
+        rho_s[k] = d.rho[k];
         col_s[k] = d.col_of[k];
     }
     __syncthreads();
     for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) {
         double acc = 0.0;
         for (int k = 0; k < m; ++k)
-            if (col_s[k] == c) acc += q_old[k] * dx2;
+            if (col_s[k] == c) acc += rho_s[k] * dx2;
         d.qcol[c] = acc;
     }
+}
+
+// Deposit, leak, conductor and column charges of one batch over the m-vector
+// (one workgroup).
+__device__ __forceinline__ void charge_update(const ChargeDev &d, double *sh, double *scratch) {
+    double *q_old = sh, *q_new = sh + d.m;
+    charge_deposit(d, q_old, q_new);
+    __syncthreads();
+    if (d.has_leak) {
+        leak_rows(d, q_new, q_old, threadIdx.x >> 6, blockDim.x >> 6);
+        __syncthreads();
+    }
+    charge_finish(d, q_old, sh, scratch);
+}
+
+// The same update in three launches when there is a leak operator: the m x m
+// GEMV streams M (2.2 MB at 520 surface cells) with one wave per row over many
+// workgroups instead of one workgroup's latency-bound reads.  q: global
+// scratch of 2m doubles (q_old, q_new).
+__global__ void __launch_bounds__(kUpdBlock) charge_deposit_kernel(ChargeDev d, double *q) {
+    charge_deposit(d, q, q + d.m);
+}
+__global__ void __launch_bounds__(256) leak_rows_kernel(ChargeDev d, double *q) {
+    extern __shared__ double qn[];
+    for (int k = threadIdx.x; k < d.m; k += blockDim.x) qn[k] = q[d.m + k];
+    __syncthreads();
+    leak_rows(d, qn, q, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
+}
+__global__ void __launch_bounds__(kUpdBlock) charge_finish_kernel(ChargeDev d, const double *q) {
+    extern __shared__ double sh[];
+    __shared__ double scratch[40];
+    charge_finish(d, q, sh, scratch);
 }
 
 __global__ void __launch_bounds__(kUpdBlock) charge_update_kernel(ChargeDev d) {

@@ -151,6 +151,7 @@ struct Work {
     uint32_t *counts;                    // [2][m] per-surface-cell tallies
     uint32_t *overflow;                  // [2][nx*ny] tallies for non-surface cells
     unsigned long long *scal;            // kScalCount counters
+    unsigned long long *queue;           // this launch's work counter (zero at launch)
     int max_steps, refill;
     int order, prio_steps, check_steps;
     Record rec;                          // record mode only
@@ -793,7 +794,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
         if (idle != 0ull && (__popcll(idle) >= w.refill || busy == 0ull)) {
             int leader = __ffsll((long long)idle) - 1;
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&w.scal[kScalQueue], (unsigned long long)__popcll(idle));
+            if (lane == leader) base = atomicAdd(w.queue, (unsigned long long)__popcll(idle));
             base = __shfl(base, leader);
             if (pid == -1) {
                 int64_t q = (int64_t)(base + (unsigned long long)__popcll(idle & below));
