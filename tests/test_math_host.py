@@ -62,6 +62,18 @@ def test_push_range_operands(host_hypot):
         assert same(host_hypot(a, b), trace_ref.glibc_hypot(a, b))
 
 
+def test_widely_separated_operands_in_the_core_range(host_hypot):
+    """ay <= ax 2^-54 inside the core's exponent range goes through the core (no ratio test): it must return
+    ax, as glibc's widely-separated branch does."""
+    rng = np.random.default_rng(2)
+    n = 1 << 20
+    ax = np.ldexp(rng.random(n) + 0.5, rng.integers(-330, 511, n))
+    ay = np.ldexp(ax * (rng.random(n) + 0.5), -rng.integers(54, 60, n))
+    ay = np.where(ay >= 2.0 ** -383, ay, 2.0 ** -383)
+    for a, b in ((ax, ay), (ay, -ax), (-ax, ay)):
+        assert same(host_hypot(a, b), trace_ref.glibc_hypot(a, b))
+
+
 def test_special_values(host_hypot):
     vals = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, 2.0 ** -459, 2.0 ** -460, 2.0 ** 511,
                      2.0 ** 512, 1.7976931348623157e308, np.inf, -np.inf, np.nan, 1.0, 3.0, 4.0,

@@ -286,7 +286,7 @@ int th_tracer_create(const uint8_t *vac, int32_t nx, int32_t ny, int32_t device,
     if (!out) return fail(TH_EINVAL, "out is NULL");
     *out = nullptr;
     if (!vac || nx < 1 || ny < 1) return fail(TH_EINVAL, "invalid vacuum map %dx%d", nx, ny);
-    if ((int64_t)nx * ny > (1 << 30)) return fail(TH_EINVAL, "grid too large");
+    if ((int64_t)nx * ny > (1 << 30) || nx >= (1 << 24) || ny >= (1 << 24)) return fail(TH_EINVAL, "grid too large");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
     if (device < 0 || device >= count)

@@ -141,19 +141,24 @@ TH_HD double th_hypot_core_n(double ax, double ay) {
     const double d2 = h - ax;
     const double t12 = 2.0 * d2 * (ax - 2.0 * ay);
     const double t22 = (4.0 * d2 - ay) * ay + d2 * d2;
-    const bool first = h <= 2.0 * ay;
+    const int first = h <= 2.0 * ay;
     const double t1 = first ? t11 : t12, t2 = first ? t21 : t22;
     return h - th_div_n(t1 + t2, 2.0 * h);
 }
 
 // The same function with the common case first: operands in the core's range
-// (no rescaling, not widely separated, finite, non-zero) go straight to the
-// core, which is what th_hypot_full computes for them (its scalings are then
-// by 1.0).  NaN fails every comparison and takes the full path.
+// (no rescaling, finite, non-zero) go straight to the core.  For those that
+// are not widely separated this is what th_hypot_full computes (its scalings
+// are then by 1.0).  For widely separated ones (ay <= ax 2^-54, where
+// th_hypot_full returns ax + ay, i.e. ax, since ay < ulp(ax) / 2) the core
+// returns ax too: RN(ax^2) + RN(ay^2) rounds to RN(ax^2), the correctly
+// rounded square root of RN(ax^2) is ax (binary floating point with
+// round-to-nearest), d = 0, t1 + t2 = -RN(ay^2), and ax + RN(ay^2) / (2 ax)
+// rounds to ax.  NaN fails every comparison and takes the full path.
 TH_HD double th_hypot(double x, double y) {
     double fx = fabs(x), fy = fabs(y);
     double ax = fx < fy ? fy : fx, ay = fx < fy ? fx : fy;
-    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-383 && ay > ax * 0x1p-54, 1))
+    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-383, 1))
         return th_hypot_core_n(ax, ay);
     return th_hypot_full(x, y);
 }

@@ -394,7 +394,8 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         }
         if (!staged) {
             // one 64-byte corner record and one flag byte per step
-            const int key = (i + 1) * (ny + 1) + (j + 1);
+            // 24-bit multiply (grids are under 2^24 cells per side)
+            const int key = (int)__umul24((unsigned)(i + 1), (unsigned)(ny + 1)) + (j + 1);
             th_dvec4 a, b;
             // two 32-byte vector loads (kept whole: element-wise double2 reads get split into 8-byte loads)
             const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(key), 64);
@@ -492,7 +493,7 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         // key 0's record and marks it unusable).
         const int ni = (int)floor(s.q - 0.5), nj = (int)floor(s.r - 0.5);
         const bool inside = (unsigned)(ni + 1) <= (unsigned)nx && (unsigned)(nj + 1) <= (unsigned)ny;
-        const int nkey = inside ? (ni + 1) * (ny + 1) + (nj + 1) : 0;
+        const int nkey = inside ? (int)__umul24((unsigned)(ni + 1), (unsigned)(ny + 1)) + (nj + 1) : 0;
         const th_dvec4 *c = (const th_dvec4 *)__builtin_assume_aligned(g.corner(nkey), 64);
         cc->a = c[0];
         cc->b = c[1];
