@@ -154,11 +154,11 @@ TH_HD double th_hypot_core_n(double ax, double ay) {
 // returns ax too: RN(ax^2) + RN(ay^2) rounds to RN(ax^2), the correctly
 // rounded square root of RN(ax^2) is ax (binary floating point with
 // round-to-nearest), d = 0, t1 + t2 = -RN(ay^2), and ax + RN(ay^2) / (2 ax)
-// rounds to ax.  NaN fails every comparison and takes the full path.
+// rounds to ax.
 TH_HD double th_hypot(double x, double y) {
-    double fx = fabs(x), fy = fabs(y);
-    double ax = fx < fy ? fy : fx, ay = fx < fy ? fx : fy;
-    if (__builtin_expect(ax <= 0x1p+511 && ay >= 0x1p-383, 1))
+    const double fx = fabs(x), fy = fabs(y);
+    const double ax = fmax(fx, fy), ay = fmin(fx, fy);   // NaN operands take the full path
+    if (__builtin_expect(!__builtin_isunordered(x, y) && ax <= 0x1p+511 && ay >= 0x1p-383, 1))
         return th_hypot_core_n(ax, ay);
     return th_hypot_full(x, y);
 }

@@ -369,7 +369,7 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     double fx = s.q - 0.5, fy = s.r - 0.5;
     double fi = floor(fx), fj = floor(fy);
     int i = (int)fi, j = (int)fj;
-    double tx = fx - (double)i, ty = fy - (double)j;
+    double tx = fx - fi, ty = fy - fj;   // (double)i == fi for every position the grid can hold
     double w00 = (1.0 - tx) * (1.0 - ty), w10 = tx * (1.0 - ty), w01 = (1.0 - tx) * ty, w11 = tx * ty;
     double2 e00, e10, e01, e11;
     double v00, v10, v01, v11;              // vacuum weights 1.0 / 0.0
