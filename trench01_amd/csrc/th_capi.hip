@@ -787,7 +787,7 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
 struct th_charging {
     th_tracer *t = nullptr;
     th::ChargeDev d{};
-    int32_t *col_of = nullptr, *rows = nullptr, *surf = nullptr;
+    int32_t *col_of = nullptr, *rows = nullptr, *surf = nullptr, *col_ptr = nullptr, *col_cells = nullptr;
     double *green = nullptr, *external = nullptr, *leak_M = nullptr, *leak_c = nullptr, *g_cond_s = nullptr;
     double *leak_q = nullptr;   // [2m] the leak's input and outflow terms between the update's launches
     uint8_t *region = nullptr, *cond_s = nullptr;
@@ -888,7 +888,8 @@ void th_charging_destroy(th_charging *c) {
     if (!c) return;
     (void)hipSetDevice(c->t->device);
     (void)hipStreamSynchronize(c->t->stream);
-    for (void *p : {(void *)c->col_of, (void *)c->rows, (void *)c->surf, (void *)c->green, (void *)c->external,
+    for (void *p : {(void *)c->col_of, (void *)c->col_ptr, (void *)c->col_cells, (void *)c->rows, (void *)c->surf,
+                    (void *)c->green, (void *)c->external,
                     (void *)c->leak_M, (void *)c->leak_c, (void *)c->g_cond_s, (void *)c->region,
                     (void *)c->cond_s, (void *)c->rho, (void *)c->rho_prev, (void *)c->qcol, (void *)c->phi,
                     (void *)c->diag, (void *)c->g_hi, (void *)c->g_lo, (void *)c->rep_counts, (void *)c->rep_scal,
@@ -926,7 +927,20 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
     auto bad = [&](int code) { th_charging_destroy(c); return code; };
     hipStream_t st = t->stream;
     int rc;
+    // surface cells grouped by Green's column (ascending within a column)
+    std::vector<int32_t> col_ptr((size_t)s->n_cols + 1, 0), col_cells;
+    for (int k = 0; k < m; ++k)
+        if (s->col_of[k] >= 0) ++col_ptr[(size_t)s->col_of[k] + 1];
+    for (int q = 0; q < s->n_cols; ++q) col_ptr[(size_t)q + 1] += col_ptr[q];
+    col_cells.resize((size_t)std::max(col_ptr[s->n_cols], 1));
+    {
+        std::vector<int32_t> fill(col_ptr.begin(), col_ptr.end() - 1);
+        for (int k = 0; k < m; ++k)
+            if (s->col_of[k] >= 0) col_cells[(size_t)fill[s->col_of[k]]++] = k;
+    }
     if ((rc = upload_new(&c->col_of, s->col_of, m, st)) || (rc = upload_new(&c->rows, s->rows, s->n_rows, st)) ||
+        (rc = upload_new(&c->col_ptr, col_ptr.data(), col_ptr.size(), st)) ||
+        (rc = upload_new(&c->col_cells, col_cells.data(), col_cells.size(), st)) ||
         (rc = upload_new(&c->surf, t->surf.data(), m, st)) ||
         (rc = upload_new(&c->green, s->green, (size_t)s->n_rows * s->n_cols, st)) ||
         (rc = upload_new(&c->external, s->external, t->N, st)) ||
@@ -947,7 +961,7 @@ int th_charging_create(th_tracer *t, const th_charging_setup *s, th_charging **o
     if (hipEventCreate(&c->ev2) != hipSuccess) return bad(fail(TH_EHIP, "hipEventCreate failed"));
     th::ChargeDev &d = c->d;
     d.m = m; d.n_cols = s->n_cols; d.n_rows = s->n_rows; d.nx = t->nx; d.ny = t->ny;
-    d.col_of = c->col_of; d.rows = c->rows; d.surf = c->surf; d.green = c->green; d.external = c->external;
+    d.col_of = c->col_of; d.col_ptr = c->col_ptr; d.col_cells = c->col_cells; d.rows = c->rows; d.surf = c->surf; d.green = c->green; d.external = c->external;
     d.leak_M = c->leak_M; d.leak_c = c->leak_c; d.region = c->region; d.cond_s = c->cond_s;
     d.g_cond_s = c->g_cond_s; d.vac = t->d_vac; d.wface = t->d_wface;
     d.rho = c->rho; d.rho_prev = c->rho_prev; d.qcol = c->qcol; d.phi = c->phi; d.diag = c->diag;

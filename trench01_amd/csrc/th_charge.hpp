@@ -27,6 +27,8 @@ constexpr int kUpdBlock = 1024;   // the m-vector update runs in one workgroup
 struct ChargeDev {
     int m, n_cols, n_rows, nx, ny;
     const int32_t *col_of;      // [m] surface index -> Green's column, -1: none
+    const int32_t *col_ptr;     // [n_cols + 1] column c's surface cells are col_cells[col_ptr[c] .. col_ptr[c+1])
+    const int32_t *col_cells;   // [<= m] surface indices grouped by column, ascending within a column
     const int32_t *rows;        // [n_rows] flat cell of each kept row
     const int32_t *surf;        // [m] flat cell of each surface index
     const double *green;        // [n_rows][n_cols]
@@ -171,20 +173,18 @@ __device__ __forceinline__ void charge_finish(const ChargeDev &d, const double *
         __syncthreads();
     }
     if (threadIdx.x == 0) d.diag[kDiagLeakDQ] = leak_dq;
-    // charge per Green's column: thread c sums its surface cells in index order
-    // from LDS copies of rho and the column map
+    // charge per Green's column, one wave per column: lanes stride the
+    // column's surface cells (one for a dielectric cell, all of the floating
+    // conductor's for its column), partial sums combined by a fixed shuffle tree
     double *rho_s = sh;
-    int *col_s = reinterpret_cast<int *>(sh + m);
-    for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        rho_s[k] = d.rho[k];
-        col_s[k] = d.col_of[k];
-    }
+    for (int k = threadIdx.x; k < m; k += blockDim.x) rho_s[k] = d.rho[k];
     __syncthreads();
-    for (int c = threadIdx.x; c < d.n_cols; c += blockDim.x) {
+    const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    for (int c = threadIdx.x >> 6; c < d.n_cols; c += waves) {
         double acc = 0.0;
-        for (int k = 0; k < m; ++k)
-            if (col_s[k] == c) acc += rho_s[k] * dx2;
-        d.qcol[c] = acc;
+        for (int q = d.col_ptr[c] + lane; q < d.col_ptr[c + 1]; q += 64) acc += rho_s[d.col_cells[q]] * dx2;
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+        if (lane == 0) d.qcol[c] = acc;
     }
 }
 
