@@ -253,7 +253,7 @@ typedef double th_dvec4 __attribute__((ext_vector_type(4)));
 // a reflection sends back, uses it when the key matches and loads otherwise.
 struct CornerCache {
     int key = -1;
-    unsigned flags = 0u;
+    uint8_t flags = 0;
     th_dvec4 a, b;
 };
 
