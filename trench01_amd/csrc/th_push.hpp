@@ -786,9 +786,8 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     if constexpr (TANGENT) tn.dfield = w.tan.dfield;
     CornerCache cache;
     // Steps this lane traced: each particle adds its step count when it ends
-    // here and subtracts the count it arrived with, modulo 2^32 (a lane traces
-    // far fewer than 2^32 steps in one launch).
-    unsigned my_steps = 0u;
+    // here and subtracts the count it arrived with (64-bit: exact).
+    unsigned long long my_steps = 0ull;
     for (;;) {
         unsigned long long idle = __ballot(pid == -1);
         unsigned long long busy = __ballot(pid >= 0);
@@ -806,7 +805,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         pid = v.pid;
                         s.x = (Real)v.x; s.y = (Real)v.y; s.vx = (Real)v.vx; s.vy = (Real)v.vy;
                         k = v.k;
-                        my_steps -= (unsigned)k;
+                        my_steps -= (unsigned long long)k;
                         if constexpr (RF) {
                             rc.clock = v.clock;
                             rc.elapsed = v.elapsed;
@@ -906,7 +905,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         w.tan.face[pid] = (int8_t)tn.face;
                         w.tan.steps[pid] = k;
                     }
-                    my_steps += (unsigned)k;
+                    my_steps += (unsigned long long)k;
                     pid = -1;
                     break;
                 }
@@ -921,14 +920,14 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                     v.pid = pid;
                     v.k = k;
                     static_cast<Surv *>(w.out)[slot_out] = v;
-                    my_steps += (unsigned)k;
+                    my_steps += (unsigned long long)k;
                     pid = -1;
                     break;
                 }
             }
         }
     }
-    atomicAdd(&lds_scal[kScalSteps], (unsigned long long)my_steps);
+    atomicAdd(&lds_scal[kScalSteps], my_steps);
     __syncthreads();
     for (int c = threadIdx.x; c < 2 * g.m; c += kBlock)
         if (lds_counts[c]) atomicAdd(&w.counts[c], lds_counts[c]);
