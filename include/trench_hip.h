@@ -160,6 +160,46 @@ int th_trace_tangent(th_tracer *t, const double *x, const double *y, const doubl
                      int32_t *steps, int8_t *face, double *endpoints, double *d_endpoints, double *final_velocity,
                      double *d_final_velocity);
 
+/* One deposit of the relaxed-barrier push: weight a (tangent da) of particle
+ * pid (pool index) absorbed in cell `cell` across an x-face (face 1) or a
+ * y-face (face 0) at the crossing point (x, y), tangent (dx, dy); kind 0: the
+ * absorbed share of a relaxed barrier decision, 1: the final absorption of
+ * the particle's remaining weight; species 0 ion, 1 electron. */
+typedef struct th_soft_event {
+    double x, y, dx, dy, a, da;
+    int32_t pid, cell;
+    int8_t face, kind, species, pad0;
+    int32_t pad1;
+} th_soft_event;
+
+/* Relaxed-barrier push with its forward-mode derivative, over the resident
+ * pool[offset, offset+n) and the fields last set (barrier wall model, FP64).
+ * The reference's barrier test (trench_charging_2d.py:623-640,
+ * trench_trace.cu:128-143) reflects a particle when vn^2/2 < q/m dphi; here it
+ * reflects with probability p = 1 / (1 + exp(t)), t = (vn^2/2 - q/m dphi) /
+ * (|q/m| width) (width in volts), by splitting: the share 1 - p of the
+ * particle's weight is deposited where it hit and the share p goes on along
+ * the reflected path.  For |t| >= cut the sharp decision is taken (the
+ * relaxed model's weights there differ from 0 or 1 by less than exp(-cut)).
+ * Every deposit is face-smoothed (the hit cell and its neighbour along the
+ * crossed face share it linearly in the crossing point).  deposit[2][m] and
+ * d_deposit[2][m] (species-major, surface-cell order of th_tracer_surface)
+ * receive the weighted deposits and their derivative along the direction
+ * given by the initial-state tangents dx0..dvy0 (n each, nullable), the field
+ * tangent dEx, dEy and the potential tangent dphi (nx*ny each, nullable).
+ * events (nullable, event_capacity entries) lists each deposit; *n_events
+ * (nullable) is the number produced, which may exceed the capacity (the
+ * excess is not stored).  status / hit / steps (n each) are nullable.  The
+ * path events other than the relaxed decisions (cells, walls, step counts)
+ * are held fixed, as in th_trace_tangent.  No reference counterpart: the
+ * relaxation makes the barrier decision differentiable, for Jacobian-vector
+ * products of the charging map in a Newton-Krylov steady solve. */
+int th_trace_soft(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p, double width, double cut,
+                  const double *dx0, const double *dy0, const double *dvx0, const double *dvy0, const double *dEx,
+                  const double *dEy, const double *dphi, th_tally *tally, double *deposit, double *d_deposit,
+                  int8_t *status, int32_t *hit, int32_t *steps, th_soft_event *events, int64_t event_capacity,
+                  int64_t *n_events);
+
 /* Adjoint (reverse-mode, pathwise) push: the transpose of th_trace_tangent's
  * derivative.  For n particles traced through the fields last set, given
  * the cotangent of each recorded end point bar_end[n][2] and of each final

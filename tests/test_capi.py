@@ -40,7 +40,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     import subprocess
     structs = {"th_trace_params": _lib.TraceParams, "th_tally": _lib.Tally,
                "th_charging_setup": _lib.ChargingSetupC, "th_batch_diag": _lib.BatchDiag,
-               "th_inject_params": _lib.InjectParams}
+               "th_inject_params": _lib.InjectParams, "th_soft_event": _lib.SoftEvent}
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

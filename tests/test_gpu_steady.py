@@ -73,25 +73,35 @@ def test_replicas_in_one_push_equal_separate_batches():
     dev.close()
 
 
-def test_newton_krylov_reaches_the_time_marched_state():
-    """Newton-Krylov on the batch map (GMRES, difference JVPs over merged
-    replicas, from the coarse Anderson start) followed by plain batches to
-    the reference criterion: it converges, within 1 V of the time-marched
-    bottom-centre potential.  Its speed and accuracy are measured in the
-    bench, not asserted: the difference JVPs carry too much sampling noise
-    for its Newton steps to descend, so its plain phase starts below the
-    fixed point and, like time-marching from zero charge, passes the
-    criterion while still approaching it."""
-    print("[gpu case] steady AR8 (1.25e5 per species) newton-krylov", flush=True)
+@pytest.fixture(scope="module")
+def time_marched():
+    """Plain time-marching at 1.25e5 particles per species to the reference
+    criterion, and the mean bottom-centre potential of 300 more batches."""
     p, _ = configs.product_params("AR8", n_per_batch=125_000)
     p = replace(p, until_steady=True, max_batches=4000)
     plain = charging.run(p, log=None, injection="device")
     nb = plain["timing"]["batches"]
     more = charging.run(replace(p, until_steady=False, n_batches=300), log=None, injection="device", resume=plain)
-    long_run = float(np.mean(more["hist"]["bottom center"][nb:]))
-    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1)
+    return p, nb, float(np.mean(more["hist"]["bottom center"][nb:]))
+
+
+@pytest.mark.parametrize("jvp", ["difference", "relaxed"])
+def test_newton_krylov_reaches_the_time_marched_state(time_marched, jvp):
+    """Newton-Krylov on the batch map (GMRES from the coarse Anderson start)
+    followed by plain batches to the reference criterion: it converges,
+    within 1 V of the time-marched bottom-centre potential, with
+    Jacobian-vector products by differences over merged replicas or from the
+    relaxed-barrier push (relaxed.MapJacobian).  Speed and accuracy are
+    measured in the bench, not asserted: the difference JVPs carry too much
+    sampling noise for their Newton steps to descend; the relaxed ones
+    descend, with a plain phase whose length varies between seeds."""
+    print(f"[gpu case] steady AR8 (1.25e5 per species) newton-krylov, {jvp} JVPs", flush=True)
+    p, nb, long_run = time_marched
+    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1 if jvp == "difference" else 2, jvp=jvp)
     v = nk["steady"]["bottom center"]
     print(f"plain {nb} batches, 300 on {long_run:.3f} V; newton-krylov {nk['batches']} batches {nk['phases']}, "
           f"{v:.3f} V", flush=True)
     assert nk["converged"]
     assert abs(v - long_run) < 1.0
+    if jvp == "relaxed":
+        assert nk["phases"][1]["relaxed_jvps"] > 0

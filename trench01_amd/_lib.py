@@ -57,6 +57,14 @@ class InjectParams(ctypes.Structure):
                 ("v_per_sqrt_ev", ctypes.c_double), ("s_e", ctypes.c_double)]
 
 
+class SoftEvent(ctypes.Structure):
+    """th_soft_event."""
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("a", ctypes.c_double), ("da", ctypes.c_double), ("pid", ctypes.c_int32), ("cell", ctypes.c_int32),
+                ("face", ctypes.c_int8), ("kind", ctypes.c_int8), ("species", ctypes.c_int8), ("pad0", ctypes.c_int8),
+                ("pad1", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); every symbol include/trench_hip.h declares.
 SIGNATURES = {
     "th_last_error": (ctypes.c_char_p, []),
@@ -81,6 +89,9 @@ SIGNATURES = {
     "th_trace_tangent": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                         dp, dp, dp, dp, dp, dp, ctypes.POINTER(Tally), i8p, i32p, i32p, i8p, dp, dp,
                                         dp, dp]),
+    "th_trace_soft": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), ctypes.c_double,
+                                     ctypes.c_double, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(Tally), dp, dp, i8p,
+                                     i32p, i32p, ctypes.POINTER(SoftEvent), ctypes.c_int64, i64p]),
     "th_trace_adjoint": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                         i32p, dp, dp, dp, dp, dp]),
     "th_particles_upload_times": (ctypes.c_int, [vp, ctypes.c_int64, dp, ctypes.c_int64]),

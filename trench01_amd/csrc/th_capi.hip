@@ -226,6 +226,13 @@ int launch_kernel(th_tracer *t, K kernel, const th::Grid &g, const th::Work &w, 
 template <bool B, bool T>
 int launch_variant(th_tracer *t, const th::Grid &g, const th::Work &w, int64_t want, size_t lds, int precision,
                    bool record, bool rf) {
+    if (w.tan.soft.dep) {   // relaxed-barrier push: barrier walls only, deposits in LDS after the counts
+        if (!B) return fail(TH_EINVAL, "the relaxed-barrier push needs the barrier wall model");
+        const size_t soft_lds = th::tile_offset(t->m) + sizeof(double) * 4 * (size_t)t->m;
+        auto kernel = th::push64_soft_kernel<T>;
+        TH_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)soft_lds));
+        return launch_kernel(t, kernel, g, w, want, soft_lds);
+    }
     if (w.tan.dend) return launch_kernel(t, th::push64_tangent_kernel<B, T>, g, w, want, lds);
     if (rf)
         return record ? launch_kernel(t, th::push64_rf_record_kernel<B, T>, g, w, want, lds)
@@ -1942,6 +1949,126 @@ extern "C" int th_trace_tangent(th_tracer *t, const double *x, const double *y, 
     TH_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     t->push_ms = ms;
     fill_tally(scal, n - traced, tally);
+    return TH_OK;
+}
+
+static_assert(sizeof(th_soft_event) == sizeof(th::SoftEvent) && offsetof(th_soft_event, da) == offsetof(th::SoftEvent, da) &&
+                  offsetof(th_soft_event, pid) == offsetof(th::SoftEvent, pid) &&
+                  offsetof(th_soft_event, face) == offsetof(th::SoftEvent, face) &&
+                  offsetof(th_soft_event, species) == offsetof(th::SoftEvent, sp),
+              "th_soft_event mirrors th::SoftEvent");
+
+extern "C" int th_trace_soft(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p, double width,
+                             double cut, const double *dx0, const double *dy0, const double *dvx0, const double *dvy0,
+                             const double *dEx, const double *dEy, const double *dphi, th_tally *tally,
+                             double *deposit, double *d_deposit, int8_t *status, int32_t *hit, int32_t *steps,
+                             th_soft_event *events, int64_t event_capacity, int64_t *n_events) {
+    if (!t || !p || !tally || !deposit || !d_deposit || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    if ((dEx == nullptr) != (dEy == nullptr)) return fail(TH_EINVAL, "dEx and dEy go together");
+    if (!(width > 0.0) || !(cut > 0.0) || !std::isfinite(width) || !std::isfinite(cut))
+        return fail(TH_EINVAL, "width and cut must be positive and finite");
+    if (event_capacity < 0 || event_capacity > (int64_t)0xffffffffu || (event_capacity > 0 && !events))
+        return fail(TH_EINVAL, "invalid event buffer");
+    if (!p->barrier) return fail(TH_EINVAL, "the relaxed-barrier push needs the barrier wall model");
+    if (p->precision != 0 || p->rf) return fail(TH_EINVAL, "the relaxed-barrier push follows the FP64 push without RF");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    memset(tally, 0, sizeof *tally);
+    const size_t m2 = 2 * (size_t)t->m;
+    memset(deposit, 0, sizeof(double) * m2);
+    memset(d_deposit, 0, sizeof(double) * m2);
+    if (n_events) *n_events = 0;
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    // device buffers: initial tangents (pool-indexed), field and potential tangents, deposits, events
+    const size_t span = (size_t)(offset + n), nn = (size_t)n;
+    double *d_init = nullptr, *d_dphi = nullptr, *d_dep = nullptr;
+    double2 *d_field = nullptr;
+    int32_t *d_steps = nullptr;
+    th::SoftEvent *d_ev = nullptr;
+    unsigned *d_nev = nullptr;
+    auto release = [&]() {
+        for (void *q : {(void *)d_init, (void *)d_dphi, (void *)d_dep, (void *)d_field, (void *)d_steps, (void *)d_ev,
+                        (void *)d_nev})
+            if (q) (void)hipFree(q);
+    };
+    const double *h_init[4] = {dx0, dy0, dvx0, dvy0};
+    const bool any_init = dx0 || dy0 || dvx0 || dvy0;
+    if ((any_init && (rc = dev_alloc(&d_init, 4 * span))) || (dEx && (rc = dev_alloc(&d_field, (size_t)t->N))) ||
+        (dphi && (rc = dev_alloc(&d_dphi, (size_t)t->N))) || (rc = dev_alloc(&d_dep, 2 * m2)) ||
+        (steps && (rc = dev_alloc(&d_steps, span))) ||
+        (event_capacity > 0 && (rc = dev_alloc(&d_ev, (size_t)event_capacity))) || (rc = dev_alloc(&d_nev, 1))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(d_dep, 0, sizeof(double) * 2 * m2, t->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_nev, 0, sizeof(unsigned), t->stream);
+    for (int c = 0; c < 4 && e == hipSuccess; ++c)
+        if (h_init[c])
+            e = hipMemcpyAsync(d_init + c * span + offset, h_init[c], sizeof(double) * nn, hipMemcpyHostToDevice,
+                               t->stream);
+    std::vector<double2> field;
+    if (dEx && e == hipSuccess) {   // vacuum-masked, as the push's own records
+        field.resize((size_t)t->N);
+        for (int c = 0; c < t->N; ++c)
+            field[c] = t->vac[c] ? make_double2(dEx[c], dEy[c]) : make_double2(0.0, 0.0);
+        e = hipMemcpyAsync(d_field, field.data(), sizeof(double2) * field.size(), hipMemcpyHostToDevice, t->stream);
+    }
+    if (dphi && e == hipSuccess)
+        e = hipMemcpyAsync(d_dphi, dphi, sizeof(double) * (size_t)t->N, hipMemcpyHostToDevice, t->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(t->stream);
+        release();
+        return fail(TH_EHIP, "relaxed push upload: %s", hipGetErrorString(e));
+    }
+    th::Record rec{};
+    th::TangentIO tan{};
+    for (int c = 0; c < 4; ++c) {
+        const double *q = h_init[c] ? d_init + c * span : nullptr;
+        if (c == 0) tan.dx0 = q; else if (c == 1) tan.dy0 = q; else if (c == 2) tan.dvx0 = q; else tan.dvy0 = q;
+    }
+    tan.dfield = d_field;
+    tan.steps = d_steps;
+    tan.soft.dphi = d_dphi;
+    tan.soft.inv_width = 1.0 / width;
+    tan.soft.cut = cut;
+    tan.soft.dep = d_dep;
+    tan.soft.ddep = d_dep + m2;
+    tan.soft.events = d_ev;
+    tan.soft.n_events = d_nev;
+    tan.soft.ev_cap = (unsigned)event_capacity;
+    int64_t traced = 0;
+    rc = launch_trace(t, offset, n, p, nullptr, status != nullptr, hit != nullptr, &traced, tally, &rec, &tan);
+    if (rc) {
+        (void)hipStreamSynchronize(t->stream);
+        release();
+        return rc;
+    }
+    unsigned long long scal[th::kScalCount];
+    unsigned nev = 0;
+    e = hipMemcpyAsync(scal, t->d_scal, sizeof scal, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&nev, d_nev, sizeof nev, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(deposit, d_dep, sizeof(double) * m2, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_deposit, d_dep + m2, sizeof(double) * m2, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && status)
+        e = hipMemcpyAsync(status, t->d_status + offset, nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && hit)
+        e = hipMemcpyAsync(hit, t->d_hit + offset, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && steps)
+        e = hipMemcpyAsync(steps, d_steps + offset, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e == hipSuccess && events && nev > 0)
+        e = hipMemcpy(events, d_ev, sizeof(th::SoftEvent) * std::min<size_t>(nev, (size_t)event_capacity),
+                      hipMemcpyDeviceToHost);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "relaxed push readback: %s", hipGetErrorString(e));
+    if (n_events) *n_events = (int64_t)nev;
+    float ms = 0.0f;
+    TH_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    t->push_ms = ms;
+    fill_tally(scal, offset + n - traced, tally);
     return TH_OK;
 }
 

@@ -133,12 +133,41 @@ struct SurvivorRF : Survivor {
 // initial-state tangents per particle and field tangent per cell (each null =
 // zero), and per particle the tangent of the recorded end point and of the
 // final velocity, the face crossed at the end, and the step count.
+// One deposit of the relaxed-barrier push (push64_soft_kernel): weight a
+// (tangent da) absorbed in cell `cell` across an x-face (face 1) or a y-face
+// (face 0) at the crossing point (x, y) with tangent (dx, dy).  kind 0: the
+// absorbed share of a relaxed barrier decision; 1: the particle's final
+// absorption (its remaining weight).
+struct SoftEvent {
+    double x, y, dx, dy, a, da;
+    int32_t pid, cell;
+    int8_t face, kind, sp, pad0;
+    int32_t pad1;
+};
+
+// Relaxed-barrier mode: at a barrier test the particle reflects with
+// probability p = 1 / (1 + exp(t)), t = (vn^2 / 2 - q/m dphi) / (|q/m| width)
+// (the sharp test reflects for t < 0), depositing the share 1 - p of its
+// weight where it would have been absorbed and going on reflected with the
+// share p.  For |t| >= cut the decision is the sharp one.  Deposits are
+// face-smoothed and accumulated per species and surface cell with their
+// tangents; each is also listed in `events` when that is non-null.
+struct SoftIO {
+    const double *dphi;       // [nx*ny] potential tangent per cell, or null
+    double inv_width, cut;    // 1 / width [1/V], |t| beyond which the decision is sharp
+    double *dep, *ddep;       // [2][m] accumulated (null: not the soft push)
+    SoftEvent *events;        // [ev_cap] or null
+    unsigned *n_events;       // events produced (may exceed ev_cap: the excess is not stored)
+    unsigned ev_cap;
+};
+
 struct TangentIO {
     const double *dx0, *dy0, *dvx0, *dvy0;   // [n] or null
     const double2 *dfield;                   // [nx*ny] d{Ex*vac, Ey*vac} or null
-    double *dend, *dvel;                     // [n][2]
+    double *dend, *dvel;                     // [n][2] (soft push: may be null)
     int8_t *face;                            // [n] 1 x-face, 0 y-face, -1 none
     int32_t *steps;                          // [n]
+    SoftIO soft;                             // relaxed-barrier push only
 };
 
 struct Work {
@@ -354,6 +383,119 @@ struct Tangent64 {
     }
 };
 
+// Face-smoothed deposit of weight a (tangent da) absorbed in cell c across an
+// x-face (xf) or a y-face at the crossing point (xq, yq), tangent (dxq, dyq):
+// the neighbour along the face on the side of the crossing point's coordinate
+// u along it (u in [0, 1] in cell units) takes |u - 1/2| of the weight when it
+// is a surface cell with vacuum in front of it on the particle's side, the
+// hit cell the rest; the share moves with u, so its tangent is sign(u - 1/2)
+// du (zero when u was clipped to the face's end).  Accumulated into dep /
+// ddep ([2][m], species sp).  Off-surface cells receive nothing.
+template <class V>
+__device__ __forceinline__ void soft_face_deposit(const V &g, double *dep, double *ddep, int sp, int c, bool xf,
+                                                  double xq, double yq, double dxq, double dyq, double a,
+                                                  double da) {
+    const int k = g.smap[c];
+    if (k < 0) return;
+    const int nx = g.nx, ny = g.ny, cx = c / ny, cy = c - (c / ny) * ny;
+    const double ex = xq / g.dx, ey = yq / g.dx;
+    double along = xf ? ey - (double)cy : py_mod(ex - (double)cx, (double)nx);
+    const double across = xf ? py_mod(ex - (double)cx, (double)nx) : ey - (double)cy;
+    const bool moving = along > 0.0 && along < 1.0;   // a clipped crossing point stays put
+    along = fmin(fmax(along, 0.0), 1.0);
+    const int side = py_mod(rint(across), (double)nx) == 1.0 ? 1 : -1;   // from the cell after (+1) or before
+    const int step = along >= 0.5 ? 1 : -1;
+    const int nbx = xf ? cx : ((cx + step) % nx + nx) % nx, nby = xf ? cy + step : cy;
+    const int fx = xf ? ((nbx + side) % nx + nx) % nx : nbx, fy = xf ? nby : nby + side;
+    bool valid = nby >= 0 && nby < ny && fy >= 0 && fy < ny;
+    int knb = k;
+    if (valid) {
+        const int nb = nbx * ny + nby;
+        valid = !g.is_vac(nb) && g.is_vac(fx * ny + fy) && g.smap[nb] >= 0;
+        if (valid) knb = g.smap[nb];
+    }
+    const double w_nb = valid ? fabs(along - 0.5) : 0.0;
+    const double sgn = along > 0.5 ? 1.0 : (along < 0.5 ? -1.0 : 0.0);
+    const double slope = valid && moving ? sgn / g.dx : 0.0;
+    const double dw = slope * (xf ? dyq : dxq);
+    const int m = g.m;
+    atomicAdd(&dep[sp * m + k], a * (1.0 - w_nb));
+    atomicAdd(&ddep[sp * m + k], da * (1.0 - w_nb) - a * dw);
+    if (valid) {
+        atomicAdd(&dep[sp * m + knb], a * w_nb);
+        atomicAdd(&ddep[sp * m + knb], da * w_nb + a * dw);
+    }
+}
+
+// Tangent of the relaxed-barrier push (SoftIO): Tangent64's path derivative
+// plus the particle's weight W and its tangent.  At a barrier test with
+// |t| < cut the particle deposits W (1 - p) where it hit and goes on
+// reflected with W p; the tangent of p = 1 / (1 + exp(t)) is -p (1 - p) dt,
+// dt = (vn dvn - q/m wface (dphi_c - dphi_v)) / (|q/m| width), vn the normal
+// velocity at the step's end and dphi the potential tangent (held fixed
+// between cut and infinity, where the sharp decision is taken).
+struct SoftTangent : Tangent64 {
+    const double *__restrict__ dphi = nullptr;
+    double inv_width = 0.0, cut = 0.0;
+    double W = 1.0, dW = 0.0;
+    bool pend = false;           // this step's barrier decision was relaxed
+    double pp = 0.0, dpp = 0.0;  // its reflection probability and tangent
+    int sp = 0, pid = 0;
+    double *dep = nullptr, *ddep = nullptr;   // LDS accumulators [2][m]
+    SoftEvent *events = nullptr;
+    unsigned *n_events = nullptr;
+    unsigned ev_cap = 0;
+
+    // have = vn^2 / 2, need = q/m wface (phi_c - phi_v): the sharp test
+    // reflects when have < need.
+    __device__ __forceinline__ bool reflects(double have, double need, double vn, double qm, double wf, int c, int v,
+                                             bool face_x) {
+        pend = false;
+        const double aq = fabs(qm);
+        const double t = (have - need) / aq * inv_width;
+        if (!(t > -cut && t < cut)) return have < need;
+        const double p = 1.0 / (1.0 + exp(t));
+        const double dvn = face_x ? dvx : dvy;
+        const double dneed = dphi ? qm * wf * (dphi[c] - dphi[v]) : 0.0;
+        const double dt = (vn * dvn - dneed) / aq * inv_width;
+        pp = p;
+        dpp = -p * (1.0 - p) * dt;
+        pend = true;
+        return true;   // the share p goes on reflected
+    }
+    // Deposit of weight a at the recorded crossing point (dpx, dpy set by crossing()).
+    template <class V>
+    __device__ __forceinline__ void deposit(const V &g, int c, bool xf, double xq, double yq, double a, double da,
+                                            int kind) {
+        if (events) {
+            const unsigned slot = atomicAdd(n_events, 1u);
+            if (slot < ev_cap) {
+                SoftEvent e;
+                e.x = xq; e.y = yq; e.dx = dpx; e.dy = dpy; e.a = a; e.da = da;
+                e.pid = pid; e.cell = c; e.face = xf ? 1 : 0; e.kind = (int8_t)kind; e.sp = (int8_t)sp;
+                e.pad0 = 0; e.pad1 = 0;
+                events[slot] = e;
+            }
+        }
+        soft_face_deposit(g, dep, ddep, sp, c, xf, xq, yq, dpx, dpy, a, da);
+    }
+    // The relaxed decision's absorbed share, deposited at the crossing point
+    // of the hit face; the weight goes on with the reflected path.
+    template <class V>
+    __device__ __forceinline__ void relaxed(const V &g, int c, bool xf, double xq, double yq) {
+        deposit(g, c, xf, xq, yq, W * (1.0 - pp), dW * (1.0 - pp) - W * dpp, 0);
+        dW = dW * pp + W * dpp;
+        W = W * pp;
+        pend = false;
+        dpx = dx;
+        dpy = dy;
+        face = -1;
+    }
+};
+
+template <class T> struct is_soft_tangent : std::false_type {};
+template <> struct is_soft_tangent<SoftTangent> : std::true_type {};
+
 // One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
 // (escaped).  With RECORD, (xp, yp) is the step's recorded point.  With RF the
 // voltage wave acts during the step and rc advances.  V reads one instance's
@@ -555,7 +697,12 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
                 double2 pc = g.rfphi[c], pv = g.rfphi[v];
                 dphi = dphi + g.wface[c] * (cs * (pc.x - pv.x) + cm * (pc.y - pv.y));
             }
-            if (0.5 * vn * vn < s.qm * dphi) {   // cannot climb the wall barrier: reflect
+            bool reflect;
+            if constexpr (is_soft_tangent<T>::value)
+                reflect = tan.reflects(0.5 * vn * vn, s.qm * dphi, vn, s.qm, g.face(c), c, v, face_x);
+            else
+                reflect = 0.5 * vn * vn < s.qm * dphi;
+            if (reflect) {   // cannot climb the wall barrier: reflect
                 if (face_x) s.vx = -s.vx; else s.vy = -s.vy;
                 s.x = x0; s.y = y0; s.q = q0; s.r = r0;
                 hit = false;
@@ -570,8 +717,8 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         yp = s.y;
         double tf = 1.0;   // fraction of the step to the recorded point
         tan.point();
-        if (hit) {
-            double raw;
+        // the point where the step crosses the hit cell's face
+        auto face_point = [&](double &raw, double &frac, double &xq, double &yq) {
             if (face_x) {
                 double xb = (double)(cx + (sx < 0.0)) * g.dx;
                 xb += nearbyint((x0 - xb) / g.len_x()) * g.len_x();
@@ -579,11 +726,23 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
             } else {
                 raw = ((double)(cy + (sy < 0.0)) * g.dx - y0) / sy;
             }
-            double frac = fmin(fmax(raw, 0.0), 1.0);
+            frac = fmin(fmax(raw, 0.0), 1.0);
+            xq = wrap_pos(x0 + frac * sx, g.len_x());
+            yq = y0 + frac * sy;
+        };
+        if (hit) {
+            double raw, frac;
+            face_point(raw, frac, xp, yp);
             tf = frac;
-            xp = wrap_pos(x0 + frac * sx, g.len_x());
-            yp = y0 + frac * sy;
             tan.crossing(face_x, raw, frac, sx, sy);
+        }
+        if constexpr (is_soft_tangent<T>::value) {
+            if (tan.pend) {   // the relaxed decision's absorbed share, at the face it would have crossed
+                double raw, frac, xq, yq;
+                face_point(raw, frac, xq, yq);
+                tan.crossing(face_x, raw, frac, sx, sy);
+                tan.relaxed(g, cx * ny + cy, face_x, xq, yq);
+            }
         }
         if (escaped) {
             double raw = (g.len_y() - y0) / sy;
@@ -725,7 +884,10 @@ __device__ __forceinline__ void finish(const Grid &g, const Work &w, uint32_t *l
     }
 }
 
-enum : int { kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4, kModeF64Tangent = 5 };
+enum : int {
+    kModeF64 = 0, kModeF64Record = 1, kModeF32 = 2, kModeF64Rf = 3, kModeF64RfRecord = 4, kModeF64Tangent = 5,
+    kModeF64Soft = 6
+};
 
 // LDS bytes of the tile push beyond the tallies: the tallies [2][m] (u32)
 // rounded up to 16 bytes, then tile_w * ny double2 and their vacuum bits.
@@ -739,15 +901,22 @@ __host__ __device__ __forceinline__ size_t tile_bytes(int w, int ny) {
 template <int MODE, bool BARRIER, bool TAPERED, bool TILE = false>
 __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     using Real = typename std::conditional<MODE == kModeF32, float, double>::type;
-    constexpr bool TANGENT = MODE == kModeF64Tangent;
+    constexpr bool SOFT = MODE == kModeF64Soft;
+    constexpr bool TANGENT = MODE == kModeF64Tangent || SOFT;
     constexpr bool RECORD = MODE == kModeF64Record || MODE == kModeF64RfRecord || TANGENT;
     constexpr bool RF = MODE == kModeF64Rf || MODE == kModeF64RfRecord;
     static_assert(!TILE || MODE == kModeF64, "the tile push is the plain FP64 push");
-    using Tan = typename std::conditional<TANGENT, Tangent64, NoTangent>::type;
+    using Tan = typename std::conditional<SOFT, SoftTangent,
+                                          typename std::conditional<TANGENT, Tangent64, NoTangent>::type>::type;
     using Surv = typename std::conditional<RF, SurvivorRF, Survivor>::type;
-    extern __shared__ uint32_t lds_counts[];   // [2][m] (then the tile, TILE only)
+    extern __shared__ uint32_t lds_counts[];   // [2][m] (then the tile, TILE only; the soft deposits, SOFT only)
     __shared__ unsigned long long lds_scal[kScalCount];
     for (int k = threadIdx.x; k < 2 * g.m; k += kBlock) lds_counts[k] = 0u;
+    double *lds_dep = nullptr;   // SOFT: deposits [2][m], then their tangents [2][m]
+    if constexpr (SOFT) {
+        lds_dep = reinterpret_cast<double *>(reinterpret_cast<char *>(lds_counts) + tile_offset(g.m));
+        for (int k = threadIdx.x; k < 4 * g.m; k += kBlock) lds_dep[k] = 0.0;
+    }
     if (threadIdx.x < kScalCount) lds_scal[threadIdx.x] = 0ull;
     TileView tview;
     if constexpr (TILE) {
@@ -784,6 +953,16 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     RfClock rc{0.0, 0.0, 0.0, 0.0};
     Tan tn;
     if constexpr (TANGENT) tn.dfield = w.tan.dfield;
+    if constexpr (SOFT) {
+        tn.dphi = w.tan.soft.dphi;
+        tn.inv_width = w.tan.soft.inv_width;
+        tn.cut = w.tan.soft.cut;
+        tn.dep = lds_dep;
+        tn.ddep = lds_dep + 2 * g.m;
+        tn.events = w.tan.soft.events;
+        tn.n_events = w.tan.soft.n_events;
+        tn.ev_cap = w.tan.soft.ev_cap;
+    }
     CornerCache cache;
     // Steps this lane traced: each particle adds its step count when it ends
     // here and subtracts the count it arrived with (64-bit: exact).
@@ -836,6 +1015,13 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                         tn.dpx = tn.dx;
                         tn.dpy = tn.dy;
                         tn.face = -1;
+                        if constexpr (SOFT) {
+                            tn.W = 1.0;
+                            tn.dW = 0.0;
+                            tn.pend = false;
+                            tn.sp = sp;
+                            tn.pid = pid;
+                        }
                     }
                     if constexpr (RECORD) {
                         xp = s.x;
@@ -886,7 +1072,7 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                 ++k;
                 if (st != 0 || k >= w.max_steps) {
                     finish(g, w, lds_counts, lds_scal, pid, sp, st != 0 ? st : 3, cell);
-                    if constexpr (RECORD) {
+                    if (RECORD && (!SOFT || w.rec.end)) {
                         w.rec.end[2 * pid] = xp;
                         w.rec.end[2 * pid + 1] = yp;
                         w.rec.vel[2 * pid] = s.vx;
@@ -897,13 +1083,18 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
                             w.rec.clock[pid] = rc.ck;
                         }
                     }
+                    if constexpr (SOFT) {   // the remaining weight where the path ends on a wall
+                        if (st == 1) tn.deposit(g, cell, tn.face == 1, xp, yp, tn.W, tn.dW, 1);
+                    }
                     if constexpr (TANGENT) {
-                        w.tan.dend[2 * pid] = tn.dpx;
-                        w.tan.dend[2 * pid + 1] = tn.dpy;
-                        w.tan.dvel[2 * pid] = tn.dvx;
-                        w.tan.dvel[2 * pid + 1] = tn.dvy;
-                        w.tan.face[pid] = (int8_t)tn.face;
-                        w.tan.steps[pid] = k;
+                        if (!SOFT || w.tan.dend) {
+                            w.tan.dend[2 * pid] = tn.dpx;
+                            w.tan.dend[2 * pid + 1] = tn.dpy;
+                            w.tan.dvel[2 * pid] = tn.dvx;
+                            w.tan.dvel[2 * pid + 1] = tn.dvy;
+                            w.tan.face[pid] = (int8_t)tn.face;
+                        }
+                        if (!SOFT || w.tan.steps) w.tan.steps[pid] = k;
                     }
                     my_steps += (unsigned long long)k;
                     pid = -1;
@@ -931,6 +1122,12 @@ __device__ __forceinline__ void push_body(const Grid &g, const Work &w) {
     __syncthreads();
     for (int c = threadIdx.x; c < 2 * g.m; c += kBlock)
         if (lds_counts[c]) atomicAdd(&w.counts[c], lds_counts[c]);
+    if constexpr (SOFT) {
+        for (int c = threadIdx.x; c < 2 * g.m; c += kBlock) {
+            if (lds_dep[c] != 0.0) unsafeAtomicAdd(&w.tan.soft.dep[c], lds_dep[c]);
+            if (lds_dep[2 * g.m + c] != 0.0) unsafeAtomicAdd(&w.tan.soft.ddep[c], lds_dep[2 * g.m + c]);
+        }
+    }
     if (threadIdx.x >= kScalEsc0 && threadIdx.x < kScalCount && lds_scal[threadIdx.x])
         atomicAdd(&w.scal[threadIdx.x], lds_scal[threadIdx.x]);
 }
@@ -970,6 +1167,13 @@ __global__ void __launch_bounds__(kBlock) push64_rf_record_kernel(Grid g, Work w
 template <bool BARRIER, bool TAPERED>
 __global__ void __launch_bounds__(kBlock) push64_tangent_kernel(Grid g, Work w) {
     push_body<kModeF64Tangent, BARRIER, TAPERED>(g, w);
+}
+
+// FP64 record push with a relaxed barrier decision and the forward-mode
+// derivative of the face-smoothed weighted deposit (single pass; SoftIO).
+template <bool TAPERED>
+__global__ void __launch_bounds__(kBlock) push64_soft_kernel(Grid g, Work w) {
+    push_body<kModeF64Soft, true, TAPERED>(g, w);
 }
 
 // FP32 push.

@@ -354,12 +354,13 @@ def gmres(matvec, b, krylov):
     return V[:j].T @ y, k
 
 
-def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, log=None):
+def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, log=None, jvp=None):
     """Jacobian-free Newton-Krylov on F(x) = g(x) - x: every Newton step
     solves (J_g - I) dx = -F(x) by ``krylov`` GMRES steps whose Jacobian-vector
     products are forward differences of g with step ``h`` along the unit
     direction (J_g v ~ |v| (g(x + h v/|v|) - g(x)) / h; g sees the same
-    particles at every evaluation: common random numbers).  A step that does
+    particles at every evaluation: common random numbers), or ``jvp(x, v)``
+    (J_g v at the last evaluated point x) when given.  A step that does
     not lower max |F| is halved up to ``backtrack`` times (then the plain step
     x <- g(x) is taken).  Stops when max |F| < tol or after ``max_newton``
     steps.  ``g(x)`` returns (state, image).  Returns (state at the last
@@ -375,6 +376,8 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
         f = gx - x
 
         def matvec(v, x=x, gx=gx):
+            if jvp is not None:
+                return jvp(x, v) - v
             nrm = float(np.linalg.norm(v))
             if nrm == 0.0:
                 return np.zeros_like(v)
@@ -382,7 +385,8 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
             return (gp - gx) * (nrm / h) - v
 
         dx, k = gmres(matvec, -f, krylov)
-        evals += k
+        if jvp is None:   # each difference JVP is one evaluation of g
+            evals += k
         lam = 1.0
         for _ in range(backtrack + 1):
             xt = x + lam * dx
@@ -401,14 +405,22 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
 
 
 def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
-                  setup=None):
-    """Steady state by Jacobian-free Newton-Krylov on the batch map (GMRES,
-    ``krylov`` steps per Newton step, JVPs by forward differences of step
-    ``h_volts`` on the surface potentials, R replicas of K batches per map
-    evaluation with common random numbers, traced in one push for K = 1),
-    started from the Anderson stages ``start`` (default: the coarse first
-    default stage), then plain batches to the reference criterion.  Same
-    return dict as ``anderson``."""
+                  setup=None, jvp="difference", width=1.0, jvp_replicas=1, trajectory=True):
+    """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
+    steps per Newton step, R replicas of K batches per map evaluation with
+    common random numbers, traced in one push for K = 1), started from the
+    Anderson stages ``start`` (default: the coarse first default stage), then
+    plain batches to the reference criterion.  Jacobian-vector products:
+    ``jvp`` = "difference", forward differences of step ``h_volts`` on the
+    surface potentials; "relaxed", the derivative of the map with the barrier
+    decision relaxed to a logistic of ``width`` volts (relaxed.MapJacobian)
+    over the particles of the first ``jvp_replicas`` replicas of the last
+    evaluation (K = 1), with the paths' response to the field (``trajectory``)
+    or with the paths held fixed.  Same return dict as ``anderson``."""
+    if jvp not in ("difference", "relaxed"):
+        raise ValueError(f"jvp must be 'difference' or 'relaxed': {jvp!r}")
+    if jvp == "relaxed" and (K != 1 or not 1 <= jvp_replicas <= R):
+        raise ValueError("relaxed JVPs need K = 1 and 1 <= jvp_replicas <= R")
     log = log or (lambda s: None)
     start = default_stages(p)[:1] if start is None else start
     noise = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
@@ -422,23 +434,45 @@ def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, s
         rho_in = ss.to_rho(x, box["like"])
         rho_g, gx = ss.evaluate(rho_in, first, K, R)
         box["like"] = rho_g
+        box["x"], box["rho_in"] = x.copy(), rho_in
         return rho_g, gx
+
+    jac = None
+    if jvp == "relaxed":
+        from .relaxed import MapJacobian
+        jac = MapJacobian(ss.s, ss.dev, width, trajectory=trajectory)
+        n_pool = 2 * max(1, int(round(p.n_per_batch * ss.frac)))   # one replica's particles
+
+        def jvp_fn(x, v):
+            if not np.array_equal(x, box["x"]):
+                raise RuntimeError("relaxed JVP requested away from the last evaluated point")
+            if box.get("lin") is not box["rho_in"]:   # linearise once per point (one relaxed push)
+                jac.at(box["rho_in"], jvp_replicas * n_pool, ss.s.q_macro / ss.frac, jvp_replicas)
+                box["lin"] = box["rho_in"]
+            return jac(v)
+    else:
+        jvp_fn = None
 
     rho_g, x, evals = newton_krylov_iterate(
         g, x, tol, h_volts, krylov, max_newton,
-        log=lambda k, res, e: log(f"  [newton-krylov] step {k}: max |g(x) - x| = {res:.4g} V after {e} evaluations"))
+        log=lambda k, res, e: log(f"  [newton-krylov] step {k}: max |g(x) - x| = {res:.4g} V after {e} evaluations"),
+        jvp=jvp_fn)
     rho = ss.to_rho(x, rho_g)
     solver_batches = ss.traced
     done, b2, steady, hist, rho_s = ss.plain_phase(rho, log)
     wall = time.perf_counter() - ss.t_start
     ss.close()
-    return dict(method="Jacobian-free Newton-Krylov (GMRES) on the batch map (common random numbers, forward-"
-                       "difference JVPs over merged replicas) from a coarse Anderson start, then plain batches to the "
-                       "reference criterion",
+    how = ("forward-difference JVPs over merged replicas" if jvp == "difference" else
+           f"JVPs of the relaxed-barrier map (width {width} V) over {jvp_replicas} replica(s)")
+    return dict(method=f"Newton-Krylov (GMRES) on the batch map (common random numbers, {how}) from a coarse "
+                       "Anderson start, then plain batches to the reference criterion",
                 converged=done, batches=ss.traced,
                 phases=[dict(kind="secant", stages=[list(st) for st in start], evaluations_per_stage=evals_start,
                              batches=start_batches),
-                        dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov,
+                        dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov, jvp=jvp,
+                             width=width if jac else None, trajectory=trajectory if jac else None,
+                             relaxed_jvps=jac.jvps if jac else 0,
+                             relaxed_push_ms=jac.push_ms if jac else 0.0,
                              evaluations=evals, batches=solver_batches - start_batches),
                         dict(kind="plain", batches=b2, converged=done)],
                 steady=steady, hist=hist, rho_s=rho_s, wall_s=wall, setup_s=ss.setup_s)

@@ -200,11 +200,12 @@ def _face_geometry(vac, surface, spc, status, hit, face, endpoints, dx, weights=
     return a, sp, wt, smap[c], smap[np.where(valid, nb, c)], w_nb, slope, xf
 
 
-def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx, weights=None):
+def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, dx, weights=None, d_weights=None):
     """Face-smoothed deposit per species on the surface cells and its
     derivative: (dep, ddep), each [2, m] in particles (``surface`` = flat
     indices of the surface cells, ascending).  ``weights`` (per particle)
-    scale each particle's contribution to both."""
+    scale each particle's contribution to both; ``d_weights`` (per particle)
+    are their derivatives, which the derivative then includes."""
     m = surface.size
     dep, ddep = np.zeros((2, m)), np.zeros((2, m))
     a, sp, wt, k_hit, k_nb, w_nb, slope, xf = _face_geometry(vac, surface, spc, status, hit, face, endpoints, dx,
@@ -213,11 +214,24 @@ def face_deposit(vac, surface, spc, status, hit, face, endpoints, d_endpoints, d
         return dep, ddep
     de = d_endpoints[a]
     dw_nb = slope * np.where(xf, de[:, 1], de[:, 0])
+    dwt = np.zeros(a.size) if d_weights is None else np.asarray(d_weights, dtype=float)[a]
     for s in (0, 1):
         sel = sp == s
         dep[s] = np.bincount(k_hit[sel], wt[sel] * (1.0 - w_nb[sel]), m) + np.bincount(k_nb[sel], wt[sel] * w_nb[sel], m)
-        ddep[s] = np.bincount(k_hit[sel], -wt[sel] * dw_nb[sel], m) + np.bincount(k_nb[sel], wt[sel] * dw_nb[sel], m)
+        ddep[s] = (np.bincount(k_hit[sel], dwt[sel] * (1.0 - w_nb[sel]) - wt[sel] * dw_nb[sel], m) +
+                   np.bincount(k_nb[sel], dwt[sel] * w_nb[sel] + wt[sel] * dw_nb[sel], m))
     return dep, ddep
+
+
+def event_deposit(vac, surface, events, dx):
+    """face_deposit over the deposit list of the relaxed-barrier push
+    (HipTracer.soft events): each entry is one weighted absorption with its
+    weight tangent."""
+    ev = events
+    n = ev.size
+    return face_deposit(vac, surface, ev["species"], np.ones(n, np.int8), ev["cell"], ev["face"],
+                        np.column_stack((ev["x"], ev["y"])), np.column_stack((ev["dx"], ev["dy"])), dx,
+                        weights=ev["a"], d_weights=ev["da"])
 
 
 def face_deposit_vjp(vac, surface, spc, status, hit, face, endpoints, dx, gbar, weights=None):
@@ -437,6 +451,39 @@ class PathwiseBatch:
         ion = np.array([bar_state[:, 2] @ tan[k][0] + bar_state[:, 3] @ tan[k][1] for k in ("sigma", "energy")])
         return inc, grad_rho, ion
 
+    def relaxed(self, rho_s, width, drho_s=None, key=None, cut=30.0, sigma_deg=None, energy_shift=0.0, events=0,
+                trajectory=True):
+        """The increment with the relaxed barrier decision (relaxed.py,
+        th_trace_soft: reflection with probability 1 / (1 + exp(t)) of the
+        normal-energy margin t in units of ``width`` volts), and its derivative
+        along a surface-charge direction ``drho_s`` and/or an ion-parameter
+        direction ``key`` ("sigma" [per degree] or "energy" [per eV]).
+        ``trajectory`` False leaves the field tangent of the charge direction
+        out (paths held fixed, only the decisions respond).  Returns
+        (increment [m], derivative [m], the push's output dict)."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else rho_s
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        parts, tan = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        total = parts[0].size
+        t = self.tracer
+        t.reserve(total)
+        t.upload(0, *parts)
+        t.set_fields(ex, ey, (phi, self.fields.wface))
+        dfield = dphi = None
+        if drho_s is not None:
+            drho = np.zeros(self.geo.solid.size)
+            drho[self.surface] = drho_s
+            dphi, dex, dey = charge_direction(self.fields, self.geo.vac, self.p, drho.reshape(self.geo.solid.shape))
+            dfield = (dex, dey) if trajectory else None
+        dstate = None if key is None else (None, None, *tan[key])
+        r = t.soft(0, total, self.p, width, cut, dstate, dfield, dphi, events=events, per_particle=True)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        dinc = (r["d_deposit"][0] - r["d_deposit"][1]) * self.scale
+        inc[self.cond_s] = 0.0
+        dinc[self.cond_s] = 0.0
+        return inc, dinc, r
+
     def charge_jvp(self, rho_s, drho_s, sigma_deg=None, energy_shift=0.0):
         """(increment [m], d increment along the surface-charge direction drho_s [m])."""
         drho = np.zeros(self.geo.solid.size)
@@ -451,7 +498,7 @@ def _cos(a, b):
     return float(a @ b / (na * nb)) if na > 0 and nb > 0 else 0.0
 
 
-def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None, device=0):
+def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None, device=0, relaxed_width=None):
     """How well the pathwise derivative of the batch increment estimates the
     derivative of its expectation, per direction (ion angular sigma, energy
     shift, the surface charge's own direction rho_s), and the likelihood-ratio
@@ -460,7 +507,9 @@ def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None
     numbers, relative steps ``steps``) of the two streams, and between
     estimate and difference, with their norms.  The differences include the
     particles that change events; the pathwise derivative leaves them out,
-    the likelihood ratio does not."""
+    the likelihood ratio does not.  With ``relaxed_width`` (volts) also the
+    derivative of the relaxed-barrier increment ("*_relaxed"), against the
+    differences of the plain increment."""
     batches = [PathwiseBatch(p, seed=s, n=n, device=device) for s in seeds]
     sig = p.ion_angle_deg
     e_mean = p.ion_energy_eV if p.ion_distribution == "mono" else 0.5 * (p.ion_energy_lo_eV + p.ion_energy_hi_eV)
@@ -468,10 +517,26 @@ def derivative_agreement(p, rho_s, seeds=(7, 8), steps=(0.01, 0.05, 0.2), n=None
     names = ["sigma", "sigma_score", "energy", "charge"]
     if p.ion_distribution != "mono":
         names.insert(3, "energy_score")
+    if relaxed_width:
+        names += ["sigma_relaxed", "energy_relaxed", "charge_relaxed"]
     for name in names:
         pw, fds = [], []
         for b in batches:
-            if name == "sigma_score":
+            if name.endswith("_relaxed"):
+                base = name[:-len("_relaxed")]
+                if base == "charge":
+                    pw.append(b.relaxed(rho_s, relaxed_width, drho_s=rho_s)[1])
+                    f = lambda e: b.increment(rho_s * (1.0 + e))
+                    scale = 1.0
+                elif base == "sigma":
+                    pw.append(b.relaxed(rho_s, relaxed_width, key="sigma", sigma_deg=sig)[1])
+                    f = lambda e: b.increment(rho_s, sig * (1.0 + e))
+                    scale = sig
+                else:
+                    pw.append(b.relaxed(rho_s, relaxed_width, key="energy", sigma_deg=sig)[1])
+                    f = lambda e: b.increment(rho_s, sig, e * e_mean)
+                    scale = e_mean
+            elif name == "sigma_score":
                 pw.append(b.sigma_score_gradient(rho_s, sig, 0.0)[1])
                 f = lambda e: b.increment(rho_s, sig * (1.0 + e))
                 scale = sig

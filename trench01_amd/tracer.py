@@ -327,6 +327,57 @@ class HipTracer:
                    total_steps=int(tally.steps))
         return out
 
+    def soft(self, offset, n, p, width, cut=30.0, dstate=None, dfield=None, dphi=None, events=0, per_particle=False):
+        """Relaxed-barrier push of the resident pool[offset, offset+n) through
+        the fields last set (``set_fields`` with a barrier), with the forward
+        derivative along one direction (th_trace_soft): a barrier test
+        reflects with probability 1 / (1 + exp(t)), t = (vn^2/2 - q/m dphi) /
+        (|q/m| ``width`` [V]), sharp for |t| >= ``cut``; the absorbed shares
+        are face-smoothed deposits.  ``dstate`` = (dx, dy, dvx, dvy) initial
+        tangents (n each, entries may be None), ``dfield`` = (dEx, dEy) and
+        ``dphi`` [nx, ny] the field and potential tangents (None = 0).
+        Returns dict(deposit [2, m], d_deposit [2, m], escaped, lost,
+        total_steps, push_ms) plus status / hit / steps with
+        ``per_particle`` and the deposit list (numpy structured array of
+        th_soft_event) when ``events`` > 0 gives its capacity."""
+        L = _lib.load()
+        nx, ny = self.vac.shape
+        m = self.surface_cells().size
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        ds = [None] * 4 if dstate is None else [f64(v) for v in dstate]
+        if any(v is not None and v.shape != (n,) for v in ds):
+            raise ValueError("initial tangents must have one entry per particle")
+        dex, dey = (None, None) if dfield is None else (f64(v) for v in dfield)
+        dph = f64(dphi)
+        for a in (dex, dey, dph):
+            if a is not None and a.shape != (nx, ny):
+                raise ValueError("field and potential tangents must match the geometry")
+        prm = self.params_for(p, True)
+        prm.precision = 0
+        dep, ddep = np.empty((2, m)), np.empty((2, m))
+        pp = dict(status=np.empty(n, np.int8), hit=np.empty(n, np.int32),
+                  steps=np.empty(n, np.int32)) if per_particle else {}
+        ev = np.zeros(int(events), dtype=SOFT_EVENT) if events else None
+        n_ev = ctypes.c_int64(0)
+        tally = _lib.Tally()
+        _lib.check(L.th_trace_soft(self._h, int(offset), int(n), ctypes.byref(prm), float(width), float(cut),
+                                   *(_lib.ptr(v, _lib.dp) for v in ds), _lib.ptr(dex, _lib.dp),
+                                   _lib.ptr(dey, _lib.dp), _lib.ptr(dph, _lib.dp), ctypes.byref(tally),
+                                   _lib.ptr(dep, _lib.dp), _lib.ptr(ddep, _lib.dp),
+                                   _lib.ptr(pp.get("status"), _lib.i8p), _lib.ptr(pp.get("hit"), _lib.i32p),
+                                   _lib.ptr(pp.get("steps"), _lib.i32p), _lib.ptr(ev, ctypes.POINTER(_lib.SoftEvent)),
+                                   int(events), ctypes.byref(n_ev)))
+        ms = ctypes.c_double(0.0)
+        _lib.check(L.th_last_push_ms(self._h, ctypes.byref(ms)))
+        out = dict(deposit=dep, d_deposit=ddep, escaped=np.array([tally.escaped[0], tally.escaped[1]], np.int64),
+                   lost=int(tally.lost), total_steps=int(tally.steps), push_ms=float(ms.value),
+                   n_events=int(n_ev.value), **pp)
+        if events:
+            if n_ev.value > events:
+                raise RuntimeError(f"{n_ev.value} relaxed-push deposits exceed the event capacity {events}")
+            out["events"] = ev[:n_ev.value]
+        return out
+
     def adjoint(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, steps, bar_end=None, bar_vel=None, field=True):
         """Reverse-mode pathwise derivative (th_trace_adjoint): the transpose
         of ``tangent``.  ``steps`` are the particles' path lengths from a
@@ -370,6 +421,12 @@ class HipTracer:
         _lib.check(_lib.load().th_selftest_math(self._h, _lib.ptr(a, _lib.dp), _lib.ptr(b, _lib.dp), a.size,
                                                 _lib.ptr(out, _lib.dp)))
         return out
+
+
+# numpy layout of th_soft_event (64 bytes)
+SOFT_EVENT = np.dtype([("x", "f8"), ("y", "f8"), ("dx", "f8"), ("dy", "f8"), ("a", "f8"), ("da", "f8"),
+                       ("pid", "i4"), ("cell", "i4"), ("face", "i1"), ("kind", "i1"), ("species", "i1"),
+                       ("pad0", "i1"), ("pad1", "i4")])
 
 
 def make_particle_tracer(p, vac, log=None):
