@@ -145,9 +145,9 @@ class MapJacobian:
         drho' = derivative of the charge update,  du' = P dQ'(drho').
 
     ``dev`` is the DeviceCharging whose tracer holds the particles
-    (pool[0, n)) and whose charge sets the field; ``replicas`` batches of n /
-    replicas particles each are averaged.  ``q`` is the charge per particle
-    of those batches.  ``trajectory`` False leaves the field tangent out: the
+    (pool[offset, offset + n)) and whose charge sets the field; ``replicas``
+    batches of n / replicas particles each are averaged.  ``q`` is the charge
+    per particle of those batches.  ``trajectory`` False leaves the field tangent out: the
     paths are held fixed and only the relaxed decisions respond (through the
     potential step at the wall), which drops the path tangents' variance."""
 
@@ -177,14 +177,14 @@ class MapJacobian:
         np.add.at(q, self.idx, rho[self.has] * self.p.dx ** 2)
         return q
 
-    def at(self, rho, n, q, replicas=1):
-        """Linearise at charge ``rho`` over pool[0, n) (``replicas`` batches),
+    def at(self, rho, n, q, replicas=1, offset=0):
+        """Linearise at charge ``rho`` over pool[offset, offset + n) (``replicas`` batches),
         particles of charge ``q``: sets the device field and traces the
         relaxed deposit (its primal part is the same for every direction)."""
         self.dev.set_rho(rho)
         self.rho = np.asarray(rho, dtype=float).copy()
-        self.n, self.q, self.R = int(n), float(q), int(replicas)
-        r = self.dev.tracer.soft(0, self.n, self.p, self.width, self.cut)
+        self.n, self.q, self.R, self.offset = int(n), float(q), int(replicas), int(offset)
+        r = self.dev.tracer.soft(self.offset, self.n, self.p, self.width, self.cut)
         self.push_ms += r["push_ms"]
         dep = r["deposit"] / self.R
         _, self.point = self.update.apply(self.rho, dep[0], dep[1], self.q)
@@ -207,7 +207,7 @@ class MapJacobian:
     def __call__(self, du):
         """J du: the derivative of the map's surface potentials along du."""
         _, drho, dphi, dex, dey = self.directional(np.asarray(du, dtype=float))
-        r = self.dev.tracer.soft(0, self.n, self.p, self.width, self.cut,
+        r = self.dev.tracer.soft(self.offset, self.n, self.p, self.width, self.cut,
                                  dfield=(dex, dey) if self.trajectory else None, dphi=dphi)
         self.jvps += 1
         self.push_ms += r["push_ms"]

@@ -41,10 +41,13 @@ of the map itself:
   (window W, hold) passes; the reported steady values are their last-window
   means, as the plain method reports its own (:1342-1343).
 
-No pathwise adjoint of the push enters: deposit counts are integers, so their
-derivative with respect to the charge along a fixed particle path is zero
-almost everywhere; secant information over finite steps is what carries the
-response.
+No pathwise adjoint of the push enters the Anderson solve: deposit counts are
+integers, so their derivative with respect to the charge along a fixed
+particle path is zero almost everywhere; secant information over finite steps
+is what carries the response.  ``newton_krylov`` solves for the same fixed
+point with Newton steps whose GMRES Jacobian-vector products come from the
+push itself, with the barrier decision relaxed to a logistic
+(relaxed.MapJacobian), or from differences of the map.
 """
 import time
 
@@ -404,8 +407,8 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
     return state, gx, evals
 
 
-def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
-                  setup=None, jvp="difference", width=1.0, jvp_replicas=1, trajectory=True):
+def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
+                  setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
     steps per Newton step, R replicas of K batches per map evaluation with
     common random numbers, traced in one push for K = 1), started from the
@@ -414,13 +417,14 @@ def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, s
     ``jvp`` = "difference", forward differences of step ``h_volts`` on the
     surface potentials; "relaxed", the derivative of the map with the barrier
     decision relaxed to a logistic of ``width`` volts (relaxed.MapJacobian)
-    over the particles of the first ``jvp_replicas`` replicas of the last
-    evaluation (K = 1), with the paths' response to the field (``trajectory``)
+    over a batch of ``jvp_fraction`` times the particles (at 1 / jvp_fraction
+    the charge each; the same batch for every JVP: common random numbers),
+    with the paths' response to the field (``trajectory``)
     or with the paths held fixed.  Same return dict as ``anderson``."""
     if jvp not in ("difference", "relaxed"):
         raise ValueError(f"jvp must be 'difference' or 'relaxed': {jvp!r}")
-    if jvp == "relaxed" and (K != 1 or not 1 <= jvp_replicas <= R):
-        raise ValueError("relaxed JVPs need K = 1 and 1 <= jvp_replicas <= R")
+    if jvp == "relaxed" and not 0.0 < jvp_fraction <= 1.0:
+        raise ValueError("jvp_fraction must lie in (0, 1]")
     log = log or (lambda s: None)
     start = default_stages(p)[:1] if start is None else start
     noise = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
@@ -441,13 +445,16 @@ def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, s
     if jvp == "relaxed":
         from .relaxed import MapJacobian
         jac = MapJacobian(ss.s, ss.dev, width, trajectory=trajectory)
-        n_pool = 2 * max(1, int(round(p.n_per_batch * ss.frac)))   # one replica's particles
+        n_jvp = max(1, int(round(p.n_per_batch * jvp_fraction)))   # per species
+        jvp_offset = 2 * max(1, int(round(p.n_per_batch * ss.frac))) * R * K   # after the evaluations' particles
+        ss.dev.reserve(jvp_offset + 2 * n_jvp)
 
         def jvp_fn(x, v):
             if not np.array_equal(x, box["x"]):
                 raise RuntimeError("relaxed JVP requested away from the last evaluated point")
             if box.get("lin") is not box["rho_in"]:   # linearise once per point (one relaxed push)
-                jac.at(box["rho_in"], jvp_replicas * n_pool, ss.s.q_macro / ss.frac, jvp_replicas)
+                ss.dev.inject_device(jvp_offset, n_jvp, p.seed, (1 << 19) + first)   # the JVPs' own batch
+                jac.at(box["rho_in"], 2 * n_jvp, ss.s.q_macro * p.n_per_batch / n_jvp, offset=jvp_offset)
                 box["lin"] = box["rho_in"]
             return jac(v)
     else:
@@ -463,7 +470,7 @@ def newton_krylov(p, K=1, R=8, h_volts=2.0, krylov=30, max_newton=2, tol=None, s
     wall = time.perf_counter() - ss.t_start
     ss.close()
     how = ("forward-difference JVPs over merged replicas" if jvp == "difference" else
-           f"JVPs of the relaxed-barrier map (width {width} V) over {jvp_replicas} replica(s)")
+           f"JVPs of the relaxed-barrier map (width {width} V) over {jvp_fraction:g} of a batch")
     return dict(method=f"Newton-Krylov (GMRES) on the batch map (common random numbers, {how}) from a coarse "
                        "Anderson start, then plain batches to the reference criterion",
                 converged=done, batches=ss.traced,
