@@ -200,6 +200,24 @@ int th_trace_soft(th_tracer *t, int64_t offset, int64_t n, const th_trace_params
                   int8_t *status, int32_t *hit, int32_t *steps, th_soft_event *events, int64_t event_capacity,
                   int64_t *n_events);
 
+/* Adjoint of th_trace_soft: for the cotangent gbar[2][m] of the relaxed push's
+ * deposits (species-major, surface-cell order), the cotangent of every
+ * particle's initial state bar_state[n][4] (x, y, vx, vy), of the vacuum-masked
+ * field bar_field[nx*ny][2] (nullable) and of the potential bar_phi[nx*ny]
+ * (nullable; through the relaxed barrier decisions), summed over particles and
+ * steps, so that for any direction sum(gbar * d_deposit) = <bar_state, d
+ * state0> + <bar_field, dE> + <bar_phi, dphi>.  Same particles (pool[offset,
+ * offset+n)), fields, width and cut as the th_trace_soft call whose per-
+ * particle step counts are `steps` (a different re-traced length is an
+ * error).  Each path is re-traced once onto a tape in device memory (32
+ * bytes per step, in chunks of at most 4 GiB) and swept back; the weight
+ * before a relaxed decision is recovered from the weight after it.  *ms
+ * (nullable) receives the kernels' time.  No reference counterpart, as
+ * th_trace_soft. */
+int th_trace_soft_adjoint(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p, double width,
+                          double cut, const int32_t *steps, const double *gbar, double *bar_state, double *bar_field,
+                          double *bar_phi, double *ms);
+
 /* Adjoint (reverse-mode, pathwise) push: the transpose of th_trace_tangent's
  * derivative.  For n particles traced through the fields last set, given
  * the cotangent of each recorded end point bar_end[n][2] and of each final

@@ -215,3 +215,56 @@ def test_map_jacobian_directions_and_linearity():
         assert np.abs(ju - du).max() > 0
     finally:
         dev.close()
+
+
+def test_relaxed_adjoint_is_the_transpose_of_the_relaxed_tangent(charged):
+    """th_trace_soft_adjoint against th_trace_soft on the same particles and
+    fields: for random deposit cotangents g and random directions of the
+    initial states, the field and the potential, <g, d deposit> equals
+    <bar_state, d state> + <bar_field, d field> + <bar_phi, d phi>."""
+    _case("relaxed push: adjoint identity")
+    b, rho = charged
+    t, p = b.tracer, b.p
+    sampler = tangent.ion_sampler(p)
+    parts, _ = tangent.draw_batch(p, b.geo.ny, b.seed, sampler, b.n)
+    phi, ex, ey = b.field(rho)
+    n = parts[0].size
+    t.reserve(n)
+    t.upload(0, *parts)
+    t.set_fields(ex, ey, (phi, b.fields.wface))
+    rng = np.random.default_rng(12)
+    vac = b.geo.vac
+    ds = [rng.standard_normal(n) * s for s in (1e-10, 1e-10, 1e2, 1e2)]
+    dex, dey = (rng.standard_normal(vac.shape) * 1e5 * vac for _ in range(2))
+    dphi = rng.standard_normal(vac.shape) * 0.1
+    g = rng.standard_normal((2, b.surface.size))
+    for width in (1.0, 0.2):
+        r = t.soft(0, n, p, width, 30.0, ds, (dex, dey), dphi, per_particle=True)
+        bs, bf, bp = t.soft_adjoint(0, n, p, width, r["steps"], g)
+        lhs = float(np.sum(g * r["d_deposit"]))
+        parts_rhs = [sum(float(bs[:, c] @ ds[c]) for c in range(4)),
+                     float(np.sum(bf[..., 0] * dex + bf[..., 1] * dey)), float(np.sum(bp * dphi))]
+        rhs = sum(parts_rhs)
+        print(f"  width {width}: <g, J v> = {lhs:.9e}, <J^T g, v> = {rhs:.9e} (state, field, potential "
+              f"{parts_rhs[0]:.3e}, {parts_rhs[1]:.3e}, {parts_rhs[2]:.3e}); adjoint {t.last_adjoint_ms:.1f} ms",
+              flush=True)
+        assert all(abs(x) > 0 for x in parts_rhs)
+        assert rhs == pytest.approx(lhs, rel=1e-9)
+
+
+def test_relaxed_batch_gradient_is_the_transpose_of_its_jvps(charged):
+    """PathwiseBatch.relaxed_vjp (one relaxed push and its adjoint, the field
+    stencil's and G's transposes) against the relaxed JVPs along a surface-
+    charge direction and along the two ion parameters."""
+    _case("relaxed push: batch gradient")
+    b, rho = charged
+    rng = np.random.default_rng(13)
+    g = rng.standard_normal(b.surface.size)
+    v = np.where(b.cond_s, 0.0, rng.standard_normal(b.surface.size)) * 1e7
+    inc, grad, ion = b.relaxed_vjp(rho, g, 1.0)
+    inc1, jv, _ = b.relaxed(rho, 1.0, drho_s=v)
+    np.testing.assert_allclose(inc, inc1, rtol=0, atol=1e-13 * np.abs(inc1).max())   # FP64 atomics' order
+    assert float(grad @ v) == pytest.approx(float(g @ jv), rel=1e-9)
+    for k, key in enumerate(("sigma", "energy")):
+        _, jk, _ = b.relaxed(rho, 1.0, key=key)
+        assert ion[k] == pytest.approx(float(g @ jk), rel=1e-9)

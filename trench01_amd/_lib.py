@@ -92,6 +92,8 @@ SIGNATURES = {
     "th_trace_soft": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams), ctypes.c_double,
                                      ctypes.c_double, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(Tally), dp, dp, i8p,
                                      i32p, i32p, ctypes.POINTER(SoftEvent), ctypes.c_int64, i64p]),
+    "th_trace_soft_adjoint": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(TraceParams),
+                                             ctypes.c_double, ctypes.c_double, i32p, dp, dp, dp, dp, dp]),
     "th_trace_adjoint": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, i8p, ctypes.c_int64, ctypes.POINTER(TraceParams),
                                         i32p, dp, dp, dp, dp, dp]),
     "th_particles_upload_times": (ctypes.c_int, [vp, ctypes.c_int64, dp, ctypes.c_int64]),

@@ -118,16 +118,21 @@ struct FieldBar {
 };
 
 // Transpose of one step.  `out` is the cotangent of the state after the step;
-// on the recorded (last) step, `pend` is the cotangent of the recorded point
-// (else zero).  Returns the cotangent of the state before the step and adds
-// the field cotangent of the step's four corners to fb (then fbar, [N] {Ex,
-// Ey}, which may be null).
+// `pend` is the cotangent of the point the step recorded: on the last step
+// its end point (clipped to the face it crossed), on a step with a relaxed
+// barrier decision the crossing point of its deposit (else zero).  (bvn_x,
+// bvn_y) is an extra cotangent of the step's end velocity before a
+// reflection flips it (the relaxed decision's dependence on the normal
+// velocity).  Returns the cotangent of the state before the step and adds the
+// field cotangent of the step's four corners to fb (then fbar, [N] {Ex, Ey},
+// which may be null).
 __device__ __forceinline__ Bar4 step_transpose(const StepCapture &c, Bar4 out, double pend_x, double pend_y,
-                                               bool last, int ny, double inv_dx, double *fbar, FieldBar &fb) {
+                                               bool last, int ny, double inv_dx, double *fbar, FieldBar &fb,
+                                               double bvn_x = 0.0, double bvn_y = 0.0) {
     double bsx = 0.0, bsy = 0.0, bx0 = 0.0, by0 = 0.0;
     double bvxn = out.vx, bvyn = out.vy;
     double bxo = out.x, byo = out.y;
-    if (last) {
+    if (last || c.crossed) {
         if (c.crossed) {
             // p = x0 + f s (+ df s), df = -(x0 + f_raw s - face)/s on the crossed axis
             bx0 += pend_x;
@@ -159,6 +164,8 @@ __device__ __forceinline__ Bar4 step_transpose(const StepCapture &c, Bar4 out, d
         bx0 += bxo;
         by0 += byo;
     }
+    bvxn += bvn_x;
+    bvyn += bvn_y;
     // push: vxn = vx + ax dt, s = (v + vn) dt / 2
     const double vxn = c.vx + c.ax * c.dt, vyn = c.vy + c.ay * c.dt;
     double bvx = 0.5 * c.dt * bsx, bvy = 0.5 * c.dt * bsy;
@@ -268,6 +275,166 @@ __global__ void __launch_bounds__(256) push64_adjoint_kernel(Grid g, AdjointIO a
             (void)step64t<BARRIER, TAPERED, true, false>(g, r, cj, xp, yp, rc, cap);
             const bool last = j == n_steps - 1;
             b = step_transpose(cap, b, last ? pe_x : 0.0, last ? pe_y : 0.0, last, g.ny, g.inv_dx, a.bar_field, fb);
+        }
+        fb.flush(a.bar_field);
+        if (a.bar_state) {
+            double *o4 = a.bar_state + 4 * (int64_t)pid;
+            o4[0] = b.x; o4[1] = b.y; o4[2] = b.vx; o4[3] = b.vy;
+        }
+    }
+}
+
+// StepCapture with the relaxed barrier decision of SoftTangent (the same
+// margin t, probability p and cut, so a re-traced path takes the same
+// branches): a step whose decision was relaxed records p, the normal velocity
+// and the two cells of the potential step, and its deposit's cell, face and
+// crossing point (the crossing itself through StepCapture::crossing).  W is
+// the weight carried along the path (a forward sweep multiplies it by p at
+// every relaxed decision).
+struct SoftCapture : StepCapture {
+    double inv_width = 0.0, cut = 0.0;
+    double W = 1.0;
+    bool pend = false, event = false;
+    double p = 0.0, vn = 0.0, aq = 0.0, qmwf = 0.0;
+    int ec = 0, ev = 0, e_cell = 0;
+    bool e_xf = false;
+    double exq = 0.0, eyq = 0.0;
+
+    template <class V>
+    __device__ __forceinline__ void field(const V &g, int i0_, int i1_, int j0_, int j1_, double tx_, double ty_,
+                                          double w00_, double w10_, double w01_, double w11_, double2 e00_,
+                                          double2 e10_, double2 e01_, double2 e11_, double v00_, double v10_,
+                                          double v01_, double v11_, bool clamped_, double sw_, double ex_,
+                                          double ey_) {
+        StepCapture::field(g, i0_, i1_, j0_, j1_, tx_, ty_, w00_, w10_, w01_, w11_, e00_, e10_, e01_, e11_, v00_,
+                           v10_, v01_, v11_, clamped_, sw_, ex_, ey_);
+        pend = false;
+        event = false;
+    }
+    __device__ __forceinline__ bool reflects(double have, double need, double vn_, double qm, double wf, int c, int v,
+                                             bool face_x_) {
+        (void)face_x_;
+        pend = false;
+        const double a = fabs(qm);
+        const double t = (have - need) / a * inv_width;
+        if (!(t > -cut && t < cut)) return have < need;
+        p = 1.0 / (1.0 + exp(t));
+        vn = vn_;
+        aq = a;
+        qmwf = qm * wf;
+        ec = c;
+        ev = v;
+        pend = true;
+        return true;
+    }
+    template <class V>
+    __device__ __forceinline__ void relaxed(const V &, int c, bool xf, double xq, double yq) {
+        event = true;
+        e_cell = c;
+        e_xf = xf;
+        exq = xq;
+        eyq = yq;
+        pend = false;
+        W = W * p;
+    }
+};
+template <> struct is_soft_policy<SoftCapture> : std::true_type {};
+
+// Inputs and outputs of the relaxed push's adjoint (th_trace_soft_adjoint).
+struct SoftAdjointIO {
+    const double *x, *y, *vx, *vy, *qm;    // initial states [n] (the range's own indices)
+    const int8_t *spc;                     // [n]
+    const int32_t *order;                  // lanes take particles in this order [n]
+    const int32_t *steps;                  // path length of each particle [n]
+    const int64_t *group_off;              // tape offset of each group of 64 positions of `order`
+    double4 *tape;                         // per group, step-major start states
+    const double *gbar;                    // [2][m] cotangent of the deposits
+    double inv_width, cut;
+    double *bar_state;                     // [n][4] out
+    double *bar_field;                     // [N][2] accumulated, nullable
+    double *bar_phi;                       // [N] accumulated, nullable
+    unsigned long long *mismatch;
+    int64_t begin, end;
+    int max_steps;
+};
+
+// Adjoint of the relaxed-barrier push: for the cotangent gbar of its
+// face-smoothed deposits, the cotangent of every particle's initial state,
+// of the field and of the potential (through the relaxed decisions).  Along
+// a path the weight obeys W' = W p at a relaxed decision, which deposits
+// a = W (1 - p); backwards, with abar the deposit's cotangent and Wbar that
+// of the weight after the decision: Wbar <- abar (1 - p) + Wbar p, pbar =
+// W (Wbar - abar), tbar = -p (1 - p) pbar, which reaches the normal velocity
+// (dt/dvn = vn / (|q/m| w)) and the potentials of the two cells of the
+// barrier (dt/dphi_c = -q/m wface / (|q/m| w) = -dt/dphi_v); the deposit's
+// crossing point is transposed like the last step's recorded point.  The
+// weight before a decision is recovered as W / p.
+template <bool TAPERED>
+__global__ void __launch_bounds__(256) push64_soft_adjoint_kernel(Grid g, SoftAdjointIO a) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t o = a.begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < a.end; o += stride) {
+        const int pid = a.order[o];
+        const int n_steps = a.steps[pid];
+        const int sp = a.spc[pid] != 0;
+        double4 *tape = a.tape + a.group_off[o >> 6] + (o & 63);
+        State64 s;
+        s.x = a.x[pid]; s.y = a.y[pid]; s.vx = a.vx[pid]; s.vy = a.vy[pid]; s.qm = a.qm[pid];
+        s.q = div_dx(s.x, g.dx, g.inv_dx);
+        s.r = div_dx(s.y, g.dx, g.inv_dx);
+        RfClock rc{0.0, 0.0, 0.0, 0.0};
+        double xp = 0.0, yp = 0.0;
+        int cell = -1, k = 0, st = 0;
+        SoftCapture fw;
+        fw.inv_width = a.inv_width;
+        fw.cut = a.cut;
+        while (k < n_steps) {
+            tape[64 * (int64_t)k] = make_double4(s.x, s.y, s.vx, s.vy);
+            st = step64t<true, TAPERED, true, false>(g, s, cell, xp, yp, rc, fw);
+            ++k;
+            if (st != 0 || k >= a.max_steps) break;
+        }
+        const bool ended = st != 0 || k >= a.max_steps;
+        if (k != n_steps || !ended) {
+            atomicAdd(a.mismatch, 1ull);
+            if (a.bar_state)
+                for (int c = 0; c < 4; ++c) a.bar_state[4 * (int64_t)pid + c] = 0.0;
+            continue;
+        }
+        // the final absorption deposits the remaining weight at the recorded point
+        double W = fw.W, Wbar = 0.0, pfx = 0.0, pfy = 0.0;
+        if (st == 1) soft_face_vjp(g, a.gbar, sp, cell, fw.x_face, xp, yp, W, Wbar, pfx, pfy);
+        Bar4 b{0.0, 0.0, 0.0, 0.0};
+        FieldBar fb;
+        for (int j = n_steps - 1; j >= 0; --j) {
+            const double4 t0 = tape[64 * (int64_t)j];
+            State64 r;
+            r.x = t0.x; r.y = t0.y; r.vx = t0.z; r.vy = t0.w; r.qm = s.qm;
+            r.q = div_dx(r.x, g.dx, g.inv_dx);
+            r.r = div_dx(r.y, g.dx, g.inv_dx);
+            SoftCapture cap;
+            cap.inv_width = a.inv_width;
+            cap.cut = a.cut;
+            int cj = -1;
+            (void)step64t<true, TAPERED, true, false>(g, r, cj, xp, yp, rc, cap);
+            const bool last = j == n_steps - 1;
+            double px = last ? pfx : 0.0, py = last ? pfy : 0.0, bvx = 0.0, bvy = 0.0;
+            if (cap.event) {
+                const double p = cap.p, Wb = W / p;   // the weight before this decision
+                double ab, ex, ey;
+                soft_face_vjp(g, a.gbar, sp, cap.e_cell, cap.e_xf, cap.exq, cap.eyq, Wb * (1.0 - p), ab, ex, ey);
+                const double tbar = -p * (1.0 - p) * (Wb * (Wbar - ab));
+                const double u = tbar * a.inv_width / cap.aq;
+                if (cap.face_x) bvx = u * cap.vn; else bvy = u * cap.vn;
+                if (a.bar_phi) {
+                    unsafeAtomicAdd(&a.bar_phi[cap.ec], -u * cap.qmwf);
+                    unsafeAtomicAdd(&a.bar_phi[cap.ev], u * cap.qmwf);
+                }
+                Wbar = ab * (1.0 - p) + Wbar * p;
+                W = Wb;
+                px += ex;
+                py += ey;
+            }
+            b = step_transpose(cap, b, px, py, last, g.ny, g.inv_dx, a.bar_field, fb, bvx, bvy);
         }
         fb.flush(a.bar_field);
         if (a.bar_state) {

@@ -2193,6 +2193,116 @@ extern "C" int th_trace_adjoint(th_tracer *t, const double *x, const double *y, 
     return TH_OK;
 }
 
+extern "C" int th_trace_soft_adjoint(th_tracer *t, int64_t offset, int64_t n, const th_trace_params *p,
+                                     double width, double cut, const int32_t *steps, const double *gbar,
+                                     double *bar_state, double *bar_field, double *bar_phi, double *ms) {
+    if (!t || !p || !steps || !gbar || !bar_state || offset < 0 || n < 0) return fail(TH_EINVAL, "invalid argument");
+    if (offset + n > t->cap) return fail(TH_EINVAL, "pool holds %lld particles", (long long)t->cap);
+    if (!(width > 0.0) || !(cut > 0.0) || !std::isfinite(width) || !std::isfinite(cut))
+        return fail(TH_EINVAL, "width and cut must be positive and finite");
+    if (!p->barrier) return fail(TH_EINVAL, "the relaxed-barrier push needs the barrier wall model");
+    if (p->precision != 0 || p->rf) return fail(TH_EINVAL, "the relaxed-barrier push follows the FP64 push without RF");
+    int rc = check_params(t, p);
+    if (rc) return rc;
+    for (int64_t k = 0; k < n; ++k)
+        if (steps[k] < 1 || steps[k] > p->max_steps)
+            return fail(TH_EINVAL, "steps[%lld] = %d is outside [1, max_steps]", (long long)k, steps[k]);
+    if (bar_field) memset(bar_field, 0, sizeof(double) * 2 * (size_t)t->N);
+    if (bar_phi) memset(bar_phi, 0, sizeof(double) * (size_t)t->N);
+    if (ms) *ms = 0.0;
+    if (n == 0) return TH_OK;
+    TH_HIP(hipSetDevice(t->device));
+    // tape layout and chunks as th_trace_adjoint: longest paths first, 64
+    // positions of `order` per wave sharing a step-major tape region
+    const size_t nn = (size_t)n, m2 = 2 * (size_t)t->m;
+    std::vector<int32_t> order(nn);
+    for (size_t k = 0; k < nn; ++k) order[k] = (int32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
+    constexpr int64_t kTapeSteps = (int64_t)1 << 27;
+    const size_t ng = (nn + 63) / 64;
+    std::vector<int64_t> goff(ng);
+    std::vector<std::pair<int64_t, int64_t>> chunks;
+    int64_t run = 0, begin = 0, tape_steps = 0;
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const int64_t len = 64 * (int64_t)steps[order[64 * gi]];
+        if (run + len > kTapeSteps && run > 0) {
+            chunks.emplace_back(begin, (int64_t)(64 * gi));
+            tape_steps = std::max(tape_steps, run);
+            begin = (int64_t)(64 * gi);
+            run = 0;
+        }
+        goff[gi] = run;
+        run += len;
+    }
+    chunks.emplace_back(begin, (int64_t)nn);
+    tape_steps = std::max(tape_steps, run);
+    int32_t *d_order = nullptr, *d_steps = nullptr;
+    int64_t *d_off = nullptr;
+    double *d_state = nullptr, *d_field = nullptr, *d_phi = nullptr, *d_g = nullptr;
+    double4 *d_tape = nullptr;
+    unsigned long long *d_mis = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(t->stream);
+        for (void *q : {(void *)d_order, (void *)d_steps, (void *)d_off, (void *)d_state, (void *)d_field,
+                        (void *)d_phi, (void *)d_g, (void *)d_tape, (void *)d_mis})
+            if (q) (void)hipFree(q);
+    };
+    if ((rc = dev_alloc(&d_order, nn)) || (rc = dev_alloc(&d_steps, nn)) || (rc = dev_alloc(&d_off, ng)) ||
+        (rc = dev_alloc(&d_state, 4 * nn)) || (bar_field && (rc = dev_alloc(&d_field, 2 * (size_t)t->N))) ||
+        (bar_phi && (rc = dev_alloc(&d_phi, (size_t)t->N))) || (rc = dev_alloc(&d_g, m2)) ||
+        (rc = dev_alloc(&d_tape, (size_t)tape_steps)) || (rc = dev_alloc(&d_mis, 1))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_order, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_steps, steps, sizeof(int32_t) * nn, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, goff.data(), sizeof(int64_t) * ng, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_g, gbar, sizeof(double) * m2, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess && d_field) e = hipMemsetAsync(d_field, 0, sizeof(double) * 2 * (size_t)t->N, t->stream);
+    if (e == hipSuccess && d_phi) e = hipMemsetAsync(d_phi, 0, sizeof(double) * (size_t)t->N, t->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d_mis, 0, sizeof(unsigned long long), t->stream);
+    if (e != hipSuccess) {
+        release();
+        return fail(TH_EHIP, "relaxed adjoint upload: %s", hipGetErrorString(e));
+    }
+    th::Grid g = make_grid(t, p);
+    th::SoftAdjointIO a{t->d_x + offset, t->d_y + offset, t->d_vx + offset, t->d_vy + offset, t->d_qm + offset,
+                        t->d_spc + offset, d_order, d_steps, d_off, d_tape, d_g, 1.0 / width, cut, d_state, d_field,
+                        d_phi, d_mis, 0, 0, p->max_steps};
+    e = hipEventRecord(t->ev0, t->stream);
+    for (const auto &ch : chunks) {
+        if (e != hipSuccess) break;
+        a.begin = ch.first;
+        a.end = ch.second;
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ch.second - ch.first + 255) / 256,
+                                                                          (int64_t)t->cus * 8));
+        auto kernel = p->tapered ? th::push64_soft_adjoint_kernel<true> : th::push64_soft_adjoint_kernel<false>;
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, t->stream, g, a);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess) e = hipEventRecord(t->ev1, t->stream);
+    unsigned long long mismatch = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(bar_state, d_state, sizeof(double) * 4 * nn, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && bar_field)
+        e = hipMemcpyAsync(bar_field, d_field, sizeof(double) * 2 * (size_t)t->N, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && bar_phi)
+        e = hipMemcpyAsync(bar_phi, d_phi, sizeof(double) * (size_t)t->N, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&mismatch, d_mis, sizeof mismatch, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    float el = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&el, t->ev0, t->ev1);
+    release();
+    if (e != hipSuccess) return fail(TH_EHIP, "relaxed adjoint push: %s", hipGetErrorString(e));
+    if (ms) *ms = el;
+    if (mismatch)
+        return fail(TH_EINVAL, "%llu particle(s) took a different number of steps than `steps` gives: the steps "
+                               "must come from th_trace_soft of the same particles, field, width and cut",
+                    mismatch);
+    return TH_OK;
+}
+
 extern "C" int th_charging_set_monitors(th_charging *c, int32_t n_probes, const int32_t *probe_cells,
                                         int32_t n_regions, const int32_t *region_ptr, const int32_t *region_idx,
                                         int32_t abs_region, int32_t n_bottom, const int32_t *bottom_s) {

@@ -383,20 +383,19 @@ struct Tangent64 {
     }
 };
 
-// Face-smoothed deposit of weight a (tangent da) absorbed in cell c across an
-// x-face (xf) or a y-face at the crossing point (xq, yq), tangent (dxq, dyq):
-// the neighbour along the face on the side of the crossing point's coordinate
-// u along it (u in [0, 1] in cell units) takes |u - 1/2| of the weight when it
-// is a surface cell with vacuum in front of it on the particle's side, the
-// hit cell the rest; the share moves with u, so its tangent is sign(u - 1/2)
-// du (zero when u was clipped to the face's end).  Accumulated into dep /
-// ddep ([2][m], species sp).  Off-surface cells receive nothing.
+// Face smoothing of a deposit in cell c (surface index k) crossed through an
+// x-face (xf) or a y-face at the point (xq, yq): the neighbour along the face
+// on the side of the point's coordinate u along it (u in [0, 1] in cell
+// units) takes the share w_nb = |u - 1/2| when it is a surface cell (index
+// knb) with vacuum in front of it on the particle's side, the hit cell the
+// rest; slope = d w_nb / d(coordinate along the face) [1/m] (zero when u was
+// clipped to the face's end or no neighbour shares).  Returns false for a
+// cell off the surface.
 template <class V>
-__device__ __forceinline__ void soft_face_deposit(const V &g, double *dep, double *ddep, int sp, int c, bool xf,
-                                                  double xq, double yq, double dxq, double dyq, double a,
-                                                  double da) {
-    const int k = g.smap[c];
-    if (k < 0) return;
+__device__ __forceinline__ bool face_share(const V &g, int c, bool xf, double xq, double yq, int &k, int &knb,
+                                           double &w_nb, double &slope) {
+    k = g.smap[c];
+    if (k < 0) return false;
     const int nx = g.nx, ny = g.ny, cx = c / ny, cy = c - (c / ny) * ny;
     const double ex = xq / g.dx, ey = yq / g.dx;
     double along = xf ? ey - (double)cy : py_mod(ex - (double)cx, (double)nx);
@@ -408,23 +407,53 @@ __device__ __forceinline__ void soft_face_deposit(const V &g, double *dep, doubl
     const int nbx = xf ? cx : ((cx + step) % nx + nx) % nx, nby = xf ? cy + step : cy;
     const int fx = xf ? ((nbx + side) % nx + nx) % nx : nbx, fy = xf ? nby : nby + side;
     bool valid = nby >= 0 && nby < ny && fy >= 0 && fy < ny;
-    int knb = k;
+    knb = k;
     if (valid) {
         const int nb = nbx * ny + nby;
         valid = !g.is_vac(nb) && g.is_vac(fx * ny + fy) && g.smap[nb] >= 0;
         if (valid) knb = g.smap[nb];
     }
-    const double w_nb = valid ? fabs(along - 0.5) : 0.0;
+    w_nb = valid ? fabs(along - 0.5) : 0.0;
     const double sgn = along > 0.5 ? 1.0 : (along < 0.5 ? -1.0 : 0.0);
-    const double slope = valid && moving ? sgn / g.dx : 0.0;
+    slope = valid && moving ? sgn / g.dx : 0.0;
+    return true;
+}
+
+// Face-smoothed deposit of weight a (tangent da) absorbed in cell c at the
+// crossing point (xq, yq), tangent (dxq, dyq) (face_share): the shares and
+// their tangents (the share's tangent is slope times the tangent of the
+// coordinate along the face) accumulated into dep / ddep ([2][m], species sp).
+template <class V>
+__device__ __forceinline__ void soft_face_deposit(const V &g, double *dep, double *ddep, int sp, int c, bool xf,
+                                                  double xq, double yq, double dxq, double dyq, double a,
+                                                  double da) {
+    int k, knb;
+    double w_nb, slope;
+    if (!face_share(g, c, xf, xq, yq, k, knb, w_nb, slope)) return;
     const double dw = slope * (xf ? dyq : dxq);
     const int m = g.m;
     atomicAdd(&dep[sp * m + k], a * (1.0 - w_nb));
     atomicAdd(&ddep[sp * m + k], da * (1.0 - w_nb) - a * dw);
-    if (valid) {
+    if (knb != k || w_nb != 0.0) {
         atomicAdd(&dep[sp * m + knb], a * w_nb);
         atomicAdd(&ddep[sp * m + knb], da * w_nb + a * dw);
     }
+}
+
+// Transpose of one face-smoothed deposit for the cotangent gbar ([2][m]) of
+// the deposits: the cotangent of the weight a (abar) and of the crossing
+// point (pbx, pby).
+template <class V>
+__device__ __forceinline__ void soft_face_vjp(const V &g, const double *gbar, int sp, int c, bool xf, double xq,
+                                              double yq, double a, double &abar, double &pbx, double &pby) {
+    int k, knb;
+    double w_nb, slope;
+    abar = pbx = pby = 0.0;
+    if (!face_share(g, c, xf, xq, yq, k, knb, w_nb, slope)) return;
+    const double gk = gbar[sp * g.m + k], gn = gbar[sp * g.m + knb];
+    abar = (1.0 - w_nb) * gk + w_nb * gn;
+    const double pa = a * slope * (gn - gk);
+    if (xf) pby = pa; else pbx = pa;
 }
 
 // Tangent of the relaxed-barrier push (SoftIO): Tangent64's path derivative
@@ -493,8 +522,9 @@ struct SoftTangent : Tangent64 {
     }
 };
 
-template <class T> struct is_soft_tangent : std::false_type {};
-template <> struct is_soft_tangent<SoftTangent> : std::true_type {};
+// Policies that take the relaxed barrier decision (reflects / pend / relaxed hooks).
+template <class T> struct is_soft_policy : std::false_type {};
+template <> struct is_soft_policy<SoftTangent> : std::true_type {};
 
 // One FP64 push step.  Returns 0 (continue), 1 (absorbed at *cell), 2
 // (escaped).  With RECORD, (xp, yp) is the step's recorded point.  With RF the
@@ -698,7 +728,7 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
                 dphi = dphi + g.wface[c] * (cs * (pc.x - pv.x) + cm * (pc.y - pv.y));
             }
             bool reflect;
-            if constexpr (is_soft_tangent<T>::value)
+            if constexpr (is_soft_policy<T>::value)
                 reflect = tan.reflects(0.5 * vn * vn, s.qm * dphi, vn, s.qm, g.face(c), c, v, face_x);
             else
                 reflect = 0.5 * vn * vn < s.qm * dphi;
@@ -736,7 +766,7 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
             tf = frac;
             tan.crossing(face_x, raw, frac, sx, sy);
         }
-        if constexpr (is_soft_tangent<T>::value) {
+        if constexpr (is_soft_policy<T>::value) {
             if (tan.pend) {   // the relaxed decision's absorbed share, at the face it would have crossed
                 double raw, frac, xq, yq;
                 face_point(raw, frac, xq, yq);

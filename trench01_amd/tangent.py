@@ -484,6 +484,35 @@ class PathwiseBatch:
         dinc[self.cond_s] = 0.0
         return inc, dinc, r
 
+    def relaxed_vjp(self, rho_s, g, width, cut=30.0, sigma_deg=None, energy_shift=0.0):
+        """Reverse mode of ``relaxed``: from one relaxed push and its adjoint
+        (th_trace_soft_adjoint), the gradient of <g, relaxed increment> with
+        respect to every surface cell's charge (through the field and through
+        the potential at the walls' relaxed decisions) and to (ion angular
+        sigma [deg], energy shift [eV]).  Returns (increment [m], gradient
+        [m], ion gradient [2])."""
+        rho_s = np.zeros(self.surface.size) if rho_s is None else np.asarray(rho_s, dtype=float)
+        sampler = ion_sampler(self.p, energy_shift, sigma_deg)
+        parts, tan = draw_batch(self.p, self.geo.ny, self.seed, sampler, self.n)
+        phi, ex, ey = self.field(rho_s)
+        total = parts[0].size
+        t = self.tracer
+        t.reserve(total)
+        t.upload(0, *parts)
+        t.set_fields(ex, ey, (phi, self.fields.wface))
+        r = t.soft(0, total, self.p, width, cut, per_particle=True)
+        inc = (r["deposit"][0] - r["deposit"][1]) * self.scale
+        inc[self.cond_s] = 0.0
+        gs = np.where(self.cond_s, 0.0, np.asarray(g, dtype=float)) * self.scale
+        bar_state, bar_field, bar_phi = t.soft_adjoint(0, total, self.p, width, r["steps"], np.stack([gs, -gs]), cut)
+        vac = self.geo.vac
+        bphi = linear_field_transpose(bar_field[..., 0] * vac, bar_field[..., 1] * vac, vac, self.p) + bar_phi
+        s = self.setup
+        per_col = s.green.T @ bphi.ravel()[s.rows]
+        grad_rho = np.where(s.col_of >= 0, per_col[np.maximum(s.col_of, 0)], 0.0) * self.p.dx ** 2
+        ion = np.array([bar_state[:, 2] @ tan[k][0] + bar_state[:, 3] @ tan[k][1] for k in ("sigma", "energy")])
+        return inc, grad_rho, ion
+
     def charge_jvp(self, rho_s, drho_s, sigma_deg=None, energy_shift=0.0):
         """(increment [m], d increment along the surface-charge direction drho_s [m])."""
         drho = np.zeros(self.geo.solid.size)

@@ -378,6 +378,32 @@ class HipTracer:
             out["events"] = ev[:n_ev.value]
         return out
 
+    def soft_adjoint(self, offset, n, p, width, steps, gbar, cut=30.0, field=True, potential=True):
+        """Reverse mode of ``soft`` (th_trace_soft_adjoint): for the cotangent
+        ``gbar`` [2, m] of the relaxed deposits of pool[offset, offset+n)
+        (``steps`` from the ``soft`` call with ``per_particle``), returns
+        (bar_state [n, 4], bar_field [nx, ny, 2] or None, bar_phi [nx, ny] or
+        None): the cotangents of the initial states, the vacuum-masked field
+        and the potential."""
+        L = _lib.load()
+        nx, ny = self.vac.shape
+        steps = np.ascontiguousarray(steps, dtype=np.int32)
+        g = np.ascontiguousarray(gbar, dtype=np.float64)
+        if steps.shape != (n,) or g.shape != (2, self.surface_cells().size):
+            raise ValueError("steps must be [n] and gbar [2, m]")
+        prm = self.params_for(p, True)
+        prm.precision = 0
+        bar_state = np.empty((n, 4))
+        bar_field = np.empty((nx, ny, 2)) if field else None
+        bar_phi = np.empty((nx, ny)) if potential else None
+        ms = ctypes.c_double(0.0)
+        _lib.check(L.th_trace_soft_adjoint(self._h, int(offset), int(n), ctypes.byref(prm), float(width), float(cut),
+                                           _lib.ptr(steps, _lib.i32p), _lib.ptr(g, _lib.dp),
+                                           _lib.ptr(bar_state, _lib.dp), _lib.ptr(bar_field, _lib.dp),
+                                           _lib.ptr(bar_phi, _lib.dp), ctypes.byref(ms)))
+        self.last_adjoint_ms = ms.value
+        return bar_state, bar_field, bar_phi
+
     def adjoint(self, x, y, vx, vy, qm, spc, Ex, Ey, p, barrier, steps, bar_end=None, bar_vel=None, field=True):
         """Reverse-mode pathwise derivative (th_trace_adjoint): the transpose
         of ``tangent``.  ``steps`` are the particles' path lengths from a
