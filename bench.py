@@ -277,6 +277,12 @@ def differentiation(p, dev, rho, push_ms):
     t0 = time.perf_counter()
     vb.charge_jvp(rho, rho)
     jvp_s = time.perf_counter() - t0
+    # the relaxed-barrier push's reverse mode: the same gradient for the relaxed increment
+    vb.relaxed_vjp(rho, g, 1.0)                      # warm-up
+    t0 = time.perf_counter()
+    vb.relaxed_vjp(rho, g, 1.0)
+    relaxed_vjp_s = time.perf_counter() - t0
+    relaxed_adjoint_ms = vb.tracer.last_adjoint_ms
     m = int(vb.surface.size)
     vb.close()
     agreement = tangent.derivative_agreement(replace(p, n_per_batch=250_000), rho, relaxed_width=1.0)
@@ -284,7 +290,8 @@ def differentiation(p, dev, rho, push_ms):
             "kernel": "push64_tangent_kernel", "agreement": agreement,
             "relaxed_push_ms": min(soft_times), "relaxed_kernel": "push64_soft_kernel",
             "reverse_mode": {"vjp_s": vjp_s, "adjoint_kernel_ms": adjoint_ms, "jvp_s": jvp_s, "surface_cells": m,
-                             "m_jvp_loop_s_estimate": m * jvp_s,
+                             "m_jvp_loop_s_estimate": m * jvp_s, "relaxed_vjp_s": relaxed_vjp_s,
+                             "relaxed_adjoint_kernel_ms": relaxed_adjoint_ms,
                              "note": "PathwiseBatch.vjp (primal trace, face-deposit VJP, th_trace_adjoint, field "
                                      "stencil and G^T on the host) against one forward-mode JVP; the backward that "
                                      "the forward mode alone gives takes one JVP per surface cell"}}
