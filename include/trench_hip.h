@@ -437,9 +437,10 @@ int th_charging_green_adjoint_noalloc(th_charging *c, const double *w, double *o
 int th_charging_read(th_charging *c, double *rho_s, double *phi /* nullable */, int32_t *counts /* nullable */);
 
 /* Device self-check of the FP64 arithmetic the push relies on: for each k,
- * out[7k..7k+6] = {hypot(a,b), sqrt(a), a/b, fmod(a,b), sqrt(|a|) and a/b in
- * their range-free forms, and the CFL step 1.75e-9 / (|a| + sqrt(1e-8 |b|) +
- * 1)} computed on the GPU with the kernels' own routines.  No reference
+ * out[9k..9k+8] = {hypot(a,b), sqrt(a), a/b, fmod(a,b), sqrt(|a|) and a/b in
+ * their range-free forms, the CFL step 1.75e-9 / (|a| + sqrt(1e-8 |b|) + 1),
+ * and hypot(a,b), hypot(b 2^-500, a) as one paired evaluation} computed on
+ * the GPU with the kernels' own routines.  No reference
  * counterpart; used by tests to pin the device arithmetic against the host C
  * library. */
 int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, double *out);

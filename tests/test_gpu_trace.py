@@ -47,6 +47,8 @@ def test_device_arithmetic_matches_host_c_library():
     out = t.selftest_math(a, b)
     with np.errstate(all="ignore"):
         np.testing.assert_array_equal(out[:, 0], trace_ref.glibc_hypot(a, b))
+        np.testing.assert_array_equal(out[:, 7], out[:, 0])   # the paired hypot, both pairs
+        np.testing.assert_array_equal(out[:, 8], trace_ref.glibc_hypot(b * 2.0 ** -500, a))
         np.testing.assert_array_equal(out[:, 1], np.sqrt(a))
         np.testing.assert_array_equal(out[:, 2], a / b)
         np.testing.assert_array_equal(out[:, 3], np.fmod(a, b))

@@ -22,6 +22,9 @@ def host_hypot(tmp_path_factory):
     src = d / "h.c"
     src.write_text('#include "%s"\nvoid th_hypot_arr(const double *a, const double *b, double *o, long n) '
                    '{ for (long k = 0; k < n; ++k) o[k] = th_hypot(a[k], b[k]); }\n'
+                   'void th_hypot2_arr(const double *a, const double *b, const double *c, const double *d, '
+                   'double *o0, double *o1, long n) '
+                   '{ for (long k = 0; k < n; ++k) th_hypot2(a[k], b[k], c[k], d[k], &o0[k], &o1[k]); }\n'
                    % (ROOT / "trench01_amd" / "csrc" / "th_math.h"))
     lib = d / "libh.so"
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC", str(src), "-o", str(lib),
@@ -35,6 +38,7 @@ def host_hypot(tmp_path_factory):
         o = np.empty_like(a)
         L.th_hypot_arr(a.ctypes.data_as(dp), b.ctypes.data_as(dp), o.ctypes.data_as(dp), a.size)
         return o
+    f.lib = L
     return f
 
 
@@ -86,3 +90,26 @@ def test_special_values(host_hypot):
     for r in (2.0 ** -53, 2.0 ** -54, 2.0 ** -55):
         y = x * r * (1 + np.linspace(-1e-3, 1e-3, x.size))
         assert same(host_hypot(x, y), trace_ref.glibc_hypot(x, y))
+
+
+def test_paired_hypot_is_two_hypots(host_hypot):
+    """th_hypot2 (both cores evaluated, out-of-range pairs recomputed) gives
+    the doubles of two th_hypot calls, glibc's, for every combination of a
+    pair inside and outside the core's range (special values included)."""
+    L = host_hypot.lib
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.th_hypot2_arr.argtypes = [dp, dp, dp, dp, dp, dp, ctypes.c_long]
+    rng = np.random.default_rng(5)
+    n = 1 << 18
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, 2.0 ** -383, 2.0 ** -384, 2.0 ** 511,
+                        2.0 ** 512, 1.0, -3.5e7])
+    def draw():
+        v = np.ldexp(rng.random(n) + 0.5, rng.integers(-1074, 1023, n)) * rng.choice([-1, 1], n)
+        k = rng.random(n) < 0.1
+        v[k] = rng.choice(special, k.sum())
+        return v
+    a, b, c, d = (np.ascontiguousarray(draw()) for _ in range(4))
+    o0, o1 = np.empty(n), np.empty(n)
+    L.th_hypot2_arr(*(x.ctypes.data_as(dp) for x in (a, b, c, d, o0, o1)), n)
+    assert same(o0, trace_ref.glibc_hypot(a, b))
+    assert same(o1, trace_ref.glibc_hypot(c, d))

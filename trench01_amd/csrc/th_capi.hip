@@ -768,7 +768,7 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
     if (n == 0) return TH_OK;
     TH_HIP(hipSetDevice(t->device));
     double *d = nullptr;
-    TH_HIP(hipMalloc((void **)&d, sizeof(double) * 9 * (size_t)n));
+    TH_HIP(hipMalloc((void **)&d, sizeof(double) * 11 * (size_t)n));
     int rc = TH_OK;
     hipError_t e = hipMemcpyAsync(d, a, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d + n, b, sizeof(double) * n, hipMemcpyHostToDevice, t->stream);
@@ -779,7 +779,7 @@ int th_selftest_math(th_tracer *t, const double *a, const double *b, int64_t n, 
                            d + n, n, d + 2 * n);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, sizeof(double) * 7 * n, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, t->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
     if (e != hipSuccess) rc = fail(TH_EHIP, "selftest: %s", hipGetErrorString(e));
     (void)hipFree(d);

@@ -163,3 +163,22 @@ TH_HD double th_hypot(double x, double y) {
     return th_hypot_full(x, y);
 }
 
+// th_hypot of two independent pairs: both cores are evaluated unconditionally
+// in one straight-line block (their dependent chains interleave), and a pair
+// outside the core's range -- rare -- is then recomputed by th_hypot_full.
+// The same doubles as two th_hypot calls.
+TH_HD void th_hypot2(double x0, double y0, double x1, double y1, double *h0, double *h1) {
+    const double fx0 = fabs(x0), fy0 = fabs(y0), fx1 = fabs(x1), fy1 = fabs(y1);
+    const double a0 = fmax(fx0, fy0), b0 = fmin(fx0, fy0);
+    const double a1 = fmax(fx1, fy1), b1 = fmin(fx1, fy1);
+    const int ok0 = !__builtin_isunordered(x0, y0) && a0 <= 0x1p+511 && b0 >= 0x1p-383;
+    const int ok1 = !__builtin_isunordered(x1, y1) && a1 <= 0x1p+511 && b1 >= 0x1p-383;
+    double r0 = th_hypot_core_n(a0, b0), r1 = th_hypot_core_n(a1, b1);
+    if (__builtin_expect(!(ok0 && ok1), 0)) {
+        if (!ok0) r0 = th_hypot_full(x0, y0);
+        if (!ok1) r1 = th_hypot_full(x1, y1);
+    }
+    *h0 = r0;
+    *h1 = r1;
+}
+

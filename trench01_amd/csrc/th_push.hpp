@@ -633,8 +633,9 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         // Acceleration bound over the period, then the field at mid-step.
         double bx = fabs(ex) + fma(g.bound0, fabs(e0x), g.bound1 * fabs(e1x));
         double by = fabs(ey) + fma(g.bound0, fabs(e0y), g.bound1 * fabs(e1y));
-        double accel = fabs(s.qm) * th_hypot(bx, by);
-        dt = th_cfl_dt(g.cfl_dx, g.two_dx, th_hypot(s.vx, s.vy), accel);
+        double hb, hv;
+        th_hypot2(bx, by, s.vx, s.vy, &hb, &hv);
+        dt = th_cfl_dt(g.cfl_dx, g.two_dx, hv, fabs(s.qm) * hb);
         if (!(dt < g.dt_max) && !isnan(dt)) dt = g.dt_max;
         double vs, vm;
         rf_coefficients(g, rc.clock + dt / 2.0, vs, vm);
@@ -647,7 +648,9 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
     } else {
         ax = s.qm * ex;
         ay = s.qm * ey;
-        dt = th_cfl_dt(g.cfl_dx, g.two_dx, th_hypot(s.vx, s.vy), th_hypot(ax, ay));
+        double hv, ha;
+        th_hypot2(s.vx, s.vy, ax, ay, &hv, &ha);
+        dt = th_cfl_dt(g.cfl_dx, g.two_dx, hv, ha);
         tan.push(s.vx, s.vy, s.qm, ax, ay, dt, g.two_dx);
     }
     double vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
@@ -1261,13 +1264,17 @@ __global__ void selftest_math_kernel(const double *a, const double *b, int64_t n
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double x = a[k], y = b[k];
-    out[7 * k + 0] = th_hypot(x, y);
-    out[7 * k + 1] = sqrt(x);
-    out[7 * k + 2] = x / y;
-    out[7 * k + 3] = fmod(x, y);
-    out[7 * k + 4] = th_sqrt_n(fabs(x));
-    out[7 * k + 5] = th_div_n(x, y);
-    out[7 * k + 6] = th_cfl_dt(1.75e-9, 1e-8, fabs(x), fabs(y));
+    out[9 * k + 0] = th_hypot(x, y);
+    out[9 * k + 1] = sqrt(x);
+    out[9 * k + 2] = x / y;
+    out[9 * k + 3] = fmod(x, y);
+    out[9 * k + 4] = th_sqrt_n(fabs(x));
+    out[9 * k + 5] = th_div_n(x, y);
+    out[9 * k + 6] = th_cfl_dt(1.75e-9, 1e-8, fabs(x), fabs(y));
+    double h0, h1;
+    th_hypot2(x, y, y * 0x1p-500, x, &h0, &h1);   // the second pair often outside the core's range
+    out[9 * k + 7] = h0;
+    out[9 * k + 8] = h1;
 }
 
 __global__ void copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst, int64_t n) {

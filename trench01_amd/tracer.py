@@ -440,10 +440,11 @@ class HipTracer:
 
     def selftest_math(self, a, b):
         """Device arithmetic on arrays (for arithmetic pinning): columns hypot(a, b), sqrt(a), a / b,
-        fmod(a, b), the range-free sqrt(|a|) and a / b, and the CFL step 1.75e-9 / (|a| + sqrt(1e-8 |b|) + 1)."""
+        fmod(a, b), the range-free sqrt(|a|) and a / b, the CFL step 1.75e-9 / (|a| + sqrt(1e-8 |b|) + 1), and
+        hypot(a, b), hypot(b 2^-500, a) from the paired evaluation."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
-        out = np.empty((a.size, 7), dtype=np.float64)
+        out = np.empty((a.size, 9), dtype=np.float64)
         _lib.check(_lib.load().th_selftest_math(self._h, _lib.ptr(a, _lib.dp), _lib.ptr(b, _lib.dp), a.size,
                                                 _lib.ptr(out, _lib.dp)))
         return out
