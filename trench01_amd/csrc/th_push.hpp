@@ -602,12 +602,13 @@ __device__ __forceinline__ int step64t(const V &g, State64 &s, int &cell, double
         v00 = c00.z; v10 = c10.z; v01 = c01.z; v11 = c11.z;
         fl = (v00 != 0.0 ? 1u : 0u) | (v10 != 0.0 ? 2u : 0u) | (v01 != 0.0 ? 4u : 0u) | (v11 != 0.0 ? 8u : 0u);
     }
-    // sw starts at its first term: 0 + w v == w v for w v >= +0
-    double ex = 0.0, ey = 0.0, sw = w00 * v00;
+    // sw starts at its first term: 0 + w v == w v for w v >= +0; with v in
+    // {0, 1} (vacuum flag bit k of fl) and w >= 0, w v is a select of w or +0
+    double ex = 0.0, ey = 0.0, sw = (fl & 1u) ? w00 : 0.0;
     ex += w00 * e00.x; ey += w00 * e00.y;
-    ex += w10 * e10.x; ey += w10 * e10.y; sw += w10 * v10;
-    ex += w01 * e01.x; ey += w01 * e01.y; sw += w01 * v01;
-    ex += w11 * e11.x; ey += w11 * e11.y; sw += w11 * v11;
+    ex += w10 * e10.x; ey += w10 * e10.y; sw += (fl & 2u) ? w10 : 0.0;
+    ex += w01 * e01.x; ey += w01 * e01.y; sw += (fl & 4u) ? w01 : 0.0;
+    ex += w11 * e11.x; ey += w11 * e11.y; sw += (fl & 8u) ? w11 : 0.0;
     const bool clamped = sw < 1e-12;
     sw = clamped ? 1e-12 : sw;
     // Dividing by a weight sum of exactly 1 (four vacuum cells, as for most
