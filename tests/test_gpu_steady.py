@@ -97,7 +97,8 @@ def test_newton_krylov_reaches_the_time_marched_state(time_marched, jvp):
     descend, with a plain phase whose length varies between seeds."""
     print(f"[gpu case] steady AR8 (1.25e5 per species) newton-krylov, {jvp} JVPs", flush=True)
     p, nb, long_run = time_marched
-    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1 if jvp == "difference" else 2, jvp=jvp)
+    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1 if jvp == "difference" else 2, jvp=jvp,
+                              forcing=None if jvp == "difference" else 0.03)
     v = nk["steady"]["bottom center"]
     print(f"plain {nb} batches, 300 on {long_run:.3f} V; newton-krylov {nk['batches']} batches {nk['phases']}, "
           f"{v:.3f} V", flush=True)

@@ -121,6 +121,22 @@ def test_gmres_solves_a_small_system_in_n_steps():
     assert k == n
 
 
+def test_gmres_forcing_term_stops_at_the_requested_residual():
+    """With rtol, GMRES stops at the first iteration whose least-squares
+    residual |b - A x| is at most rtol |b| -- the true residual of the x it
+    returns -- and the previous iteration's was above it."""
+    rng = np.random.default_rng(9)
+    n = 40
+    A = np.eye(n) + 0.3 * rng.normal(size=(n, n)) / np.sqrt(n)
+    b = rng.normal(size=n)
+    for rtol in (0.3, 0.1, 1e-3):
+        x, k = steady.gmres(lambda v: A @ v, b, krylov=30, rtol=rtol)
+        assert np.linalg.norm(A @ x - b) <= rtol * np.linalg.norm(b) * (1 + 1e-9)
+        xp, _ = steady.gmres(lambda v: A @ v, b, krylov=k - 1)
+        assert k == 1 or np.linalg.norm(A @ xp - b) > rtol * np.linalg.norm(b)
+        assert k < 30
+
+
 def test_newton_krylov_solves_an_affine_map_in_one_newton_step():
     """With exact differences (affine map) and a Krylov space of full
     dimension, one Newton step lands on the fixed point."""

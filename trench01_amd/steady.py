@@ -325,11 +325,13 @@ def anderson(p, stages=None, memory=10, max_evals=40, patience=10, log=None, dev
                 steady=steady, hist=hist, rho_s=rho_s, wall_s=wall, setup_s=ss.setup_s)
 
 
-def gmres(matvec, b, krylov):
+def gmres(matvec, b, krylov, rtol=None):
     """``krylov`` steps of GMRES from x0 = 0 (Arnoldi with modified
     Gram-Schmidt, least squares on the Hessenberg matrix; Saad & Schultz,
     SIAM J. Sci. Stat. Comput. 7(3), 1986).  Stops early when the Krylov space
-    is exhausted.  Returns (x, matvecs used)."""
+    is exhausted or, with ``rtol``, once the least-squares residual |b - A x|
+    falls to rtol |b| (the forcing term of an inexact Newton step).  Returns
+    (x, matvecs used)."""
     n = b.size
     beta = float(np.linalg.norm(b))
     if beta == 0.0:
@@ -349,6 +351,13 @@ def gmres(matvec, b, krylov):
             j += 1
             break
         V[j + 1] = w / H[j + 1, j]
+        if rtol is not None:
+            e1 = np.zeros(j + 2)
+            e1[0] = beta
+            y = np.linalg.lstsq(H[:j + 2, :j + 1], e1, rcond=None)[0]
+            if np.linalg.norm(H[:j + 2, :j + 1] @ y - e1) <= rtol * beta:
+                j += 1
+                break
     else:
         j = krylov
     e1 = np.zeros(j + 1)
@@ -357,13 +366,14 @@ def gmres(matvec, b, krylov):
     return V[:j].T @ y, k
 
 
-def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, log=None, jvp=None):
+def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, log=None, jvp=None, forcing=None):
     """Jacobian-free Newton-Krylov on F(x) = g(x) - x: every Newton step
     solves (J_g - I) dx = -F(x) by ``krylov`` GMRES steps whose Jacobian-vector
     products are forward differences of g with step ``h`` along the unit
     direction (J_g v ~ |v| (g(x + h v/|v|) - g(x)) / h; g sees the same
     particles at every evaluation: common random numbers), or ``jvp(x, v)``
-    (J_g v at the last evaluated point x) when given.  A step that does
+    (J_g v at the last evaluated point x) when given; ``forcing`` stops
+    GMRES once its residual is that fraction of |F(x)|.  A step that does
     not lower max |F| is halved up to ``backtrack`` times (then the plain step
     x <- g(x) is taken).  Stops when max |F| < tol or after ``max_newton``
     steps.  ``g(x)`` returns (state, image).  Returns (state at the last
@@ -387,7 +397,7 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
             _, gp = g(x + (h / nrm) * v)
             return (gp - gx) * (nrm / h) - v
 
-        dx, k = gmres(matvec, -f, krylov)
+        dx, k = gmres(matvec, -f, krylov, forcing)
         if jvp is None:   # each difference JVP is one evaluation of g
             evals += k
         lam = 1.0
@@ -408,7 +418,7 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
 
 
 def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
-                  setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True):
+                  setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
     steps per Newton step, R replicas of K batches per map evaluation with
     common random numbers, traced in one push for K = 1), started from the
@@ -420,7 +430,10 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     over a batch of ``jvp_fraction`` times the particles (at 1 / jvp_fraction
     the charge each; the same batch for every JVP: common random numbers),
     with the paths' response to the field (``trajectory``)
-    or with the paths held fixed.  Same return dict as ``anderson``."""
+    or with the paths held fixed.  ``forcing`` ends a Newton step's GMRES
+    early once its residual is that fraction of the map's residual (None: all
+    ``krylov`` steps).
+    Same return dict as ``anderson``."""
     if jvp not in ("difference", "relaxed"):
         raise ValueError(f"jvp must be 'difference' or 'relaxed': {jvp!r}")
     if jvp == "relaxed" and not 0.0 < jvp_fraction <= 1.0:
@@ -463,7 +476,7 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     rho_g, x, evals = newton_krylov_iterate(
         g, x, tol, h_volts, krylov, max_newton,
         log=lambda k, res, e: log(f"  [newton-krylov] step {k}: max |g(x) - x| = {res:.4g} V after {e} evaluations"),
-        jvp=jvp_fn)
+        jvp=jvp_fn, forcing=forcing)
     rho = ss.to_rho(x, rho_g)
     solver_batches = ss.traced
     done, b2, steady, hist, rho_s = ss.plain_phase(rho, log)
@@ -476,7 +489,7 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
                 converged=done, batches=ss.traced,
                 phases=[dict(kind="secant", stages=[list(st) for st in start], evaluations_per_stage=evals_start,
                              batches=start_batches),
-                        dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov, jvp=jvp,
+                        dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov, forcing=forcing, jvp=jvp,
                              width=width if jac else None, trajectory=trajectory if jac else None,
                              relaxed_jvps=jac.jvps if jac else 0,
                              relaxed_push_ms=jac.push_ms if jac else 0.0,
