@@ -829,11 +829,19 @@ __device__ __forceinline__ int step32(const Grid &g, State32 &s, int &cell) {
     float ex = w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
     float ey = w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
     float sw = w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
-    float inv_sw = 1.0f / fmaxf(sw, 1e-12f);
+    // The hardware's one-ulp reciprocal and square root (this push is not
+    // bit-compatible anyway): |v| and |E| as square roots of sums of squares,
+    // which stay finite for any speed below 1e19 m/s and field below 1e19 V/m
+    // (hypotf otherwise); |a| = |q/m| |E| (|q/m| E can exceed the float range
+    // when squared).
+    float inv_sw = __builtin_amdgcn_rcpf(fmaxf(sw, 1e-12f));
     ex *= inv_sw;
     ey *= inv_sw;
     float ax = s.qm * ex, ay = s.qm * ey;
-    float dt = (float)g.cfl_dx / (hypotf(s.vx, s.vy) + sqrtf((float)g.two_dx * hypotf(ax, ay)) + 1.0f);
+    const float v2 = s.vx * s.vx + s.vy * s.vy, e2 = ex * ex + ey * ey;
+    const float hv = v2 < 1e38f ? __builtin_amdgcn_sqrtf(v2) : hypotf(s.vx, s.vy);
+    const float ha = fabsf(s.qm) * (e2 < 1e38f ? __builtin_amdgcn_sqrtf(e2) : hypotf(ex, ey));
+    float dt = (float)g.cfl_dx * __builtin_amdgcn_rcpf(hv + __builtin_amdgcn_sqrtf((float)g.two_dx * ha) + 1.0f);
     float vxn = s.vx + ax * dt, vyn = s.vy + ay * dt;
     const float x0 = s.x, y0 = s.y, q0 = s.q, r0 = s.r;
     float sx = 0.5f * (s.vx + vxn) * dt, sy = 0.5f * (s.vy + vyn) * dt;
