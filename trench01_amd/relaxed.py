@@ -24,6 +24,8 @@ trench_charging_2d.py:1237-1266), which ``ChargeUpdate`` restates on the host
 with its derivative -- in the surface-potential coordinates of the steady
 solve, for Jacobian-vector products J v of the map u -> u'.
 """
+from dataclasses import replace
+
 import numpy as np
 from scipy.linalg import lu_factor, lu_solve
 
@@ -149,11 +151,16 @@ class MapJacobian:
     batches of n / replicas particles each are averaged.  ``q`` is the charge
     per particle of those batches.  ``trajectory`` False leaves the field tangent out: the
     paths are held fixed and only the relaxed decisions respond (through the
-    potential step at the wall), which drops the path tangents' variance."""
+    potential step at the wall), which drops the path tangents' variance.
+    ``max_steps`` (below the map's own) ends the derivative's paths earlier:
+    the few particles still in flight then add nothing, and the push's
+    latency-bound tail -- the longest paths, traced one step after another --
+    shortens."""
 
-    def __init__(self, setup, dev, width, cut=30.0, trajectory=True):
+    def __init__(self, setup, dev, width, cut=30.0, trajectory=True, max_steps=None):
         s = setup
         self.s, self.dev, self.p = s, dev, s.p
+        self.p_push = s.p if max_steps is None else replace(s.p, max_steps=min(int(max_steps), s.p.max_steps))
         self.width, self.cut = float(width), float(cut)
         self.trajectory = bool(trajectory)
         m = s.geo.surface.size
@@ -184,7 +191,7 @@ class MapJacobian:
         self.dev.set_rho(rho)
         self.rho = np.asarray(rho, dtype=float).copy()
         self.n, self.q, self.R, self.offset = int(n), float(q), int(replicas), int(offset)
-        r = self.dev.tracer.soft(self.offset, self.n, self.p, self.width, self.cut)
+        r = self.dev.tracer.soft(self.offset, self.n, self.p_push, self.width, self.cut)
         self.push_ms += r["push_ms"]
         dep = r["deposit"] / self.R
         _, self.point = self.update.apply(self.rho, dep[0], dep[1], self.q)
@@ -207,7 +214,7 @@ class MapJacobian:
     def __call__(self, du):
         """J du: the derivative of the map's surface potentials along du."""
         _, drho, dphi, dex, dey = self.directional(np.asarray(du, dtype=float))
-        r = self.dev.tracer.soft(self.offset, self.n, self.p, self.width, self.cut,
+        r = self.dev.tracer.soft(self.offset, self.n, self.p_push, self.width, self.cut,
                                  dfield=(dex, dey) if self.trajectory else None, dphi=dphi)
         self.jvps += 1
         self.push_ms += r["push_ms"]

@@ -418,7 +418,8 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
 
 
 def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
-                  setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03):
+                  setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03,
+                  jvp_max_steps=1000):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
     steps per Newton step, R replicas of K batches per map evaluation with
     common random numbers, traced in one push for K = 1), started from the
@@ -432,7 +433,10 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     with the paths' response to the field (``trajectory``)
     or with the paths held fixed.  ``forcing`` ends a Newton step's GMRES
     early once its residual is that fraction of the map's residual (None: all
-    ``krylov`` steps).
+    ``krylov`` steps).  ``jvp_max_steps`` ends the relaxed JVPs' paths after
+    that many steps (None: the map's own limit): the few longer paths -- the
+    trapped particles, whose tangents grow along the path -- add variance
+    to the JVPs, and tracing them is the push's latency-bound tail.
     Same return dict as ``anderson``."""
     if jvp not in ("difference", "relaxed"):
         raise ValueError(f"jvp must be 'difference' or 'relaxed': {jvp!r}")
@@ -457,7 +461,7 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     jac = None
     if jvp == "relaxed":
         from .relaxed import MapJacobian
-        jac = MapJacobian(ss.s, ss.dev, width, trajectory=trajectory)
+        jac = MapJacobian(ss.s, ss.dev, width, trajectory=trajectory, max_steps=jvp_max_steps)
         n_jvp = max(1, int(round(p.n_per_batch * jvp_fraction)))   # per species
         jvp_offset = 2 * max(1, int(round(p.n_per_batch * ss.frac))) * R * K   # after the evaluations' particles
         ss.dev.reserve(jvp_offset + 2 * n_jvp)
@@ -491,6 +495,7 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
                              batches=start_batches),
                         dict(kind="newton-krylov", K=K, R=R, h_volts=h_volts, krylov=krylov, forcing=forcing, jvp=jvp,
                              width=width if jac else None, trajectory=trajectory if jac else None,
+                             jvp_max_steps=jvp_max_steps if jac else None,
                              relaxed_jvps=jac.jvps if jac else 0,
                              relaxed_push_ms=jac.push_ms if jac else 0.0,
                              evaluations=evals, batches=solver_batches - start_batches),
