@@ -187,7 +187,8 @@ def test_charge_update_restates_the_device_update():
 def test_map_jacobian_directions_and_linearity():
     """MapJacobian's potential tangent (Green's matrix) equals the LU solve of
     the same charge direction on the rows the push reads, and its JVPs are
-    linear in the direction."""
+    linear in the direction; with a lower step limit its deposit is the
+    relaxed push's with that limit."""
     _case("relaxed: map Jacobian")
     p, _ = configs.product_params("AR8", n_per_batch=20_000)
     setup = charging.ChargingSetup(p)
@@ -213,6 +214,15 @@ def test_map_jacobian_directions_and_linearity():
         ju, jv, jw = jac(du), jac(dv), jac(2.0 * du - 0.5 * dv)
         np.testing.assert_allclose(jw, 2.0 * ju - 0.5 * jv, rtol=0, atol=1e-9 * np.abs(jw).max())
         assert np.abs(ju - du).max() > 0
+        # paths ended early: the relaxed push of the same particles with that step limit
+        capped = relaxed.MapJacobian(setup, dev, 1.0, max_steps=300)
+        assert capped.p_push.max_steps == 300 and jac.p_push.max_steps == p.max_steps
+        dep_c = capped.at(rho, n, setup.q_macro)
+        ref = dev.tracer.soft(0, n, replace(p, max_steps=300), 1.0)
+        np.testing.assert_allclose(dep_c, ref["deposit"], rtol=1e-12, atol=0)
+        assert dep_c.sum() < jac.deposit.sum()
+        jc, jd = capped(du), capped(2.0 * du)
+        np.testing.assert_allclose(jd, 2.0 * jc, rtol=0, atol=1e-9 * np.abs(jd).max())
     finally:
         dev.close()
 
