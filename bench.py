@@ -21,9 +21,11 @@ FLOP at 69 per particle-step over its HIP-event time), and the CPU baseline
 same field).  The second half of the metric, time-to-steady at the same
 workload, is measured too (``--no-steady`` skips it): plain time-marching from
 zero charge to the reference's steady criterion, and the chosen method
-(trench01_amd.steady.anderson: a Krylov solve of the batch map's fixed point
+(trench01_amd.steady.newton_krylov: two Anderson stages, then a Newton step
+whose GMRES Jacobian-vector products come from the relaxed-barrier push, all
 with common random numbers, then plain batches to the same criterion), with
-the bottom-centre potential of each.
+the bottom-centre potential of each; trench01_amd.steady.anderson alone is
+measured beside it.
 """
 import argparse
 import json
@@ -531,16 +533,17 @@ def cpu_baseline(p, setup, parts, field, phi, threads, gpu=None):
 def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, seeds=(0, 1, 2, 3), long_run=300):
     """Time-to-steady from zero charge at this workload (device injection,
     1e6 particles per batch), per seed: plain time-marching until the
-    reference's criterion passes (W = 30, hold = 2), and the chosen method
-    (Anderson solve of the batch map's fixed point, then plain batches to the
-    same criterion), with the bottom-centre potential of each.  The first
-    seed's plain run is then continued ``long_run`` batches: the mean over
-    the last three quarters is the long-run time-marched value (the plain run
-    stops while still drifting up towards it).  The chosen method with the
-    precision variants (FP32 push, BF16 split Green's update, unless others
-    are given) runs on the first seed.  Newton-Krylov (GMRES-30) with the
-    relaxed-barrier JVPs runs on every seed, with difference JVPs on the
-    first.  Wall times include each method's host set-up."""
+    reference's criterion passes (W = 30, hold = 2), the chosen method
+    (Newton-Krylov on the batch map with Jacobian-vector products of the
+    relaxed-barrier push, from two Anderson stages, then plain batches to the
+    same criterion) and the Anderson solve alone, with the bottom-centre
+    potential of each.  The first seed's plain run is then continued
+    ``long_run`` batches: the mean over the last three quarters is the
+    long-run time-marched value (the plain run stops while still drifting up
+    towards it).  Newton-Krylov with difference JVPs, and the Anderson solve
+    with the precision variants (FP32 push, BF16 split Green's update, unless
+    others are given), run on the first seed.  Wall times include each
+    method's host set-up."""
     from trench01_amd import charging, steady
     per_seed, tails = [], []
     for seed in seeds:
@@ -560,14 +563,22 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
             tails.append(np.asarray(more["hist"]["bottom center"][nb + long_run // 4:]))
         log(f"time-to-steady seed {seed}, plain: converged={res['converged']} after {nb} batches in {wall:.1f}s, "
             f"bottom centre {plain['bottom_center_V']:.3f} V")
-        r = steady.anderson(ps)
+        r = steady.newton_krylov(ps)
         chosen = {"method": r["method"], "converged": bool(r["converged"]), "batches": r["batches"],
                   "wall_s": r["wall_s"], "setup_s": r["setup_s"], "phases": r["phases"],
                   "bottom_center_V": r["steady"]["bottom center"]}
-        log(f"time-to-steady seed {seed}, chosen: converged={r['converged']} after {r['batches']} batches in "
-            f"{r['wall_s']:.1f}s, bottom centre {chosen['bottom_center_V']:.3f} V")
+        log(f"time-to-steady seed {seed}, chosen (newton-krylov): converged={r['converged']} after {r['batches']} "
+            f"batches in {r['wall_s']:.1f}s, bottom centre {chosen['bottom_center_V']:.3f} V")
+        a = steady.anderson(ps)
+        anderson = {"method": a["method"], "converged": bool(a["converged"]), "batches": a["batches"],
+                    "wall_s": a["wall_s"], "setup_s": a["setup_s"], "phases": a["phases"],
+                    "bottom_center_V": a["steady"]["bottom center"], "speedup_vs_plain": wall / a["wall_s"],
+                    "bottom_center_diff_V": a["steady"]["bottom center"] - plain["bottom_center_V"]}
+        log(f"time-to-steady seed {seed}, anderson: converged={a['converged']} after {a['batches']} batches in "
+            f"{a['wall_s']:.1f}s, bottom centre {anderson['bottom_center_V']:.3f} V")
         per_seed.append({"seed": seed, "plain": plain, "chosen": chosen, "speedup": wall / r["wall_s"],
-                         "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"]})
+                         "bottom_center_diff_V": chosen["bottom_center_V"] - plain["bottom_center_V"],
+                         "anderson": anderson})
     out = {"seeds": per_seed, "plain": per_seed[0]["plain"], "chosen": per_seed[0]["chosen"],
            "speedup": per_seed[0]["speedup"], "bottom_center_diff_V": per_seed[0]["bottom_center_diff_V"],
            "speedup_min": min(e["speedup"] for e in per_seed),
@@ -585,27 +596,16 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
         for e in per_seed:
             e["chosen_minus_long_run_V"] = e["chosen"]["bottom_center_V"] - long["value_V"]
             e["plain_minus_long_run_V"] = e["plain"]["bottom_center_V"] - long["value_V"]
+            e["anderson"]["minus_long_run_V"] = e["anderson"]["bottom_center_V"] - long["value_V"]
         out["max_abs_diff_vs_long_run_V"] = max(abs(e["chosen_minus_long_run_V"]) for e in per_seed)
-    # Newton-Krylov on the batch map: JVPs of the relaxed-barrier map on every seed, differences on the first
-    nks = []
-    for e in per_seed:
-        ps = replace(p, seed=e["seed"], until_steady=True, max_batches=max_batches)
-        nk = steady.newton_krylov(ps)
-        v = nk["steady"]["bottom center"]
-        r = {"seed": e["seed"], "converged": bool(nk["converged"]), "batches": nk["batches"], "wall_s": nk["wall_s"],
-             "phases": nk["phases"], "bottom_center_V": v, "speedup_vs_plain": e["plain"]["wall_s"] / nk["wall_s"],
-             "bottom_center_diff_V": v - e["plain"]["bottom_center_V"]}
-        if "long_run_time_marched" in out:
-            r["minus_long_run_V"] = v - out["long_run_time_marched"]["value_V"]
-        nks.append(r)
-        log(f"time-to-steady, seed {e['seed']}, newton-krylov (relaxed JVPs): {nk['wall_s']:.1f}s, bottom centre "
-            f"{v:.3f} V")
-    out["newton_krylov"] = {
-        "method": nk["method"], "per_seed": nks, "worst_speedup": min(r["speedup_vs_plain"] for r in nks),
-        "max_abs_diff_vs_plain_V": max(abs(r["bottom_center_diff_V"]) for r in nks),
-        "max_abs_diff_vs_long_run_V": (max(abs(r["minus_long_run_V"]) for r in nks)
+    an = [e["anderson"] for e in per_seed]
+    out["anderson"] = {
+        "method": an[0]["method"], "worst_speedup": min(r["speedup_vs_plain"] for r in an),
+        "mean_speedup": float(np.mean([r["speedup_vs_plain"] for r in an])),
+        "max_abs_diff_vs_plain_V": max(abs(r["bottom_center_diff_V"]) for r in an),
+        "max_abs_diff_vs_long_run_V": (max(abs(r["minus_long_run_V"]) for r in an)
                                        if "long_run_time_marched" in out else None),
-        "all_converged": all(r["converged"] for r in nks)}
+        "all_converged": all(r["converged"] for r in an)}
     ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
     nk = steady.newton_krylov(ps, R=8, jvp="difference")
     out["newton_krylov_difference"] = {
@@ -620,14 +620,14 @@ def time_to_steady(p, max_batches, log, push_precision=0, green_precision=0, see
         push_precision, green_precision = 1, 1
     ps = replace(p, seed=seeds[0], until_steady=True, max_batches=max_batches)
     rv = steady.anderson(ps, push_precision=push_precision, green_precision=green_precision)
-    plain0, chosen0 = per_seed[0]["plain"], per_seed[0]["chosen"]
-    out["chosen_precision_variant"] = {
+    plain0, anderson0 = per_seed[0]["plain"], per_seed[0]["anderson"]
+    out["anderson_precision_variant"] = {
         "seed": seeds[0], "push": "FP32" if push_precision else "FP64", "green": GREEN_NAMES[green_precision],
         "converged": bool(rv["converged"]), "batches": rv["batches"], "wall_s": rv["wall_s"],
         "speedup_vs_plain": plain0["wall_s"] / rv["wall_s"], "bottom_center_V": rv["steady"]["bottom center"],
         "drift_vs_plain_fp64_V": rv["steady"]["bottom center"] - plain0["bottom_center_V"],
-        "drift_vs_chosen_fp64_V": rv["steady"]["bottom center"] - chosen0["bottom_center_V"]}
-    log(f"time-to-steady, chosen with precision variants: {rv['wall_s']:.1f}s, bottom centre "
+        "drift_vs_anderson_fp64_V": rv["steady"]["bottom center"] - anderson0["bottom_center_V"]}
+    log(f"time-to-steady, anderson with precision variants: {rv['wall_s']:.1f}s, bottom centre "
         f"{rv['steady']['bottom center']:.3f} V")
     return out
 

@@ -87,8 +87,9 @@ def time_marched():
 
 @pytest.mark.parametrize("jvp", ["difference", "relaxed"])
 def test_newton_krylov_reaches_the_time_marched_state(time_marched, jvp):
-    """Newton-Krylov on the batch map (GMRES from the coarse Anderson start)
-    followed by plain batches to the reference criterion: it converges,
+    """Newton-Krylov on the batch map (one GMRES Newton step from the first
+    two Anderson stages) followed by plain batches to the reference
+    criterion: it converges,
     within 1 V of the time-marched bottom-centre potential, with
     Jacobian-vector products by differences over merged replicas or from the
     relaxed-barrier push (relaxed.MapJacobian).  Speed and accuracy are
@@ -97,7 +98,7 @@ def test_newton_krylov_reaches_the_time_marched_state(time_marched, jvp):
     descend, with a plain phase whose length varies between seeds."""
     print(f"[gpu case] steady AR8 (1.25e5 per species) newton-krylov, {jvp} JVPs", flush=True)
     p, nb, long_run = time_marched
-    nk = steady.newton_krylov(p, R=8, krylov=10, max_newton=1 if jvp == "difference" else 2, jvp=jvp,
+    nk = steady.newton_krylov(p, R=8, krylov=10, jvp=jvp,
                               forcing=None if jvp == "difference" else 0.03)
     v = nk["steady"]["bottom center"]
     print(f"plain {nb} batches, 300 on {long_run:.3f} V; newton-krylov {nk['batches']} batches {nk['phases']}, "

@@ -417,13 +417,13 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
     return state, gx, evals
 
 
-def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, start=None, log=None, device=0,
+def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=1, tol=None, start=None, log=None, device=0,
                   setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03,
                   jvp_max_steps=1000):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
     steps per Newton step, R replicas of K batches per map evaluation with
     common random numbers, traced in one push for K = 1), started from the
-    Anderson stages ``start`` (default: the coarse first default stage), then
+    Anderson stages ``start`` (default: the first two default stages), then
     plain batches to the reference criterion.  Jacobian-vector products:
     ``jvp`` = "difference", forward differences of step ``h_volts`` on the
     surface potentials; "relaxed", the derivative of the map with the barrier
@@ -443,7 +443,7 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     if jvp == "relaxed" and not 0.0 < jvp_fraction <= 1.0:
         raise ValueError("jvp_fraction must lie in (0, 1]")
     log = log or (lambda s: None)
-    start = default_stages(p)[:1] if start is None else start
+    start = default_stages(p)[:2] if start is None else start
     noise = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
     tol = 0.002 * noise if tol is None else tol
     ss = _Session(p, device, setup, 0, 0, max([R] + [st[1] for st in start]))
@@ -488,8 +488,8 @@ def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=2, tol=None, 
     ss.close()
     how = ("forward-difference JVPs over merged replicas" if jvp == "difference" else
            f"JVPs of the relaxed-barrier map (width {width} V) over {jvp_fraction:g} of a batch")
-    return dict(method=f"Newton-Krylov (GMRES) on the batch map (common random numbers, {how}) from a coarse "
-                       "Anderson start, then plain batches to the reference criterion",
+    return dict(method=f"Newton-Krylov (GMRES) on the batch map (common random numbers, {how}) from "
+                       f"{len(start)} Anderson stage(s), then plain batches to the reference criterion",
                 converged=done, batches=ss.traced,
                 phases=[dict(kind="secant", stages=[list(st) for st in start], evaluations_per_stage=evals_start,
                              batches=start_batches),
