@@ -417,7 +417,7 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
     return state, gx, evals
 
 
-def newton_krylov(p, K=1, R=24, h_volts=2.0, krylov=30, max_newton=1, tol=None, start=None, log=None, device=0,
+def newton_krylov(p, K=1, R=16, h_volts=2.0, krylov=30, max_newton=1, tol=None, start=None, log=None, device=0,
                   setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03,
                   jvp_max_steps=1000):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
