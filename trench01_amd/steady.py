@@ -419,7 +419,7 @@ def newton_krylov_iterate(g, x, tol, h, krylov=30, max_newton=6, backtrack=4, lo
 
 def newton_krylov(p, K=1, R=16, h_volts=2.0, krylov=30, max_newton=1, tol=None, start=None, log=None, device=0,
                   setup=None, jvp="relaxed", width=1.0, jvp_fraction=0.25, trajectory=True, forcing=0.03,
-                  jvp_max_steps=1000):
+                  jvp_max_steps=1000, push_precision=0, green_precision=0):
     """Steady state by Newton-Krylov on the batch map (GMRES, ``krylov``
     steps per Newton step, R replicas of K batches per map evaluation with
     common random numbers, traced in one push for K = 1), started from the
@@ -437,6 +437,9 @@ def newton_krylov(p, K=1, R=16, h_volts=2.0, krylov=30, max_newton=1, tol=None, 
     that many steps (None: the map's own limit): the few longer paths -- the
     trapped particles, whose tangents grow along the path -- add variance
     to the JVPs, and tracing them is the push's latency-bound tail.
+    ``push_precision`` / ``green_precision`` select the map's push and
+    Green's-update precision variants (as in ``anderson``); the relaxed JVP
+    push is FP64 either way.
     Same return dict as ``anderson``."""
     if jvp not in ("difference", "relaxed"):
         raise ValueError(f"jvp must be 'difference' or 'relaxed': {jvp!r}")
@@ -446,7 +449,7 @@ def newton_krylov(p, K=1, R=16, h_volts=2.0, krylov=30, max_newton=1, tol=None, 
     start = default_stages(p)[:2] if start is None else start
     noise = max(1.0, float(np.sqrt(DEFAULT_STAGES_N / max(1, p.n_per_batch))))
     tol = 0.002 * noise if tol is None else tol
-    ss = _Session(p, device, setup, 0, 0, max([R] + [st[1] for st in start]))
+    ss = _Session(p, device, setup, push_precision, green_precision, max([R] + [st[1] for st in start]))
     rho, x, evals_start, first = _secant_stages(ss, start, 10, 40, 10, log)
     start_batches = ss.traced
     box = {"like": rho}
